@@ -1,0 +1,258 @@
+/*
+ * samrai_fac.h -- C ABI of the MI355X FAC Poisson library (libsamrai_fac.so).
+ *
+ * Two seams of the reference are replaced:
+ *
+ *  1. The per-patch Fortran kernels that CellPoissonFACOps,
+ *     CartesianRobinBcHelper and the geom/pdat transfer operators call
+ *     through SAMRAI_F77_FUNC.  Each sfac_<name> below has the argument order
+ *     and meaning of the reference routine <name>_ (scalars by value, arrays
+ *     as DEVICE pointers into pdat::ArrayData storage) plus the stream to
+ *     launch on, and returns 0 on success.
+ *
+ *  2. The FAC solver itself: solv::CellPoissonFACSolver / FACPreconditioner /
+ *     CellPoissonFACOps (the FACOperatorStrategy plug-in), over a static
+ *     hier::PatchHierarchy whose patch data (pdat::CellData / SideData) live
+ *     in device memory in ArrayData layout.
+ *
+ * Conventions: every entry point returns an int status, 0 = OK and < 0 =
+ * error, with the message in sfac_last_error() (thread-local).  No C++
+ * exception crosses the ABI.  Calls on one solver are stream-ordered on that
+ * solver's stream and not re-entrant.  Boxes are int[2*dim] = {lo..., hi...}
+ * in the level's own index space.
+ */
+#ifndef SAMRAI_FAC_H
+#define SAMRAI_FAC_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFAC_VERSION 1
+
+int sfac_version(void);
+/* Message of the last failed call on this thread. */
+const char* sfac_last_error(void);
+
+/* ======================================================================
+ * 1. Per-patch kernels (device pointers).
+ * ==================================================================== */
+
+/* compfluxvardc3d_: solv/fortran/solv_cellpoissonfacops3d.f:10-74,
+ * called from CellPoissonFACOps::computeFluxOnPatch (CellPoissonFACOps.C:2156) */
+int sfac_compfluxvardc3d(double* xflux, double* yflux, double* zflux, int fgi, int fgj,
+                         int fgk, const double* xdc, const double* ydc, const double* zdc,
+                         int dgi, int dgj, int dgk, const double* soln, int sgi, int sgj,
+                         int sgk, int ifirst, int ilast, int jfirst, int jlast, int kfirst,
+                         int klast, const double* dx, hipStream_t stream);
+/* compfluxcondc3d_: solv_cellpoissonfacops3d.f:77-133 */
+int sfac_compfluxcondc3d(double* xflux, double* yflux, double* zflux, int fgi, int fgj,
+                         int fgk, double dc, const double* soln, int sgi, int sgj, int sgk,
+                         int ifirst, int ilast, int jfirst, int jlast, int kfirst, int klast,
+                         const double* dx, hipStream_t stream);
+/* rbgswithfluxmax{vardc,condc}{varsf,consf}3d_: solv_cellpoissonfacops3d.f:136-481,
+ * called from CellPoissonFACOps::redOrBlackSmoothingOnPatch (.C:2433-2777).
+ * xdc == NULL selects constant dc; sf == NULL selects constant sfc.
+ * maxres is a device double (max |residual| of the colour). */
+int sfac_rbgswithfluxmax3d(const double* xflux, const double* yflux, const double* zflux,
+                           int fgi, int fgj, int fgk, const double* xdc, const double* ydc,
+                           const double* zdc, int dgi, int dgj, int dgk, double dc,
+                           const double* rhs, int rgi, int rgj, int rgk, const double* sf,
+                           int sfgi, int sfgj, int sfgk, double sfc, double* soln, int sgi,
+                           int sgj, int sgk, int ifirst, int ilast, int jfirst, int jlast,
+                           int kfirst, int klast, const double* dx, int offset,
+                           double* maxres, hipStream_t stream);
+/* compres{var,con}sca3d_: solv_cellpoissonfacops3d.f:485-600 (sf == NULL: constant) */
+int sfac_compres3d(const double* xflux, const double* yflux, const double* zflux, int fgi,
+                   int fgj, int fgk, const double* rhs, int rgi, int rgj, int rgk,
+                   double* residual, const double* sf, int sfgi, int sfgj, int sfgk,
+                   double sfc, const double* soln, int sgi, int sgj, int sgk, int ifirst,
+                   int ilast, int jfirst, int jlast, int kfirst, int klast, const double* dx,
+                   hipStream_t stream);
+/* ewingfixflux{var,con}dc3d_: solv_cellpoissonfacops3d.f:604-1011,
+ * called from CellPoissonFACOps::ewingFixFlux (.C:1523-1671); xdc == NULL: constant */
+int sfac_ewingfixflux3d(double* xflux, double* yflux, double* zflux, int fgi, int fgj,
+                        int fgk, const double* xdc, const double* ydc, const double* zdc,
+                        int dgi, int dgj, int dgk, double dc, const double* soln, int sgi,
+                        int sgj, int sgk, int ifirst, int ilast, int jfirst, int jlast,
+                        int kfirst, int klast, int location_index, const int* ratio,
+                        const int* blower, const int* bupper, const double* dx,
+                        hipStream_t stream);
+/* 2-D twins: solv/fortran/solv_cellpoissonfacops2d.f:10-686 */
+int sfac_compfluxvardc2d(double* xflux, double* yflux, int fgi, int fgj, const double* xdc,
+                         const double* ydc, int dgi, int dgj, const double* soln, int sgi,
+                         int sgj, int ifirst, int ilast, int jfirst, int jlast,
+                         const double* dx, hipStream_t stream);
+int sfac_compfluxcondc2d(double* xflux, double* yflux, int fgi, int fgj, double dc,
+                         const double* soln, int sgi, int sgj, int ifirst, int ilast,
+                         int jfirst, int jlast, const double* dx, hipStream_t stream);
+int sfac_rbgswithfluxmax2d(const double* xflux, const double* yflux, int fgi, int fgj,
+                           const double* xdc, const double* ydc, int dgi, int dgj, double dc,
+                           const double* rhs, int rgi, int rgj, const double* sf, int sfgi,
+                           int sfgj, double sfc, double* soln, int sgi, int sgj, int ifirst,
+                           int ilast, int jfirst, int jlast, const double* dx, int offset,
+                           double* maxres, hipStream_t stream);
+int sfac_compres2d(const double* xflux, const double* yflux, int fgi, int fgj,
+                   const double* rhs, int rgi, int rgj, double* residual, const double* sf,
+                   int sfgi, int sfgj, double sfc, const double* soln, int sgi, int sgj,
+                   int ifirst, int ilast, int jfirst, int jlast, const double* dx,
+                   hipStream_t stream);
+int sfac_ewingfixflux2d(double* xflux, double* yflux, int fgi, int fgj, const double* xdc,
+                        const double* ydc, int dgi, int dgj, double dc, const double* soln,
+                        int sgi, int sgj, int ifirst, int ilast, int jfirst, int jlast,
+                        int location_index, const int* ratio, const int* blower,
+                        const int* bupper, const double* dx, hipStream_t stream);
+
+/* settype1cells{2d,3d}_, settype2cells{2d,3d}_, settype3cells3d_:
+ * solv/fortran/solv_cartesianrobinbchelper{2d,3d}.f, called from
+ * CartesianRobinBcHelper::setBoundaryValuesInCells (CartesianRobinBcHelper.C:125-593). */
+int sfac_settype1cells3d(double* data, int dil, int dih, int djl, int djh, int dkl, int dkh,
+                         const double* a, const double* b, const double* g, int il, int ih,
+                         int jl, int jh, int kl, int kh, int ibeg, int iend, int jbeg,
+                         int jend, int kbeg, int kend, int face, int ghos, int inte,
+                         int location, double h, int zerog, hipStream_t stream);
+int sfac_settype1cells2d(double* data, int dil, int dih, int djl, int djh, const double* a,
+                         const double* b, const double* g, int il, int ih, int jl, int jh,
+                         int ibeg, int iend, int jbeg, int jend, int face, int ghos, int inte,
+                         int location, double h, int zerog, hipStream_t stream);
+int sfac_settype2cells3d(double* data, int dil, int dih, int djl, int djh, int dkl, int dkh,
+                         const int* lower, const int* upper, int location, hipStream_t stream);
+int sfac_settype2cells2d(double* data, int dil, int dih, int djl, int djh, const int* lower,
+                         const int* upper, int location, hipStream_t stream);
+int sfac_settype3cells3d(double* data, int dil, int dih, int djl, int djh, int dkl, int dkh,
+                         const int* lower, const int* upper, int location, hipStream_t stream);
+
+/* cartwgtavgcelldoub{2d,3d}_: geom/fortran/geom_m4cartcoarsenops3d.i:31-78, called from
+ * CartesianCellDoubleWeightedAverage::coarsen (geom/CartesianCellDoubleWeightedAverage.C:96-268) */
+int sfac_cartwgtavgcelldoub(int dim, const int* ifirstc, const int* ilastc, const int* filo,
+                            const int* fihi, const int* cilo, const int* cihi,
+                            const int* ratio, const double* dxf, const double* dxc,
+                            const double* arrayf, double* arrayc, hipStream_t stream);
+/* cartwgtavgoutsidedoub{2d,3d}{0,1,2}_: geom_m4cartcoarsenops3d.i:343-424, called from
+ * CartesianOutersideDoubleWeightedAverage::coarsen (.C:117-420); axis = face normal,
+ * arrays are one side's outerside slab */
+int sfac_cartwgtavgoutsidedoub(int dim, int axis, const int* ifirstc, const int* ilastc,
+                               const int* filo, const int* fihi, const int* cilo,
+                               const int* cihi, const int* ratio, const double* dxf,
+                               const double* dxc, const double* arrayf, double* arrayc,
+                               hipStream_t stream);
+/* cartclinrefcelldoub{2d,3d}_: geom_m4cartrefineops3d.i:118-167, called from
+ * CartesianCellConservativeLinearRefine<double>::refine (.C:71-337) */
+int sfac_cartclinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
+                             const int* ifirstf, const int* ilastf, const int* cilo,
+                             const int* cihi, const int* filo, const int* fihi,
+                             const int* ratio, const double* dxc, const double* dxf,
+                             const double* arrayc, double* arrayf, hipStream_t stream);
+/* conrefcelldoub{2d,3d}_: pdat/fortran/pdat_m4conrefineops3d.i:33-..., called from
+ * pdat::CellConstantRefine<double>::refine (pdat/CellConstantRefine.C:32-148) */
+int sfac_conrefcelldoub(int dim, const int* ifirstf, const int* ilastf, const int* cilo,
+                        const int* cihi, const int* filo, const int* fihi, const int* ratio,
+                        const double* arrayc, double* arrayf, hipStream_t stream);
+
+/* ======================================================================
+ * 2. Hierarchy, patch data and the FAC solver.
+ * ==================================================================== */
+
+typedef struct sfac_hierarchy sfac_hierarchy;
+typedef struct sfac_patch_data sfac_patch_data;
+typedef struct sfac_solver sfac_solver;
+
+/* hier::PatchHierarchy with a CartesianGridGeometry over [xlo, xhi] and the
+ * physical domain box [domain_lo, domain_hi] of level 0
+ * (hier/PatchHierarchy.h, geom/CartesianGridGeometry.h). */
+int sfac_hierarchy_create(int dim, const int* domain_lo, const int* domain_hi,
+                          const double* xlo, const double* xhi, sfac_hierarchy** out);
+/* PatchHierarchy::makeNewPatchLevel: append a level refined by `ratio` from the
+ * previous one (ignored for level 0). */
+int sfac_hierarchy_add_level(sfac_hierarchy* h, int ratio, int npatches, const int* boxes);
+/* Validates alignment and proper nesting; builds the device descriptor tables. */
+int sfac_hierarchy_finalize(sfac_hierarchy* h);
+int sfac_hierarchy_num_patches(const sfac_hierarchy* h, int ln);
+int sfac_hierarchy_destroy(sfac_hierarchy* h);
+
+/* pdat::CellData<double> / SideData<double> on every patch, device memory,
+ * ArrayData layout (pdat/ArrayData.C:81-193). */
+int sfac_cell_data_create(sfac_hierarchy* h, int ghost, sfac_patch_data** out);
+int sfac_side_data_create(sfac_hierarchy* h, int ghost, sfac_patch_data** out);
+int sfac_patch_data_destroy(sfac_patch_data* d);
+/* ArrayData::getPointer(axis) of patch p on level ln, and its element count. */
+int sfac_patch_data_pointer(sfac_patch_data* d, int ln, int p, int axis, double** ptr,
+                            long* count);
+/* Synchronous host copies of one patch array (whole ghost box). */
+int sfac_patch_data_put(sfac_patch_data* d, int ln, int p, int axis, const double* host);
+int sfac_patch_data_get(sfac_patch_data* d, int ln, int p, int axis, double* host);
+int sfac_patch_data_fill(sfac_patch_data* d, double value);
+
+/* Input keys of fac_precond (FACPreconditioner.C:79-114) and fac_ops
+ * (CellPoissonFACOps.C:691-739). */
+typedef struct {
+  int max_cycles;
+  double residual_tol;
+  double relative_residual_tol;
+  int num_pre_sweeps;
+  int num_post_sweeps;
+  const char* coarse_solver_choice;  /* "redblack" | "jacobi" */
+  double coarse_solver_tolerance;
+  int coarse_solver_max_iterations;
+  const char* cf_discretization;     /* "Ewing" */
+  const char* prolongation_method;   /* "CONSERVATIVE_LINEAR_REFINE" | "CONSTANT_REFINE" */
+  int enable_logging;
+} sfac_solver_params;
+
+/* The reference defaults. */
+void sfac_solver_params_default(sfac_solver_params* p);
+
+/* CellPoissonFACSolver(dim, name, fac_precond, fac_ops, db): solv/CellPoissonFACSolver.h:134 */
+int sfac_solver_create(sfac_hierarchy* h, const sfac_solver_params* p, sfac_solver** out);
+int sfac_solver_destroy(sfac_solver* s);
+/* setDConstant / setDPatchDataId / setCConstant / setCPatchDataId
+ * (CellPoissonFACSolver.h:227-290) */
+int sfac_solver_set_d_constant(sfac_solver* s, double d);
+int sfac_solver_set_d_patch_data(sfac_solver* s, sfac_patch_data* side_data);
+int sfac_solver_set_c_constant(sfac_solver* s, double c);
+int sfac_solver_set_c_patch_data(sfac_solver* s, sfac_patch_data* cell_data);
+/* setBcObject with a LocationIndexRobinBcCoefs: a u + b du/dn = g per
+ * location index 0..2*dim-1 (solv/LocationIndexRobinBcCoefs.C:75-160). */
+int sfac_solver_set_boundaries(sfac_solver* s, const double* a, const double* b,
+                               const double* g);
+/* initializeSolverState(u, f, hierarchy, coarse_level, fine_level): .h:310-330 */
+int sfac_solver_initialize(sfac_solver* s, sfac_patch_data* u, sfac_patch_data* f,
+                           int coarsest_ln, int finest_ln);
+int sfac_solver_deallocate(sfac_solver* s);
+/* solveSystem(u, f): .h:182-188; *converged as the bool it returns */
+int sfac_solver_solve(sfac_solver* s, sfac_patch_data* u, sfac_patch_data* f, int* converged);
+/* getNumberOfIterations / getResidualNorm / getConvergenceFactors: .h:395-440 */
+int sfac_solver_get_iterations(sfac_solver* s, int* n);
+int sfac_solver_get_residual_norm(sfac_solver* s, double* r);
+int sfac_solver_get_convergence_factors(sfac_solver* s, double* avg, double* final_factor);
+/* Residual norm after each cycle of the last solve. */
+int sfac_solver_get_history(sfac_solver* s, double* hist, int max, int* n);
+int sfac_solver_last_solve_seconds(sfac_solver* s, double* seconds);
+int sfac_solver_synchronize(sfac_solver* s);
+
+/* FACOperatorStrategy operations of the GPU CellPoissonFACOps
+ * (solv/FACOperatorStrategy.h:82-364), valid after sfac_solver_initialize.
+ * which: 0 error, 1 residual, 2 temporary residual, 3 weights (borrowed). */
+int sfac_ops_internal_vector(sfac_solver* s, int which, sfac_patch_data** out);
+/* xeqScheduleGhostFill (with_coarse) / xeqScheduleGhostFillNoCoarse */
+int sfac_ops_ghost_fill(sfac_solver* s, sfac_patch_data* x, int ln, int with_coarse,
+                        int homogeneous);
+/* smoothErrorByRedBlack(e, r, ln, sweeps, tol) (tol < 0: no early exit) */
+int sfac_ops_smooth(sfac_solver* s, sfac_patch_data* e, sfac_patch_data* r, int ln,
+                    int sweeps, double tol, int* sweeps_done, double* maxres);
+/* computeCompositeResidualOnLevel(r, u, f, ln, error_equation_indicator) */
+int sfac_ops_residual(sfac_solver* s, sfac_patch_data* r, sfac_patch_data* u,
+                      sfac_patch_data* f, int ln, int homogeneous);
+int sfac_ops_restrict_solution(sfac_solver* s, sfac_patch_data* u, int dest_ln);
+int sfac_ops_restrict_residual(sfac_solver* s, sfac_patch_data* src, sfac_patch_data* dst,
+                               int dest_ln);
+int sfac_ops_prolong_and_correct(sfac_solver* s, sfac_patch_data* e, int dest_ln);
+/* computeResidualNorm: weighted RMS over the initialized level range */
+int sfac_ops_rms_norm(sfac_solver* s, sfac_patch_data* x, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SAMRAI_FAC_H */

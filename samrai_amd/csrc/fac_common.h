@@ -1,0 +1,57 @@
+// Shared host/device definitions for the MI355X FAC Poisson library.
+//
+// Array layout is pdat::ArrayData's: one double array per patch and
+// component, column-major over the ghost box with i fastest
+// (source/SAMRAI/pdat/ArrayData.C:170).  A side-centred component along
+// axis a has one more entry along a.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sfac {
+
+constexpr int kMaxDim = 3;
+
+struct Box {
+  int lo[3];
+  int hi[3];
+};
+
+// Strides of an array over box [lo-g, hi+g] (+1 along `extra_axis`).
+struct View {
+  int l0, l1, l2;  // lower corner of the array box
+  int s1;          // stride of j
+  int s2;          // stride of k
+  __host__ __device__ inline long at(int i, int j, int k) const {
+    return (long)(i - l0) + (long)s1 * (j - l1) + (long)s2 * (k - l2);
+  }
+};
+
+__host__ __device__ inline View make_view(const int* lo, const int* hi, int g, int dim,
+                                          int extra_axis = -1) {
+  View v;
+  int n0 = hi[0] - lo[0] + 1 + 2 * g + (extra_axis == 0 ? 1 : 0);
+  int n1 = hi[1] - lo[1] + 1 + 2 * g + (extra_axis == 1 ? 1 : 0);
+  v.l0 = lo[0] - g;
+  v.l1 = lo[1] - g;
+  v.l2 = dim == 3 ? lo[2] - g : 0;
+  v.s1 = n0;
+  v.s2 = n0 * n1;
+  return v;
+}
+
+// Fortran x-(x/2*2): truncating remainder (negative for negative odd x).
+__host__ __device__ inline int fmod2(int x) { return x - (x / 2) * 2; }
+
+// SAMRAI's index coarsening (floor division), geom_m4cartopstuff.i:8-14.
+__host__ __device__ inline int cdiv(int a, int r) { return a < 0 ? (a + 1) / r - 1 : a / r; }
+
+// Ewing weights: the reference evaluates 1.0/(1+r) in default REAL (single
+// precision), solv_cellpoissonfacops3d.f:650.
+__host__ __device__ inline double ewing_tranwt(int r) {
+  return (double)(1.0f / (float)(1 + r));
+}
+
+}  // namespace sfac

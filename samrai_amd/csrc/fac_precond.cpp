@@ -1,0 +1,205 @@
+// FACPreconditioner and CellPoissonFACSolver (host control flow).
+//
+// FACPreconditioner::solveSystem / facCycle_Recursive /
+// computeFullCompositeResidual restate solv/FACPreconditioner.C:276-518 and
+// 774-814; CellPoissonFACSolver restates the set-up and solve wrappers of
+// solv/CellPoissonFACSolver.C:175-400.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+
+#include "fac_solver.hpp"
+
+namespace samrai_amd {
+namespace solv {
+
+using pdat::PatchDataSet;
+
+FACPreconditioner::FACPreconditioner(FACOperatorStrategy& ops, CellPoissonFACOps* gpu_ops)
+    : d_ops(ops), d_gpu(gpu_ops) {}
+
+FACPreconditioner::~FACPreconditioner() { deallocateSolverState(); }
+
+void FACPreconditioner::setMaxCycles(int n) {
+  if (!(n >= 1)) fail("FACPreconditioner: max_cycles out of range");
+  d_max_cycles = n;
+}
+void FACPreconditioner::setResidualTolerance(double t) {
+  if (!(t > 0.0)) fail("FACPreconditioner: residual_tol out of range");
+  d_residual_tol = t;
+}
+void FACPreconditioner::setRelativeResidualTolerance(double t) {
+  if (!(t > 0.0 || t == -1)) fail("FACPreconditioner: relative_residual_tol out of range");
+  d_rel_tol = t;
+}
+void FACPreconditioner::setAlgorithmChoice(const std::string& c) {
+  if (c != "default") fail("FACPreconditioner: algorithm '" + c + "' not available");
+}
+
+void FACPreconditioner::getConvergenceFactors(double& avg, double& fin) const {
+  avg = d_avg_factor;
+  fin = d_iterations > 0 ? d_factors[d_iterations - 1] : 0.0;
+}
+
+void FACPreconditioner::initializeSolverState(const SAMRAIVectorReal& u,
+                                              const SAMRAIVectorReal& f) {
+  deallocateSolverState();
+  d_lmin = u.coarsest;
+  d_lmax = u.finest;
+  const hier::PatchHierarchy& h = u.data->hierarchy();
+  if (d_lmin < 0 || d_lmin > d_lmax || d_lmax >= h.numberOfLevels())
+    fail("FACPreconditioner: bad level range");
+  // error clones u, residual and its temporary clone f (FACPreconditioner.C:157-229)
+  d_e_data.reset(new PatchDataSet(h, PatchDataSet::kCell, u.data->ghost()));
+  d_r_data.reset(new PatchDataSet(h, PatchDataSet::kCell, f.data->ghost()));
+  d_tr_data.reset(new PatchDataSet(h, PatchDataSet::kCell, f.data->ghost()));
+  d_e = {d_e_data.get(), u.weights, d_lmin, d_lmax};
+  d_r = {d_r_data.get(), f.weights, d_lmin, d_lmax};
+  d_tr = {d_tr_data.get(), f.weights, d_lmin, d_lmax};
+  d_ops.initializeOperatorState(u, f);
+  d_initialized = true;
+}
+
+void FACPreconditioner::deallocateSolverState() {
+  if (d_initialized) d_ops.deallocateOperatorState();
+  d_e_data.reset();
+  d_r_data.reset();
+  d_tr_data.reset();
+  d_initialized = false;
+}
+
+double FACPreconditioner::computeFullCompositeResidual(SAMRAIVectorReal& r, SAMRAIVectorReal& u,
+                                                       SAMRAIVectorReal& f) {
+  d_ops.computeCompositeResidualOnLevel(r, u, f, d_lmax, false);
+  for (int ln = d_lmax - 1; ln >= d_lmin; --ln) {
+    d_ops.restrictSolution(u, u, ln);
+    d_ops.computeCompositeResidualOnLevel(r, u, f, ln, false);
+    d_ops.restrictResidual(r, r, ln);
+  }
+  return d_ops.computeResidualNorm(r, d_lmax, d_lmin);
+}
+
+void FACPreconditioner::facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal& r,
+                                           SAMRAIVectorReal& u, int lmax, int lmin, int ln) {
+  if (ln == lmin) {
+    d_ops.solveCoarsestLevel(e, r, ln);
+    return;
+  }
+  d_ops.smoothError(e, r, ln, d_pre);
+  d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);
+  d_ops.restrictResidual(d_tr, r, ln - 1);
+  facCycle_Recursive(e, r, u, lmax, lmin, ln - 1);
+  d_ops.prolongErrorAndCorrect(e, e, ln);
+  d_ops.smoothError(e, r, ln, d_post);
+}
+
+void FACPreconditioner::runOneCycle(SAMRAIVectorReal& u) {
+  d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
+  facCycle_Recursive(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
+}
+
+bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
+  bool clear_when_done = false;
+  if (!d_initialized) {
+    clear_when_done = true;
+    initializeSolverState(u, f);
+  }
+  auto t0 = std::chrono::steady_clock::now();
+  d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
+  d_gpu->setToScalar(*d_r.data, 0.0, d_lmin, d_lmax);
+  const double initial = d_residual_norm = computeFullCompositeResidual(d_r, u, f);
+  double eff = d_residual_tol;
+  if (d_rel_tol >= 0) {
+    double tmp = d_ops.computeResidualNorm(f, d_lmax, d_lmin) * d_rel_tol;
+    if (eff < tmp) eff = tmp;
+  }
+  if ((int)d_factors.size() < d_max_cycles) d_factors.resize(d_max_cycles);
+  d_history.clear();
+  d_iterations = 0;
+  do {
+    d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
+    facCycle_Recursive(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
+    d_gpu->add(*u.data, *d_e.data, d_lmin, d_lmax);
+    for (int ln = d_lmax - 1; ln >= d_lmin; --ln) d_ops.restrictSolution(u, u, ln);
+    d_factors[d_iterations] = d_residual_norm;
+    d_residual_norm = computeFullCompositeResidual(d_r, u, f);
+    d_history.push_back(d_residual_norm);
+    if (d_factors[d_iterations] != 0)
+      d_factors[d_iterations] = d_residual_norm / d_factors[d_iterations];
+    else
+      d_factors[d_iterations] = 0;
+    ++d_iterations;
+    d_net_factor = d_residual_norm / (initial + 1e-20);
+    d_avg_factor = std::pow(d_net_factor, 1.0 / d_iterations);
+    d_ops.postprocessOneCycle(d_iterations - 1, u, d_r);
+  } while (d_residual_norm > eff && d_iterations < d_max_cycles);
+  hip_check(hipStreamSynchronize(d_gpu->stream()), "hipStreamSynchronize");
+  d_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (clear_when_done) deallocateSolverState();
+  return d_residual_norm < eff;
+}
+
+// ------------------------------------------------------------ solver
+
+CellPoissonFACSolver::CellPoissonFACSolver(hier::PatchHierarchy& h, const SolverParams& p)
+    : d_h(h) {
+  if (!h.finalized()) h.finalize();
+  hip_check(hipStreamCreate(&d_stream), "hipStreamCreate");
+  d_ops.reset(new CellPoissonFACOps(h, d_stream));
+  d_fac.reset(new FACPreconditioner(*d_ops, d_ops.get()));
+  d_fac->setMaxCycles(p.max_cycles);
+  d_fac->setResidualTolerance(p.residual_tol);
+  d_fac->setRelativeResidualTolerance(p.relative_residual_tol);
+  if (p.num_pre_sweeps < 0 || p.num_post_sweeps < 0) fail("CellPoissonFACSolver: bad sweeps");
+  d_fac->setPresmoothingSweeps(p.num_pre_sweeps);
+  d_fac->setPostsmoothingSweeps(p.num_post_sweeps);
+  d_ops->setCoarsestLevelSolverChoice(p.coarse_solver_choice);
+  d_ops->setCoarsestLevelSolverTolerance(p.coarse_solver_tolerance);
+  d_ops->setCoarsestLevelSolverMaxIterations(p.coarse_solver_max_iterations);
+  d_ops->setCoarseFineDiscretization(p.cf_discretization);
+  d_ops->setProlongationMethod(p.prolongation_method);
+  d_ops->enableLogging(p.enable_logging);
+}
+
+CellPoissonFACSolver::~CellPoissonFACSolver() {
+  deallocateSolverState();
+  d_fac.reset();
+  d_ops.reset();
+  if (d_stream) (void)hipStreamDestroy(d_stream);
+}
+
+void CellPoissonFACSolver::initializeSolverState(PatchDataSet& u, PatchDataSet& f, int coarsest,
+                                                 int finest) {
+  deallocateSolverState();
+  if (finest < 0) finest = d_h.numberOfLevels() - 1;
+  d_weights.reset(new PatchDataSet(d_h, PatchDataSet::kCell, 0));
+  d_ops->computeVectorWeights(*d_weights, coarsest, finest);
+  d_ops->setPoissonSpecifications(d_spec);
+  d_ops->setPhysicalBcCoefObject(d_bc);
+  d_uv = {&u, d_weights.get(), coarsest, finest};
+  d_fv = {&f, d_weights.get(), coarsest, finest};
+  d_fac->initializeSolverState(d_uv, d_fv);
+  d_initialized = true;
+}
+
+void CellPoissonFACSolver::deallocateSolverState() {
+  if (d_initialized) d_fac->deallocateSolverState();
+  d_weights.reset();
+  d_initialized = false;
+}
+
+bool CellPoissonFACSolver::solveSystem(PatchDataSet& u, PatchDataSet& f) {
+  bool clear = false;
+  if (!d_initialized) {
+    initializeSolverState(u, f);
+    clear = true;
+  } else if (d_uv.data != &u || d_fv.data != &f) {
+    fail("CellPoissonFACSolver: solveSystem vectors differ from the initialized state");
+  }
+  bool ok = d_fac->solveSystem(d_uv, d_fv);
+  if (clear) deallocateSolverState();
+  return ok;
+}
+
+}  // namespace solv
+}  // namespace samrai_amd

@@ -1,0 +1,306 @@
+// SAMRAI-shaped FAC Poisson solver on MI355X.
+//
+// Class and method names, argument meaning and input keys follow the
+// reference:
+//   solv::FACOperatorStrategy   (solv/FACOperatorStrategy.h:82-364)
+//   solv::CellPoissonFACOps     (solv/CellPoissonFACOps.h, .C)
+//   solv::FACPreconditioner     (solv/FACPreconditioner.h:146-440)
+//   solv::CellPoissonFACSolver  (solv/CellPoissonFACSolver.h:134-460)
+//   solv::PoissonSpecifications (solv/PoissonSpecifications.h:135-289)
+//   solv::LocationIndexRobinBcCoefs (solv/LocationIndexRobinBcCoefs.C)
+// Patch data live in device memory in pdat::ArrayData layout; every level
+// operation is a level-batched HIP launch (level_kernels.hip).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hierarchy.hpp"
+
+namespace samrai_amd {
+namespace pdat {
+
+// A double variable (cell- or side-centred) on every patch of every level,
+// device resident, in ArrayData layout.  ncomp = 1 (cell) or dim (side).
+class PatchDataSet {
+ public:
+  enum Centring { kCell = 0, kSide = 1 };
+  PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost);
+  ~PatchDataSet();
+  PatchDataSet(const PatchDataSet&) = delete;
+  PatchDataSet& operator=(const PatchDataSet&) = delete;
+
+  Centring centring() const { return d_centring; }
+  int ghost() const { return d_ghost; }
+  int numberOfLevels() const { return (int)d_levels.size(); }
+  const hier::PatchHierarchy& hierarchy() const { return d_h; }
+  // ArrayData::getPointer for patch p of level ln (component/axis a).
+  double* pointer(int ln, int p, int a = 0) const;
+  long size(int ln, int p, int a = 0) const;
+  // Device table of per-patch pointers for component a.
+  double* const* table(int ln, int a = 0) const { return d_levels[ln].dtab[a]; }
+  // Device table indexed [p * ncomp + a].
+  double* const* interleaved(int ln) const { return d_levels[ln].dil; }
+  double* slab(int ln) const { return d_levels[ln].slab; }
+  long slabSize(int ln) const { return d_levels[ln].total; }
+
+ private:
+  struct Lvl {
+    double* slab = nullptr;
+    long total = 0;
+    std::vector<std::vector<long>> off;  // [a][p]
+    std::vector<std::vector<long>> len;
+    double** dtab[3] = {nullptr, nullptr, nullptr};
+    double** dil = nullptr;
+  };
+  const hier::PatchHierarchy& d_h;
+  Centring d_centring;
+  int d_ghost;
+  std::vector<Lvl> d_levels;
+};
+
+}  // namespace pdat
+
+namespace solv {
+
+// A one-component SAMRAIVectorReal over levels [coarsest, finest] with its
+// control-volume (weight) data.
+struct SAMRAIVectorReal {
+  pdat::PatchDataSet* data = nullptr;
+  pdat::PatchDataSet* weights = nullptr;
+  int coarsest = 0, finest = 0;
+};
+
+class PoissonSpecifications {
+ public:
+  void setDConstant(double d) { d_dconst = d; d_D = nullptr; }
+  void setDPatchDataId(pdat::PatchDataSet* D) { d_D = D; }
+  void setCZero() { d_cmode = 0; d_C = nullptr; }
+  void setCConstant(double c) { d_cmode = 1; d_cconst = c; d_C = nullptr; }
+  void setCPatchDataId(pdat::PatchDataSet* C) { d_cmode = 2; d_C = C; }
+  bool dIsConstant() const { return d_D == nullptr; }
+  double getDConstant() const { return d_dconst; }
+  pdat::PatchDataSet* getD() const { return d_D; }
+  int cMode() const { return d_cmode; }
+  double getCConstant() const { return d_cconst; }
+  pdat::PatchDataSet* getC() const { return d_C; }
+
+ private:
+  double d_dconst = 1.0;
+  pdat::PatchDataSet* d_D = nullptr;
+  int d_cmode = 0;
+  double d_cconst = 0.0;
+  pdat::PatchDataSet* d_C = nullptr;
+};
+
+// Robin coefficients a u + b du/dn = g per location index.
+struct LocationIndexRobinBcCoefs {
+  double a[6] = {1, 1, 1, 1, 1, 1};
+  double b[6] = {0, 0, 0, 0, 0, 0};
+  double g[6] = {0, 0, 0, 0, 0, 0};
+};
+
+class FACOperatorStrategy {
+ public:
+  virtual ~FACOperatorStrategy() {}
+  virtual void restrictSolution(const SAMRAIVectorReal& s, SAMRAIVectorReal& d, int dest_ln) = 0;
+  virtual void restrictResidual(const SAMRAIVectorReal& s, SAMRAIVectorReal& d, int dest_ln) = 0;
+  virtual void prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAIVectorReal& d,
+                                      int dest_ln) = 0;
+  virtual void smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal& r, int ln,
+                           int num_sweeps) = 0;
+  virtual int solveCoarsestLevel(SAMRAIVectorReal& e, const SAMRAIVectorReal& r,
+                                 int coarsest_ln) = 0;
+  virtual void computeCompositeResidualOnLevel(SAMRAIVectorReal& r, const SAMRAIVectorReal& u,
+                                               const SAMRAIVectorReal& f, int ln,
+                                               bool error_equation_indicator) = 0;
+  virtual double computeResidualNorm(const SAMRAIVectorReal& r, int fine_ln, int coarse_ln) = 0;
+  virtual void initializeOperatorState(const SAMRAIVectorReal& u, const SAMRAIVectorReal& f) = 0;
+  virtual void deallocateOperatorState() = 0;
+  virtual void postprocessOneCycle(int fac_cycle_num, const SAMRAIVectorReal& u,
+                                   const SAMRAIVectorReal& r) = 0;
+};
+
+class CellPoissonFACOps : public FACOperatorStrategy {
+ public:
+  CellPoissonFACOps(hier::PatchHierarchy& h, hipStream_t stream);
+  ~CellPoissonFACOps() override;
+
+  // input keys of CellPoissonFACOps::getFromInput (CellPoissonFACOps.C:691-739)
+  void setCoarsestLevelSolverChoice(const std::string& c);
+  void setCoarsestLevelSolverTolerance(double t) { d_coarse_tol = t; }
+  void setCoarsestLevelSolverMaxIterations(int n) { d_coarse_max = n; }
+  void setCoarseFineDiscretization(const std::string& c);
+  void setProlongationMethod(const std::string& m);
+  void setPoissonSpecifications(const PoissonSpecifications& s) { d_spec = s; }
+  void setPhysicalBcCoefObject(const LocationIndexRobinBcCoefs& bc) { d_bc = bc; }
+  void enableLogging(bool on) { d_logging = on; }
+
+  void computeVectorWeights(pdat::PatchDataSet& w, int coarsest_ln, int finest_ln);
+
+  // FACOperatorStrategy
+  void restrictSolution(const SAMRAIVectorReal& s, SAMRAIVectorReal& d, int dest_ln) override;
+  void restrictResidual(const SAMRAIVectorReal& s, SAMRAIVectorReal& d, int dest_ln) override;
+  void prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAIVectorReal& d,
+                              int dest_ln) override;
+  void smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal& r, int ln,
+                   int num_sweeps) override;
+  int solveCoarsestLevel(SAMRAIVectorReal& e, const SAMRAIVectorReal& r,
+                         int coarsest_ln) override;
+  void computeCompositeResidualOnLevel(SAMRAIVectorReal& r, const SAMRAIVectorReal& u,
+                                       const SAMRAIVectorReal& f, int ln,
+                                       bool error_equation_indicator) override;
+  double computeResidualNorm(const SAMRAIVectorReal& r, int fine_ln, int coarse_ln) override;
+  void initializeOperatorState(const SAMRAIVectorReal& u, const SAMRAIVectorReal& f) override;
+  void deallocateOperatorState() override;
+  void postprocessOneCycle(int fac_cycle_num, const SAMRAIVectorReal& u,
+                           const SAMRAIVectorReal& r) override;
+
+  // smoothErrorByRedBlack with an explicit tolerance (<0: none); returns
+  // the sweeps done (read back only when tol >= 0).
+  int smoothErrorByRedBlack(pdat::PatchDataSet& e, const pdat::PatchDataSet& r, int ln,
+                            int num_sweeps, double tol, double* maxres_out = nullptr);
+  // xeqScheduleGhostFill / xeqScheduleGhostFillNoCoarse
+  void ghostFill(pdat::PatchDataSet& x, int ln, bool with_coarse, bool homogeneous,
+                 bool faces_only);
+  // weighted RMS over levels [coarse_ln, fine_ln]
+  double rmsNorm(const pdat::PatchDataSet& x, const pdat::PatchDataSet& w, int coarse_ln,
+                 int fine_ln);
+  // HierarchyCellDataOpsReal helpers (interior unless noted)
+  void setToScalar(pdat::PatchDataSet& x, double v, int coarse_ln, int fine_ln);  // incl. ghosts
+  void add(pdat::PatchDataSet& y, const pdat::PatchDataSet& x, int coarse_ln, int fine_ln);
+  hipStream_t stream() const { return d_stream; }
+  pdat::PatchDataSet& flux() { return *d_flux; }
+
+ private:
+  sfac::PoissonDev poissonDev(int ln) const;
+  sfac::RobinDev robinDev() const;
+  hier::PatchHierarchy& d_h;
+  hipStream_t d_stream;
+  std::string d_coarse_choice = "redblack";
+  double d_coarse_tol = 1e-8;
+  int d_coarse_max = 10;
+  bool d_ewing = true;
+  std::string d_prolongation = "CONSTANT_REFINE";
+  PoissonSpecifications d_spec;
+  LocationIndexRobinBcCoefs d_bc;
+  bool d_logging = false;
+  int d_ln_min = -1, d_ln_max = -1;
+  std::unique_ptr<pdat::PatchDataSet> d_flux;
+  std::vector<double*> d_oflux_slab;
+  std::vector<double**> d_oflux_tab;
+  std::vector<double*> d_maxres;  // per level: red[np], blk[np]
+  int* d_stop = nullptr;          // [0] stop flag, [1] sweeps
+  double* d_scalars = nullptr;    // [0] dot result, [1] maxres
+  double* d_partial = nullptr;
+  long d_partial_n = 0;
+  double* h_pinned = nullptr;
+};
+
+class FACPreconditioner {
+ public:
+  FACPreconditioner(FACOperatorStrategy& ops, CellPoissonFACOps* gpu_ops);
+  ~FACPreconditioner();
+  // input keys (FACPreconditioner.C:79-114)
+  void setMaxCycles(int n);
+  void setResidualTolerance(double t);
+  void setRelativeResidualTolerance(double t);
+  void setPresmoothingSweeps(int n) { d_pre = n; }
+  void setPostsmoothingSweeps(int n) { d_post = n; }
+  void setAlgorithmChoice(const std::string& c);
+
+  void initializeSolverState(const SAMRAIVectorReal& u, const SAMRAIVectorReal& f);
+  void deallocateSolverState();
+  bool solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f);
+
+  int getNumberOfIterations() const { return d_iterations; }
+  double getResidualNorm() const { return d_residual_norm; }
+  void getConvergenceFactors(double& avg, double& fin) const;
+  double getNetConvergenceFactor() const { return d_net_factor; }
+  double getAvgConvergenceFactor() const { return d_avg_factor; }
+  const std::vector<double>& residualHistory() const { return d_history; }
+  double lastSolveSeconds() const { return d_seconds; }
+  // one V-cycle on the error equation with the current residual (tests)
+  void runOneCycle(SAMRAIVectorReal& u);
+  SAMRAIVectorReal& errorVector() { return d_e; }
+  SAMRAIVectorReal& residualVector() { return d_r; }
+  SAMRAIVectorReal& tmpResidualVector() { return d_tr; }
+  double computeFullCompositeResidual(SAMRAIVectorReal& r, SAMRAIVectorReal& u,
+                                      SAMRAIVectorReal& f);
+
+ private:
+  void facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal& r, SAMRAIVectorReal& u,
+                          int lmax, int lmin, int ln);
+  FACOperatorStrategy& d_ops;
+  CellPoissonFACOps* d_gpu;
+  int d_max_cycles = 10;
+  double d_residual_tol = 1e-6;
+  double d_rel_tol = -1.0;
+  int d_pre = 1, d_post = 1;
+  bool d_initialized = false;
+  int d_lmin = 0, d_lmax = 0;
+  std::unique_ptr<pdat::PatchDataSet> d_e_data, d_r_data, d_tr_data;
+  SAMRAIVectorReal d_e, d_r, d_tr;
+  int d_iterations = 0;
+  double d_residual_norm = 0.0;
+  double d_avg_factor = 0.0, d_net_factor = 0.0;
+  std::vector<double> d_factors;
+  std::vector<double> d_history;
+  double d_seconds = 0.0;
+};
+
+struct SolverParams {
+  // fac_precond block
+  int max_cycles = 10;
+  double residual_tol = 1e-6;
+  double relative_residual_tol = -1.0;
+  int num_pre_sweeps = 1;
+  int num_post_sweeps = 1;
+  // fac_ops block
+  std::string coarse_solver_choice = "redblack";
+  double coarse_solver_tolerance = 1e-8;
+  int coarse_solver_max_iterations = 10;
+  std::string cf_discretization = "Ewing";
+  std::string prolongation_method = "CONSTANT_REFINE";
+  bool enable_logging = false;
+};
+
+class CellPoissonFACSolver {
+ public:
+  CellPoissonFACSolver(hier::PatchHierarchy& h, const SolverParams& p);
+  ~CellPoissonFACSolver();
+  void setDConstant(double d) { d_spec.setDConstant(d); }
+  void setDPatchDataId(pdat::PatchDataSet* D) { d_spec.setDPatchDataId(D); }
+  void setCConstant(double c) { d_spec.setCConstant(c); }
+  void setCPatchDataId(pdat::PatchDataSet* C) { d_spec.setCPatchDataId(C); }
+  void setBcObject(const LocationIndexRobinBcCoefs& bc) { d_bc = bc; }
+  void initializeSolverState(pdat::PatchDataSet& u, pdat::PatchDataSet& f, int coarsest = 0,
+                             int finest = -1);
+  void deallocateSolverState();
+  bool solveSystem(pdat::PatchDataSet& u, pdat::PatchDataSet& f);
+  int getNumberOfIterations() const { return d_fac->getNumberOfIterations(); }
+  double getResidualNorm() const { return d_fac->getResidualNorm(); }
+  void getConvergenceFactors(double& avg, double& fin) const {
+    d_fac->getConvergenceFactors(avg, fin);
+  }
+  FACPreconditioner& preconditioner() { return *d_fac; }
+  CellPoissonFACOps& ops() { return *d_ops; }
+  pdat::PatchDataSet& weights() { return *d_weights; }
+  SAMRAIVectorReal& uVector() { return d_uv; }
+  SAMRAIVectorReal& fVector() { return d_fv; }
+
+ private:
+  hier::PatchHierarchy& d_h;
+  hipStream_t d_stream = nullptr;
+  PoissonSpecifications d_spec;
+  LocationIndexRobinBcCoefs d_bc;
+  std::unique_ptr<CellPoissonFACOps> d_ops;
+  std::unique_ptr<FACPreconditioner> d_fac;
+  std::unique_ptr<pdat::PatchDataSet> d_weights;
+  SAMRAIVectorReal d_uv, d_fv;
+  bool d_initialized = false;
+};
+
+}  // namespace solv
+}  // namespace samrai_amd

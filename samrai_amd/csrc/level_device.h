@@ -1,0 +1,74 @@
+// Device-side descriptor tables for the level-batched FAC kernels.
+//
+// A level is a set of patches of uneven size.  Every level operation is one
+// launch over a work list built once per hierarchy (BoxItem + Chunk): the
+// host precomputes which cells each work-group touches, the kernels only
+// decode their chunk and index pdat::ArrayData storage through per-patch
+// pointer tables.
+#pragma once
+
+#include "fac_common.h"
+
+namespace sfac {
+
+constexpr int kThreads = 256;       // work-group size of every level kernel
+constexpr int kCellsPerChunk = 512; // cells a work-group handles per chunk
+
+// One rectangular piece of work: destination patch p, source patch q,
+// index box [lo, hi] (in the index space the kernel documents) and a tag.
+struct BoxItem {
+  int p, q;
+  int lo[3];
+  int hi[3];
+  int tag;
+  int pad;
+};
+
+struct Chunk {
+  int item;
+  int start;  // first cell (linear, i fastest) of the item's box
+};
+
+// Per-level geometry shared by all kernels.
+struct LevelGeom {
+  const Box* box;          // interior box per patch
+  int npatch;
+  int dim;
+  int ratio;               // ratio to the next coarser level
+  double dx[3];
+  // coarse-fine flags of face ghosts: for patch p, face f (2*axis+side),
+  // cf_off[p*6+f] indexes a byte array over the face (t1 fastest, then t2)
+  const unsigned char* cf;
+  const long* cf_off;
+};
+
+// Poisson specification (PoissonSpecifications): D constant or side data,
+// C zero / constant / cell data.
+struct PoissonDev {
+  const double* const* D[3];  // side data per axis (nullptr: constant)
+  int dghost;
+  double dconst;
+  int cmode;                  // 0 zero, 1 constant, 2 cell data
+  double cconst;
+  const double* const* C;
+  int cghost;
+};
+
+// Robin coefficients per location index (LocationIndexRobinBcCoefs).
+struct RobinDev {
+  double a[6], b[6], g[6];
+};
+
+__host__ __device__ inline bool decode_box(const BoxItem& it, long t, int* c) {
+  int n0 = it.hi[0] - it.lo[0] + 1, n1 = it.hi[1] - it.lo[1] + 1;
+  int n2 = it.hi[2] - it.lo[2] + 1;
+  long n01 = (long)n0 * n1;
+  if (t >= n01 * n2) return false;
+  c[2] = it.lo[2] + (int)(t / n01);
+  long r = t % n01;
+  c[1] = it.lo[1] + (int)(r / n0);
+  c[0] = it.lo[0] + (int)(r % n0);
+  return true;
+}
+
+}  // namespace sfac

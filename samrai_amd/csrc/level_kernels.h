@@ -1,0 +1,82 @@
+// Host launchers for the level-batched FAC kernels (level_kernels.hip).
+#pragma once
+
+#include "level_device.h"
+
+namespace sfac {
+
+// Ghost-item kinds (BoxItem::tag & 15); the location index is tag >> 4.
+enum GhostKind { kCopy = 0, kCoarseFine = 1, kPhys1 = 2, kPhys2 = 3, kPhys3 = 4 };
+
+struct GhostLaunch {
+  const BoxItem* items;
+  const Chunk* chunks;
+  int nchunks;
+};
+
+// Fill ghost cells of a level vector from a ghost work list (one phase).
+void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int dg,
+                        const double* const* coarse, int cg, const Box* coarse_box,
+                        int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s);
+
+// One red (colour 0) or black (colour 1) Gauss-Seidel pass with the flux
+// (and Ewing coarse-fine correction) computed on the fly.
+struct SmoothLaunch {
+  const BoxItem* items;  // one per patch (interior box)
+  const Chunk* chunks;   // chunks over (half-row, row) colour space
+  int nchunks;
+};
+void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, int eg,
+                 const double* const* rhs, int rg, const PoissonDev& ps, int ewing,
+                 int colour, double* maxres, const int* stop, hipStream_t s);
+
+// Materialised flux (3 side arrays / patch, ghost 0), with the Ewing fix.
+void launch_flux(const GhostLaunch& cells, const LevelGeom& L, double* const* flux,
+                 const double* const* u, int ug, const PoissonDev& ps, int ewing,
+                 hipStream_t s);
+
+// Coarse faces under the finer level's patch boundaries take the
+// area-weighted mean of the saved fine flux (outerside) values.
+void launch_flux_coarsen(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
+                         double* const* cflux, const double* const* foflux, hipStream_t s);
+
+// r = f - div F - C u on the interior; optionally save the boundary flux.
+void launch_residual(const GhostLaunch& cells, const LevelGeom& L, double* const* r, int rg,
+                     const double* const* f, int fg, const double* const* u, int ug,
+                     const double* const* flux, const PoissonDev& ps,
+                     double* const* oflux, hipStream_t s);
+
+// Cell weighted average fine -> coarse (items: p coarse, q fine, coarse box).
+void launch_restrict(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
+                     double* const* dst, int dg, const double* const* src, int sg,
+                     hipStream_t s);
+
+// Conservative linear refine of the coarse error and e_f += it
+// (items: p fine, q coarse, fine box).
+void launch_prolong_add(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
+                        const double* const* ce, int cg, double* const* fe, int fg,
+                        int constant, hipStream_t s);
+
+// Partial sums of w*x*y over interior chunks (one double per chunk);
+// x == nullptr sums w alone.
+void launch_dot_partial(const GhostLaunch& cells, const LevelGeom& L, const double* const* x,
+                        int xg, const double* const* y, int yg, const double* const* w,
+                        double* partial, hipStream_t s);
+// out[0] = sum of n partials in a fixed order.
+void launch_sum(const double* partial, int n, double* out, int accumulate, hipStream_t s);
+
+// y = a*y + x on interior cells (a = 1: y += x).
+void launch_axpy(const GhostLaunch& cells, const LevelGeom& L, double* const* y, int yg,
+                 const double* const* x, int xg, hipStream_t s);
+// Fill n doubles with v.
+void launch_fill(double* p, long n, double v, hipStream_t s);
+void launch_fill_int(int* p, int n, int v, hipStream_t s);
+// Set the cells of each item's box in patch item.p to v (dst ghost width g).
+void launch_set_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int g,
+                      double v, hipStream_t s);
+// Coarse-solve convergence test (smoothErrorByRedBlack): not_converged =
+// max(red[last], black[last]) > tol; counts sweeps; clears the maxres slots.
+void launch_sweep_check(double* red, double* blk, int npatch, double tol, int* stop,
+                        int* sweeps, double* maxres_out, hipStream_t s);
+
+}  // namespace sfac

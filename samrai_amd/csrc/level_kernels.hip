@@ -1,0 +1,755 @@
+// Level-batched FAC kernels for gfx950.
+//
+// Every launch covers all patches of one level (uneven sizes) through a
+// chunk table: work-group g takes chunk g, i.e. kCellsPerChunk consecutive
+// cells of one BoxItem, so a wavefront never straddles two patches and the
+// per-patch reductions need one atomic per wavefront.  Patch data are
+// pdat::ArrayData arrays reached through per-patch pointer tables.
+//
+// Arithmetic follows the reference Fortran expression order
+// (solv_cellpoissonfacops{2d,3d}.f, solv_cartesianrobinbchelper{2d,3d}.f and
+// the geom/pdat m4 templates) and the library is built with
+// -ffp-contract=off, so FP64 results match the per-patch kernels bit for bit.
+#include "level_kernels.h"
+
+namespace sfac {
+namespace {
+
+__device__ inline View vview(const Box& b, int g, int dim, int extra = -1) {
+  return make_view(b.lo, b.hi, g, dim, extra);
+}
+
+__device__ inline void tangential(int a, int dim, int& t1, int& t2) {
+  if (dim == 2) { t1 = 1 - a; t2 = 2; return; }
+  if (a == 0) { t1 = 1; t2 = 2; } else if (a == 1) { t1 = 0; t2 = 2; } else { t1 = 0; t2 = 1; }
+}
+
+// Is the ghost cell across face `face` (2*axis+side) of patch p, next to
+// interior cell c, a coarse-fine ghost?
+__device__ inline bool cf_flag(const LevelGeom& L, int p, int face, const int* c, const Box& b) {
+  long off = L.cf_off[p * 6 + face];
+  if (off < 0) return false;
+  int t1, t2;
+  tangential(face >> 1, L.dim, t1, t2);
+  long idx = c[t1] - b.lo[t1];
+  if (L.dim == 3) idx += (long)(b.hi[t1] - b.lo[t1] + 1) * (c[t2] - b.lo[t2]);
+  return L.cf[off + idx] != 0;
+}
+
+// Flux D grad u on the face of axis a with face index c (between c-e_a and
+// c), with the Ewing coarse-fine correction on flagged patch-boundary faces
+// (solv_cellpoissonfacops3d.f:10-74 and 604-1011).
+template <int DIM, bool DVAR>
+__device__ inline double face_flux(const LevelGeom& L, int p, const Box& b, const double* u,
+                                   const View& vu, const double* const* Dp, const View* vd,
+                                   double dconst, int a, int i, int j, int k, double dai,
+                                   int ewing) {
+  int c[3] = {i, j, k};
+  int m[3] = {i, j, k};
+  m[a] -= 1;
+  double D = DVAR ? Dp[a][vd[a].at(i, j, k)] : dconst;
+  double F = dai * D * (u[vu.at(i, j, k)] - u[vu.at(m[0], m[1], m[2])]);
+  if (ewing) {
+    int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
+    if (side >= 0) {
+      const int* in = side == 0 ? c : m;
+      if (cf_flag(L, p, 2 * a + side, in, b)) {
+        int r = L.ratio;
+        int q[3] = {in[0], in[1], in[2]};
+        int t1, t2;
+        tangential(a, DIM, t1, t2);
+        int bs1 = in[t1] - (in[t1] - cdiv(in[t1], r) * r);
+        q[t1] = bs1 + (r - 1 - (in[t1] - bs1));
+        if (DIM == 3) {
+          int bs2 = in[t2] - (in[t2] - cdiv(in[t2], r) * r);
+          q[t2] = bs2 + (r - 1 - (in[t2] - bs2));
+        }
+        double tranwt = ewing_tranwt(r);
+        double longwt = 2 * tranwt;
+        double h = L.dx[a];
+        double tv = tranwt * D * (u[vu.at(q[0], q[1], q[2])] - u[vu.at(in[0], in[1], in[2])]) / h;
+        F = side == 0 ? longwt * F + tv : longwt * F - tv;
+      }
+    }
+  }
+  return F;
+}
+
+__device__ inline void atomic_max_nonneg(double* p, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(p),
+            (unsigned long long)__double_as_longlong(v));
+}
+
+// ------------------------------------------------------------ ghost fill
+
+__global__ void __launch_bounds__(kThreads)
+k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* dst, int dg,
+              const double* const* coarse, int cg, const Box* cbox, int homogeneous,
+              RobinDev bc, const int* stop) {
+  if (stop && *stop) return;
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const int kind = it.tag & 15, loc = it.tag >> 4;
+  const Box b = L.box[it.p];
+  const View vd = vview(b, dg, L.dim);
+  double* x = dst[it.p];
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    if (kind == kCopy) {
+      const Box sb = L.box[it.q];
+      x[vd.at(c[0], c[1], c[2])] = dst[it.q][vview(sb, dg, L.dim).at(c[0], c[1], c[2])];
+    } else if (kind == kCoarseFine) {
+      const Box sb = cbox[it.q];
+      int r = L.ratio;
+      int cc0 = cdiv(c[0], r), cc1 = cdiv(c[1], r), cc2 = L.dim == 3 ? cdiv(c[2], r) : 0;
+      x[vd.at(c[0], c[1], c[2])] = coarse[it.q][vview(sb, cg, L.dim).at(cc0, cc1, cc2)];
+    } else if (kind == kPhys1) {
+      // settype1cells: u_g = (h g + u_in (b - h a / 2)) / (b + h a / 2)
+      int n = loc >> 1, up = loc & 1;
+      int in[3] = {c[0], c[1], c[2]};
+      in[n] += up ? -1 : 1;
+      double h = L.dx[n], av = bc.a[loc], bv = bc.b[loc];
+      double uin = x[vd.at(in[0], in[1], in[2])];
+      double v;
+      if (L.dim == 3)
+        v = ((homogeneous ? 0.0 : h * bc.g[loc]) + uin * (bv - 0.5 * h * av)) /
+            (bv + 0.5 * h * av);
+      else if (homogeneous)
+        v = (uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
+      else
+        v = (h * bc.g[loc] + uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
+      x[vd.at(c[0], c[1], c[2])] = v;
+    } else if (kind == kPhys2) {
+      // settype2cells: edge (3-D) or node (2-D) extrapolation
+      int along, p_, q_, sp, sq, order_qp;
+      if (L.dim == 3) {
+        int grp = loc / 4, m = loc % 4;
+        if (grp == 0) { along = 2; p_ = 0; q_ = 1; }
+        else if (grp == 1) { along = 1; p_ = 0; q_ = 2; }
+        else { along = 0; p_ = 1; q_ = 2; }
+        sp = (m & 1) ? -1 : 1;
+        sq = (m & 2) ? -1 : 1;
+        order_qp = grp < 2 ? (m == 3) : (m != 3);
+      } else {
+        along = 2; p_ = 0; q_ = 1;
+        sp = (loc & 1) ? -1 : 1;
+        sq = (loc & 2) ? -1 : 1;
+        order_qp = loc == 3;
+      }
+      (void)along;
+      int cpq[3] = {c[0], c[1], c[2]}, cp[3] = {c[0], c[1], c[2]}, cq[3] = {c[0], c[1], c[2]};
+      cpq[p_] += sp; cpq[q_] += sq; cp[p_] += sp; cq[q_] += sq;
+      double vpq = x[vd.at(cpq[0], cpq[1], cpq[2])];
+      double vp = x[vd.at(cp[0], cp[1], cp[2])];
+      double vq = x[vd.at(cq[0], cq[1], cq[2])];
+      double sum = order_qp ? (vq + vp) : (vp + vq);
+      x[vd.at(c[0], c[1], c[2])] = -vpq + sum;
+    } else {
+      // settype3cells3d corner
+      int si = (loc & 1) ? -1 : 1, sj = (loc & 2) ? -1 : 1, sk = (loc & 4) ? -1 : 1;
+      int i = c[0], j = c[1], k = c[2];
+      const double c23 = (double)(2.0f / 3.0f);
+      x[vd.at(i, j, k)] = -x[vd.at(i + si, j + sj, k + sk)] +
+                          c23 * (x[vd.at(i, j + sj, k + sk)] + x[vd.at(i + si, j, k + sk)] +
+                                 x[vd.at(i + si, j + sj, k)]);
+    }
+  }
+}
+
+// ----------------------------------------------------------------- RBGS
+
+template <int DIM, bool DVAR, int CMODE>
+__global__ void __launch_bounds__(kThreads)
+k_rbgs(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* e, int eg,
+       const double* const* rhs, int rg, PoissonDev ps, int ewing, int colour, double* maxres,
+       const int* stop) {
+  if (stop && *stop) return;
+  const Chunk ch = chunks[blockIdx.x];
+  const int p = items[ch.item].p;
+  const Box b = L.box[p];
+  const View vu = vview(b, eg, DIM), vr = vview(b, rg, DIM);
+  View vd[3];
+  const double* Dp[3] = {nullptr, nullptr, nullptr};
+  if (DVAR) {
+    for (int a = 0; a < DIM; ++a) {
+      vd[a] = vview(b, ps.dghost, DIM, a);
+      Dp[a] = ps.D[a][p];
+    }
+  }
+  View vc = vview(b, ps.cghost, DIM);
+  const double* Cp = CMODE == 2 ? ps.C[p] : nullptr;
+  double* u = e[p];
+  const double* f = rhs[p];
+  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
+  const int half = (nx + 1) >> 1;
+  const double dxi = 1.0 / L.dx[0], dyi = 1.0 / L.dx[1];
+  const double dzi = DIM == 3 ? 1.0 / L.dx[2] : 0.0;
+  double mr = 0.0;
+  const long total = (long)half * ny * (DIM == 3 ? (b.hi[2] - b.lo[2] + 1) : 1);
+  for (int t0 = threadIdx.x; t0 < kCellsPerChunk; t0 += kThreads) {
+    long t = (long)ch.start + t0;
+    if (t >= total) break;
+    int m = (int)(t % half);
+    long row = t / half;
+    int j = b.lo[1] + (int)(row % ny);
+    int k = DIM == 3 ? b.lo[2] + (int)(row / ny) : 0;
+    int par = DIM == 3 ? fmod2(b.lo[0] + j + k) : fmod2(b.lo[1] + j);
+    int i = b.lo[0] + (par != colour ? 1 : 0) + 2 * m;
+    if (i > b.hi[0]) continue;
+    double fxl = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 0, i, j, k, dxi, ewing);
+    double fxh =
+        face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 0, i + 1, j, k, dxi, ewing);
+    double fyl = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 1, i, j, k, dyi, ewing);
+    double fyh =
+        face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 1, i, j + 1, k, dyi, ewing);
+    double scal = CMODE == 2 ? Cp[vc.at(i, j, k)] : (CMODE == 1 ? ps.cconst : 0.0);
+    double uc = u[vu.at(i, j, k)];
+    double residual, diag;
+    if (DIM == 3) {
+      double fzl =
+          face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 2, i, j, k, dzi, ewing);
+      double fzh =
+          face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 2, i, j, k + 1, dzi, ewing);
+      residual = f[vr.at(i, j, k)] -
+                 (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
+      double dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
+      if (DVAR)
+        diag = (dxi2 * (Dp[0][vd[0].at(i + 1, j, k)] + Dp[0][vd[0].at(i, j, k)]) +
+                dyi2 * (Dp[1][vd[1].at(i, j + 1, k)] + Dp[1][vd[1].at(i, j, k)]) +
+                dzi2 * (Dp[2][vd[2].at(i, j, k + 1)] + Dp[2][vd[2].at(i, j, k)])) -
+               scal;
+      else
+        diag = (dxi2 * (ps.dconst + ps.dconst) + dyi2 * (ps.dconst + ps.dconst) +
+                dzi2 * (ps.dconst + ps.dconst)) -
+               scal;
+    } else {
+      residual = f[vr.at(i, j, k)] - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
+      if (DVAR)
+        diag = (dxi * dxi * (Dp[0][vd[0].at(i + 1, j, k)] + Dp[0][vd[0].at(i, j, k)]) +
+                dyi * dyi * (Dp[1][vd[1].at(i, j + 1, k)] + Dp[1][vd[1].at(i, j, k)])) -
+               scal;
+      else
+        diag = (dxi * dxi * (ps.dconst + ps.dconst) + dyi * dyi * (ps.dconst + ps.dconst)) -
+               scal;
+    }
+    double dudr = 1.0 / diag;
+    double du = -residual * dudr;
+    u[vu.at(i, j, k)] = uc + du * 1.0;
+    mr = fmax(mr, fabs(residual));
+  }
+  if (maxres) {
+    for (int o = 32; o > 0; o >>= 1) mr = fmax(mr, __shfl_xor(mr, o, 64));
+    if ((threadIdx.x & 63) == 0 && mr > 0.0) atomic_max_nonneg(&maxres[p], mr);
+  }
+}
+
+// ------------------------------------------------------ flux / residual
+
+template <int DIM, bool DVAR>
+__global__ void __launch_bounds__(kThreads)
+k_flux(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* flux,
+       const double* const* uu, int ug, PoissonDev ps, int ewing) {
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const int p = it.p;
+  const Box b = L.box[p];
+  const View vu = vview(b, ug, DIM);
+  View vd[3], vf[3];
+  const double* Dp[3] = {nullptr, nullptr, nullptr};
+  for (int a = 0; a < DIM; ++a) {
+    vf[a] = vview(b, 0, DIM, a);
+    if (DVAR) {
+      vd[a] = vview(b, ps.dghost, DIM, a);
+      Dp[a] = ps.D[a][p];
+    }
+  }
+  const double* u = uu[p];
+  double* F[3] = {flux[p * 3 + 0], flux[p * 3 + 1], DIM == 3 ? flux[p * 3 + 2] : nullptr};
+  double di[3] = {1.0 / L.dx[0], 1.0 / L.dx[1], DIM == 3 ? 1.0 / L.dx[2] : 0.0};
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    for (int a = 0; a < DIM; ++a) {
+      F[a][vf[a].at(c[0], c[1], c[2])] = face_flux<DIM, DVAR>(
+          L, p, b, u, vu, Dp, vd, ps.dconst, a, c[0], c[1], c[2], di[a], ewing);
+      if (c[a] == b.hi[a]) {
+        int h[3] = {c[0], c[1], c[2]};
+        h[a] += 1;
+        F[a][vf[a].at(h[0], h[1], h[2])] = face_flux<DIM, DVAR>(
+            L, p, b, u, vu, Dp, vd, ps.dconst, a, h[0], h[1], h[2], di[a], ewing);
+      }
+    }
+  }
+}
+
+// Outerside slab index for face axis a over the patch's tangential extent.
+__device__ inline long oslab(const Box& b, int a, int dim, const int* c) {
+  int t1, t2;
+  tangential(a, dim, t1, t2);
+  long idx = c[t1] - b.lo[t1];
+  if (dim == 3) idx += (long)(b.hi[t1] - b.lo[t1] + 1) * (c[t2] - b.lo[t2]);
+  return idx;
+}
+
+template <int DIM, int CMODE>
+__global__ void __launch_bounds__(kThreads)
+k_residual(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* rr, int rg,
+           const double* const* ff, int fg, const double* const* uu, int ug,
+           const double* const* flux, PoissonDev ps, double* const* oflux) {
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const int p = it.p;
+  const Box b = L.box[p];
+  const View vu = vview(b, ug, DIM), vr = vview(b, rg, DIM), vfg = vview(b, fg, DIM);
+  const View vc = vview(b, ps.cghost, DIM);
+  View vf[3];
+  for (int a = 0; a < DIM; ++a) vf[a] = vview(b, 0, DIM, a);
+  const double* F[3] = {flux[p * 3 + 0], flux[p * 3 + 1], DIM == 3 ? flux[p * 3 + 2] : nullptr};
+  const double dxi = 1.0 / L.dx[0], dyi = 1.0 / L.dx[1];
+  const double dzi = DIM == 3 ? 1.0 / L.dx[2] : 0.0;
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    int i = c[0], j = c[1], k = c[2];
+    double scal = CMODE == 2 ? ps.C[p][vc.at(i, j, k)] : (CMODE == 1 ? ps.cconst : 0.0);
+    double res;
+    if (DIM == 3)
+      res = ff[p][vfg.at(i, j, k)] -
+            (dxi * (F[0][vf[0].at(i + 1, j, k)] - F[0][vf[0].at(i, j, k)]) +
+             dyi * (F[1][vf[1].at(i, j + 1, k)] - F[1][vf[1].at(i, j, k)]) +
+             dzi * (F[2][vf[2].at(i, j, k + 1)] - F[2][vf[2].at(i, j, k)])) -
+            scal * uu[p][vu.at(i, j, k)];
+    else
+      res = ff[p][vfg.at(i, j, k)] -
+            (dxi * (F[0][vf[0].at(i + 1, j, k)] - F[0][vf[0].at(i, j, k)]) +
+             dyi * (F[1][vf[1].at(i, j + 1, k)] - F[1][vf[1].at(i, j, k)])) -
+            scal * uu[p][vu.at(i, j, k)];
+    rr[p][vr.at(i, j, k)] = res;
+    if (oflux) {
+      // OutersideData::copy(SideData): the patch's boundary faces
+      for (int a = 0; a < DIM; ++a) {
+        if (c[a] == b.lo[a]) oflux[p * 6 + 2 * a][oslab(b, a, DIM, c)] = F[a][vf[a].at(i, j, k)];
+        if (c[a] == b.hi[a]) {
+          int h[3] = {i, j, k};
+          h[a] += 1;
+          oflux[p * 6 + 2 * a + 1][oslab(b, a, DIM, c)] = F[a][vf[a].at(h[0], h[1], h[2])];
+        }
+      }
+    }
+  }
+}
+
+// items: p coarse patch, q fine patch, box = coarse faces (axis a = tag&3 at
+// one face index, side = tag>>2).
+template <int DIM>
+__global__ void __launch_bounds__(kThreads)
+k_flux_coarsen(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
+               double* const* cflux, const double* const* foflux) {
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const int a = it.tag & 3, side = it.tag >> 2;
+  const Box cb = C.box[it.p], fb = Fl.box[it.q];
+  const View vf = vview(cb, 0, DIM, a);
+  const double* src = foflux[it.q * 6 + 2 * a + side];
+  const int r = Fl.ratio;
+  int in, out;
+  if (DIM == 2) { in = 1 - a; out = -1; }
+  else if (a == 0) { in = 1; out = 2; } else if (a == 1) { in = 0; out = 2; } else { in = 0; out = 1; }
+  int t1, t2;
+  tangential(a, DIM, t1, t2);
+  const long n1 = fb.hi[t1] - fb.lo[t1] + 1;
+  const double areaf = out >= 0 ? Fl.dx[in] * Fl.dx[out] : Fl.dx[in];
+  const double areac = out >= 0 ? C.dx[in] * C.dx[out] : C.dx[in];
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    double acc = 0.0;
+    int rout = out >= 0 ? r : 1;
+    for (int ro = 0; ro < rout; ++ro)
+      for (int ri = 0; ri < r; ++ri) {
+        int f[3] = {0, 0, 0};
+        f[in] = c[in] * r + ri;
+        if (out >= 0) f[out] = c[out] * r + ro;
+        long idx = f[t1] - fb.lo[t1];
+        if (DIM == 3) idx += n1 * (f[t2] - fb.lo[t2]);
+        acc = acc + src[idx] * areaf;
+      }
+    cflux[it.p * 3 + a][vf.at(c[0], c[1], c[2])] = acc / areac;
+  }
+}
+
+// ----------------------------------------------------- restrict / prolong
+
+template <int DIM>
+__global__ void __launch_bounds__(kThreads)
+k_restrict(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
+           double* const* dst, int dg, const double* const* src, int sg) {
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const View vc = vview(C.box[it.p], dg, DIM), vf = vview(Fl.box[it.q], sg, DIM);
+  const double* x = src[it.q];
+  double* y = dst[it.p];
+  const int r = Fl.ratio;
+  double dVf = Fl.dx[0] * Fl.dx[1], dVc = C.dx[0] * C.dx[1];
+  if (DIM == 3) { dVf = dVf * Fl.dx[2]; dVc = dVc * C.dx[2]; }
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    double acc = 0.0;
+    for (int i2 = 0; i2 < (DIM == 3 ? r : 1); ++i2)
+      for (int i1 = 0; i1 < r; ++i1)
+        for (int i0 = 0; i0 < r; ++i0)
+          acc = acc + x[vf.at(c[0] * r + i0, c[1] * r + i1, DIM == 3 ? c[2] * r + i2 : 0)] * dVf;
+    y[vc.at(c[0], c[1], c[2])] = acc / dVc;
+  }
+}
+
+__device__ inline double clin_slope(double dm, double dp, double dxc) {
+  double coef2 = 0.5 * (dp + dm);
+  double bound = 2.0 * fmin(fabs(dp), fabs(dm));
+  if (dm * dp > 0.0) return copysign(fmin(fabs(coef2), bound), coef2) / dxc;
+  return 0.0;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kThreads)
+k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
+              const double* const* ce, int cg, double* const* fe, int fg, int constant) {
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const View vc = vview(C.box[it.q], cg, DIM), vf = vview(Fl.box[it.p], fg, DIM);
+  const double* uc_ = ce[it.q];
+  double* uf = fe[it.p];
+  const int r = Fl.ratio;
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int f[3];
+    if (!decode_box(it, (long)ch.start + t, f)) break;
+    int c0 = cdiv(f[0], r), c1 = cdiv(f[1], r), c2 = DIM == 3 ? cdiv(f[2], r) : 0;
+    int i0 = f[0] - c0 * r, i1 = f[1] - c1 * r, i2 = DIM == 3 ? f[2] - c2 * r : 0;
+    double uc = uc_[vc.at(c0, c1, c2)];
+    if (constant) {  // CONSTANT_REFINE (pdat_m4conrefineops3d.i)
+      long o = vf.at(f[0], f[1], f[2]);
+      uf[o] = uf[o] + uc;
+      continue;
+    }
+    double s0 = clin_slope(uc - uc_[vc.at(c0 - 1, c1, c2)], uc_[vc.at(c0 + 1, c1, c2)] - uc,
+                           C.dx[0]);
+    double s1 = clin_slope(uc - uc_[vc.at(c0, c1 - 1, c2)], uc_[vc.at(c0, c1 + 1, c2)] - uc,
+                           C.dx[1]);
+    double d0 = ((double)i0 + 0.5) * Fl.dx[0] - C.dx[0] * 0.5;
+    double d1 = ((double)i1 + 0.5) * Fl.dx[1] - C.dx[1] * 0.5;
+    double v;
+    if (DIM == 3) {
+      double s2 = clin_slope(uc - uc_[vc.at(c0, c1, c2 - 1)], uc_[vc.at(c0, c1, c2 + 1)] - uc,
+                             C.dx[2]);
+      double d2 = ((double)i2 + 0.5) * Fl.dx[2] - C.dx[2] * 0.5;
+      v = uc + s0 * d0 + s1 * d1 + s2 * d2;
+    } else {
+      v = uc + s0 * d0 + s1 * d1;
+    }
+    long o = vf.at(f[0], f[1], f[2]);
+    uf[o] = uf[o] + v;
+  }
+}
+
+// --------------------------------------------------------- reductions
+
+template <int DIM>
+__global__ void __launch_bounds__(kThreads)
+k_dot_partial(const BoxItem* items, const Chunk* chunks, LevelGeom L, const double* const* xx,
+              int xg, const double* const* yy, int yg, const double* const* ww,
+              double* partial) {
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const Box b = L.box[it.p];
+  const View vx = vview(b, xg, DIM), vy = vview(b, yg, DIM), vw = vview(b, 0, DIM);
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    // xx == nullptr: sum of the weights (sumControlVolumes)
+    if (xx)
+      acc += xx[it.p][vx.at(c[0], c[1], c[2])] * yy[it.p][vy.at(c[0], c[1], c[2])] *
+             ww[it.p][vw.at(c[0], c[1], c[2])];
+    else
+      acc += ww[it.p][vw.at(c[0], c[1], c[2])];
+  }
+  __shared__ double sm[kThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) s += sm[w];
+    partial[blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_sum(const double* partial, int n, double* out, int accumulate) {
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < n; t += kThreads) acc += partial[t];
+  __shared__ double sm[kThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) s += sm[w];
+    out[0] = accumulate ? out[0] + s : s;
+  }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kThreads)
+k_axpy(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* yy, int yg,
+       const double* const* xx, int xg) {
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const Box b = L.box[it.p];
+  const View vx = vview(b, xg, DIM), vy = vview(b, yg, DIM);
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    long o = vy.at(c[0], c[1], c[2]);
+    yy[it.p][o] = yy[it.p][o] + xx[it.p][vx.at(c[0], c[1], c[2])];
+  }
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) p[t] = v;
+}
+
+__global__ void k_fill(double* p, long n, double v) {
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) p[t] = v;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_set_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* dst, int g,
+            double v) {
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const View vd = vview(L.box[it.p], g, L.dim);
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    dst[it.p][vd.at(c[0], c[1], c[2])] = v;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_sweep_check(double* red, double* blk, int npatch, double tol, int* stop, int* sweeps,
+              double* maxres_out) {
+  __shared__ int done;
+  if (threadIdx.x == 0) {
+    done = *stop;
+    if (!done) {
+      double r = red[npatch - 1], b = blk[npatch - 1];
+      double mx = r > b ? r : b;  // tbox::MathUtilities<double>::Max
+      if (maxres_out) *maxres_out = mx;
+      *sweeps += 1;
+      if (!(mx > tol)) *stop = 1;
+    }
+  }
+  __syncthreads();
+  if (done) return;
+  for (int t = threadIdx.x; t < npatch; t += kThreads) {
+    red[t] = 0.0;
+    blk[t] = 0.0;
+  }
+}
+
+inline void check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw e;
+}
+
+}  // namespace
+
+void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int dg,
+                        const double* const* coarse, int cg, const Box* coarse_box,
+                        int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  hipLaunchKernelGGL(k_ghost_items, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L,
+                     dst, dg, coarse, cg, coarse_box, homogeneous, bc, stop);
+  check_launch();
+}
+
+template <int DIM, bool DVAR>
+static void rbgs_dispatch_c(const SmoothLaunch& w, const LevelGeom& L, double* const* e, int eg,
+                            const double* const* rhs, int rg, const PoissonDev& ps, int ewing,
+                            int colour, double* maxres, const int* stop, hipStream_t s) {
+  if (ps.cmode == 0)
+    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, 0>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop);
+  else if (ps.cmode == 1)
+    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, 1>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop);
+  else
+    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, 2>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop);
+}
+
+void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, int eg,
+                 const double* const* rhs, int rg, const PoissonDev& ps, int ewing, int colour,
+                 double* maxres, const int* stop, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  bool dvar = ps.D[0] != nullptr;
+  if (L.dim == 3) {
+    if (dvar) rbgs_dispatch_c<3, true>(w, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop, s);
+    else rbgs_dispatch_c<3, false>(w, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop, s);
+  } else {
+    if (dvar) rbgs_dispatch_c<2, true>(w, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop, s);
+    else rbgs_dispatch_c<2, false>(w, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop, s);
+  }
+  check_launch();
+}
+
+void launch_flux(const GhostLaunch& w, const LevelGeom& L, double* const* flux,
+                 const double* const* u, int ug, const PoissonDev& ps, int ewing,
+                 hipStream_t s) {
+  if (w.nchunks == 0) return;
+  bool dvar = ps.D[0] != nullptr;
+  if (L.dim == 3) {
+    if (dvar)
+      hipLaunchKernelGGL((k_flux<3, true>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                         w.chunks, L, flux, u, ug, ps, ewing);
+    else
+      hipLaunchKernelGGL((k_flux<3, false>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                         w.chunks, L, flux, u, ug, ps, ewing);
+  } else {
+    if (dvar)
+      hipLaunchKernelGGL((k_flux<2, true>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                         w.chunks, L, flux, u, ug, ps, ewing);
+    else
+      hipLaunchKernelGGL((k_flux<2, false>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                         w.chunks, L, flux, u, ug, ps, ewing);
+  }
+  check_launch();
+}
+
+void launch_flux_coarsen(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
+                         double* const* cflux, const double* const* foflux, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  if (coarse.dim == 3)
+    hipLaunchKernelGGL(k_flux_coarsen<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, coarse, fine, cflux, foflux);
+  else
+    hipLaunchKernelGGL(k_flux_coarsen<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, coarse, fine, cflux, foflux);
+  check_launch();
+}
+
+template <int DIM>
+static void residual_dispatch(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
+                              const double* const* f, int fg, const double* const* u, int ug,
+                              const double* const* flux, const PoissonDev& ps,
+                              double* const* oflux, hipStream_t s) {
+  if (ps.cmode == 0)
+    hipLaunchKernelGGL((k_residual<DIM, 0>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
+  else if (ps.cmode == 1)
+    hipLaunchKernelGGL((k_residual<DIM, 1>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
+  else
+    hipLaunchKernelGGL((k_residual<DIM, 2>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
+}
+
+void launch_residual(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
+                     const double* const* f, int fg, const double* const* u, int ug,
+                     const double* const* flux, const PoissonDev& ps, double* const* oflux,
+                     hipStream_t s) {
+  if (w.nchunks == 0) return;
+  if (L.dim == 3) residual_dispatch<3>(w, L, r, rg, f, fg, u, ug, flux, ps, oflux, s);
+  else residual_dispatch<2>(w, L, r, rg, f, fg, u, ug, flux, ps, oflux, s);
+  check_launch();
+}
+
+void launch_restrict(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
+                     double* const* dst, int dg, const double* const* src, int sg,
+                     hipStream_t s) {
+  if (w.nchunks == 0) return;
+  if (coarse.dim == 3)
+    hipLaunchKernelGGL(k_restrict<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks,
+                       coarse, fine, dst, dg, src, sg);
+  else
+    hipLaunchKernelGGL(k_restrict<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks,
+                       coarse, fine, dst, dg, src, sg);
+  check_launch();
+}
+
+void launch_prolong_add(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
+                        const double* const* ce, int cg, double* const* fe, int fg,
+                        int constant, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  if (coarse.dim == 3)
+    hipLaunchKernelGGL(k_prolong_add<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, coarse, fine, ce, cg, fe, fg, constant);
+  else
+    hipLaunchKernelGGL(k_prolong_add<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, coarse, fine, ce, cg, fe, fg, constant);
+  check_launch();
+}
+
+void launch_dot_partial(const GhostLaunch& w, const LevelGeom& L, const double* const* x,
+                        int xg, const double* const* y, int yg, const double* const* wts,
+                        double* partial, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  if (L.dim == 3)
+    hipLaunchKernelGGL(k_dot_partial<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, L, x, xg, y, yg, wts, partial);
+  else
+    hipLaunchKernelGGL(k_dot_partial<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+                       w.chunks, L, x, xg, y, yg, wts, partial);
+  check_launch();
+}
+
+void launch_sum(const double* partial, int n, double* out, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(kThreads), 0, s, partial, n, out, accumulate);
+  check_launch();
+}
+
+void launch_axpy(const GhostLaunch& w, const LevelGeom& L, double* const* y, int yg,
+                 const double* const* x, int xg, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  if (L.dim == 3)
+    hipLaunchKernelGGL(k_axpy<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L, y,
+                       yg, x, xg);
+  else
+    hipLaunchKernelGGL(k_axpy<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L, y,
+                       yg, x, xg);
+  check_launch();
+}
+
+void launch_fill(double* p, long n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  long blocks = (n + kThreads - 1) / kThreads;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)blocks), dim3(kThreads), 0, s, p, n, v);
+  check_launch();
+}
+
+void launch_fill_int(int* p, int n, int v, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, p, n, v);
+  check_launch();
+}
+
+void launch_set_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int g,
+                      double v, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  hipLaunchKernelGGL(k_set_items, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L,
+                     dst, g, v);
+  check_launch();
+}
+
+void launch_sweep_check(double* red, double* blk, int npatch, double tol, int* stop,
+                        int* sweeps, double* maxres_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sweep_check, dim3(1), dim3(kThreads), 0, s, red, blk, npatch, tol, stop,
+                     sweeps, maxres_out);
+  check_launch();
+}
+
+}  // namespace sfac
