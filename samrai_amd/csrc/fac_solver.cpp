@@ -226,8 +226,9 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
   const int ewing = (d_ewing && ln > d_ln_min) ? 1 : 0;
   const int np = T.geom.npatch;
   sfac::RobinDev bc = robinDev();
-  ghostFill(e, ln, false, true, true);
-  if (ln > d_ln_min) ghostFill(e, ln, true, true, true);
+  // full fills (edges and corners included), as xeqScheduleGhostFill[NoCoarse]
+  ghostFill(e, ln, false, true, false);
+  if (ln > d_ln_min) ghostFill(e, ln, true, true, false);
   double* red = tol >= 0.0 ? d_maxres[ln] : nullptr;
   double* blk = tol >= 0.0 ? d_maxres[ln] + np : nullptr;
   const int* stop = tol >= 0.0 ? d_stop : nullptr;
@@ -237,13 +238,22 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
   }
   // The reference refills same-level and physical ghosts before each
   // colour; the fill before sweep 0 repeats the one above, so it is elided.
+  // With the Ewing correction the coarse-fine face fluxes are snapshot
+  // before each colour (they read a same-colour mirrored cell in 3-D).
+  const int ew = ewing ? 2 : 0;
   for (int isweep = 0; isweep < num_sweeps; ++isweep) {
+    if (ewing)
+      sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, e.table(ln), e.ghost(), ps, stop,
+                               d_stream);
     sfac::launch_rbgs(T.smooth.smooth(), T.geom, e.table(ln), e.ghost(), r.table(ln), r.ghost(),
-                      ps, ewing, 0, red, stop, d_stream);
+                      ps, ew, 0, red, stop, d_stream);
     sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
                              nullptr, 1, bc, stop, d_stream);
+    if (ewing)
+      sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, e.table(ln), e.ghost(), ps, stop,
+                               d_stream);
     sfac::launch_rbgs(T.smooth.smooth(), T.geom, e.table(ln), e.ghost(), r.table(ln), r.ghost(),
-                      ps, ewing, 1, blk, stop, d_stream);
+                      ps, ew, 1, blk, stop, d_stream);
     sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
                              nullptr, 1, bc, stop, d_stream);
     if (tol >= 0.0 && np > 0)
@@ -326,7 +336,7 @@ void CellPoissonFACOps::prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAI
   const auto& Tc = d_h.tables(dest_ln - 1);
   // coarse values outside the coarse patch come from its same-level and
   // homogeneous physical ghosts, as RefineSchedule's coarse fill sets them
-  ghostFill(*s.data, dest_ln - 1, false, true, true);
+  ghostFill(*s.data, dest_ln - 1, false, true, false);
   sfac::launch_prolong_add(Tf.prolong_from_coarser.launch(), Tc.geom, Tf.geom,
                            s.data->table(dest_ln - 1), s.data->ghost(), d.data->table(dest_ln),
                            d.data->ghost(), d_prolongation == "CONSTANT_REFINE" ? 1 : 0,

@@ -369,9 +369,26 @@ void PatchHierarchy::buildLevel(int ln) {
     T.full_nc[ph] = uploadList(full_nc[ph]);
     T.full_wc[ph] = uploadList(full_wc[ph]);
   }
+  {
+    std::vector<BoxItem> faces;
+    for (int p = 0; p < np; ++p)
+      for (int f = 0; f < 2 * dim; ++f) {
+        if (cf_off[(size_t)p * 6 + f] < 0) continue;
+        Box layer = L.boxes[p];
+        int a = f >> 1;
+        if (f & 1) layer.lo[a] = layer.hi[a];
+        else layer.hi[a] = layer.lo[a];
+        faces.push_back(item(p, f, layer, 0));
+      }
+    T.cf_faces = uploadList(faces);
+  }
   if (cf.empty()) cf.push_back(0);
   T.d_cf = upload(cf);
   T.d_cf_off = upload(cf_off);
+  {
+    std::vector<double> zero(cf.size(), 0.0);
+    T.d_cf_flux = upload(zero);
+  }
 
   if (C) {
     std::vector<BoxItem> rs, pr, fc;
@@ -411,6 +428,7 @@ void PatchHierarchy::buildLevel(int ln) {
   for (int d = 0; d < 3; ++d) T.geom.dx[d] = L.dx[d];
   T.geom.cf = T.d_cf;
   T.geom.cf_off = T.d_cf_off;
+  T.geom.cf_flux = T.d_cf_flux;
 }
 
 }  // namespace hier

@@ -62,6 +62,8 @@ class PatchHierarchy {
     Box* d_box = nullptr;
     unsigned char* d_cf = nullptr;
     long* d_cf_off = nullptr;
+    double* d_cf_flux = nullptr;
+    DevList cf_faces;         // p patch, q face, interior layer next to a C/F face
     bool has_cf = false;
     sfac::LevelGeom geom{};
     DevList cells;            // interior of every patch

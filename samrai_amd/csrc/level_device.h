@@ -40,6 +40,9 @@ struct LevelGeom {
   // cf_off[p*6+f] indexes a byte array over the face (t1 fastest, then t2)
   const unsigned char* cf;
   const long* cf_off;
+  // Ewing-corrected flux of every coarse-fine face, same indexing as cf;
+  // written by the pre-pass of each colour, read by the colour kernel.
+  double* cf_flux;
 };
 
 // Poisson specification (PoissonSpecifications): D constant or side data,

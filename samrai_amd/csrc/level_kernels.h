@@ -30,7 +30,13 @@ void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, in
                  const double* const* rhs, int rg, const PoissonDev& ps, int ewing,
                  int colour, double* maxres, const int* stop, hipStream_t s);
 
-// Materialised flux (3 side arrays / patch, ghost 0), with the Ewing fix.
+// Snapshot the Ewing-corrected coarse-fine face fluxes into L.cf_flux
+// (items: p patch, q face, box = interior layer next to the face).
+void launch_ewing_faces(const GhostLaunch& w, const LevelGeom& L, const double* const* u, int ug,
+                        const PoissonDev& ps, const int* stop, hipStream_t s);
+
+// Materialised flux (dim side arrays per patch, ghost 0, pointer table
+// interleaved as [p * dim + axis]), with the Ewing fix.
 void launch_flux(const GhostLaunch& cells, const LevelGeom& L, double* const* flux,
                  const double* const* u, int ug, const PoissonDev& ps, int ewing,
                  hipStream_t s);

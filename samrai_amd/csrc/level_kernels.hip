@@ -36,9 +36,22 @@ __device__ inline bool cf_flag(const LevelGeom& L, int p, int face, const int* c
   return L.cf[off + idx] != 0;
 }
 
+// Index of interior cell c of face `face` in the per-face tables, or -1 when
+// the ghost across that face is not coarse-fine.
+__device__ inline long cf_index(const LevelGeom& L, int p, int face, const int* c, const Box& b) {
+  long off = L.cf_off[p * 6 + face];
+  if (off < 0) return -1;
+  int t1, t2;
+  tangential(face >> 1, L.dim, t1, t2);
+  long idx = c[t1] - b.lo[t1];
+  if (L.dim == 3) idx += (long)(b.hi[t1] - b.lo[t1] + 1) * (c[t2] - b.lo[t2]);
+  return L.cf[off + idx] ? off + idx : -1;
+}
+
 // Flux D grad u on the face of axis a with face index c (between c-e_a and
 // c), with the Ewing coarse-fine correction on flagged patch-boundary faces
-// (solv_cellpoissonfacops3d.f:10-74 and 604-1011).
+// (solv_cellpoissonfacops3d.f:10-74 and 604-1011).  ewing: 0 none, 1 compute
+// the correction, 2 read the corrected flux from L.cf_flux.
 template <int DIM, bool DVAR>
 __device__ inline double face_flux(const LevelGeom& L, int p, const Box& b, const double* u,
                                    const View& vu, const double* const* Dp, const View* vd,
@@ -47,9 +60,16 @@ __device__ inline double face_flux(const LevelGeom& L, int p, const Box& b, cons
   int c[3] = {i, j, k};
   int m[3] = {i, j, k};
   m[a] -= 1;
+  if (ewing == 2) {
+    int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
+    if (side >= 0) {
+      long x = cf_index(L, p, 2 * a + side, side == 0 ? c : m, b);
+      if (x >= 0) return L.cf_flux[x];
+    }
+  }
   double D = DVAR ? Dp[a][vd[a].at(i, j, k)] : dconst;
   double F = dai * D * (u[vu.at(i, j, k)] - u[vu.at(m[0], m[1], m[2])]);
-  if (ewing) {
+  if (ewing == 1) {
     int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
     if (side >= 0) {
       const int* in = side == 0 ? c : m;
@@ -265,7 +285,8 @@ k_flux(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* fl
     }
   }
   const double* u = uu[p];
-  double* F[3] = {flux[p * 3 + 0], flux[p * 3 + 1], DIM == 3 ? flux[p * 3 + 2] : nullptr};
+  // interleaved side-data table: [p * DIM + axis]
+  double* F[3] = {flux[p * DIM + 0], flux[p * DIM + 1], DIM == 3 ? flux[p * DIM + 2] : nullptr};
   double di[3] = {1.0 / L.dx[0], 1.0 / L.dx[1], DIM == 3 ? 1.0 / L.dx[2] : 0.0};
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
     int c[3];
@@ -280,6 +301,43 @@ k_flux(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* fl
             L, p, b, u, vu, Dp, vd, ps.dconst, a, h[0], h[1], h[2], di[a], ewing);
       }
     }
+  }
+}
+
+// Snapshot of the Ewing-corrected flux on every coarse-fine face before a
+// colour pass.  The reference computes all fluxes of a patch before the
+// colour update (CellPoissonFACOps.C:1392-1428); the corrected flux reads the
+// tangentially mirrored interior cell, which in 3-D has the same colour, so
+// it must be taken from the pre-pass state.  items: p patch, q face
+// (2*axis+side), box = the interior cell layer next to that face.
+template <int DIM, bool DVAR>
+__global__ void __launch_bounds__(kThreads)
+k_ewing_faces(const BoxItem* items, const Chunk* chunks, LevelGeom L, const double* const* uu,
+              int ug, PoissonDev ps, const int* stop) {
+  if (stop && *stop) return;
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const int p = it.p, face = it.q, a = face >> 1, side = face & 1;
+  const Box b = L.box[p];
+  const View vu = vview(b, ug, DIM);
+  View vd[3];
+  const double* Dp[3] = {nullptr, nullptr, nullptr};
+  if (DVAR) {
+    for (int d = 0; d < DIM; ++d) {
+      vd[d] = vview(b, ps.dghost, DIM, d);
+      Dp[d] = ps.D[d][p];
+    }
+  }
+  const double dai = 1.0 / L.dx[a];
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    long x = cf_index(L, p, face, c, b);
+    if (x < 0) continue;
+    int f[3] = {c[0], c[1], c[2]};
+    f[a] += side;
+    L.cf_flux[x] = face_flux<DIM, DVAR>(L, p, b, uu[p], vu, Dp, vd, ps.dconst, a, f[0], f[1],
+                                        f[2], dai, 1);
   }
 }
 
@@ -305,7 +363,8 @@ k_residual(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const
   const View vc = vview(b, ps.cghost, DIM);
   View vf[3];
   for (int a = 0; a < DIM; ++a) vf[a] = vview(b, 0, DIM, a);
-  const double* F[3] = {flux[p * 3 + 0], flux[p * 3 + 1], DIM == 3 ? flux[p * 3 + 2] : nullptr};
+  const double* F[3] = {flux[p * DIM + 0], flux[p * DIM + 1],
+                        DIM == 3 ? flux[p * DIM + 2] : nullptr};
   const double dxi = 1.0 / L.dx[0], dyi = 1.0 / L.dx[1];
   const double dzi = DIM == 3 ? 1.0 / L.dx[2] : 0.0;
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
@@ -375,7 +434,7 @@ k_flux_coarsen(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom
         if (DIM == 3) idx += n1 * (f[t2] - fb.lo[t2]);
         acc = acc + src[idx] * areaf;
       }
-    cflux[it.p * 3 + a][vf.at(c[0], c[1], c[2])] = acc / areac;
+    cflux[it.p * DIM + a][vf.at(c[0], c[1], c[2])] = acc / areac;
   }
 }
 
@@ -626,6 +685,28 @@ void launch_flux(const GhostLaunch& w, const LevelGeom& L, double* const* flux,
     else
       hipLaunchKernelGGL((k_flux<2, false>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
                          w.chunks, L, flux, u, ug, ps, ewing);
+  }
+  check_launch();
+}
+
+void launch_ewing_faces(const GhostLaunch& w, const LevelGeom& L, const double* const* u, int ug,
+                        const PoissonDev& ps, const int* stop, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  bool dvar = ps.D[0] != nullptr;
+  if (L.dim == 3) {
+    if (dvar)
+      hipLaunchKernelGGL((k_ewing_faces<3, true>), dim3(w.nchunks), dim3(kThreads), 0, s,
+                         w.items, w.chunks, L, u, ug, ps, stop);
+    else
+      hipLaunchKernelGGL((k_ewing_faces<3, false>), dim3(w.nchunks), dim3(kThreads), 0, s,
+                         w.items, w.chunks, L, u, ug, ps, stop);
+  } else {
+    if (dvar)
+      hipLaunchKernelGGL((k_ewing_faces<2, true>), dim3(w.nchunks), dim3(kThreads), 0, s,
+                         w.items, w.chunks, L, u, ug, ps, stop);
+    else
+      hipLaunchKernelGGL((k_ewing_faces<2, false>), dim3(w.nchunks), dim3(kThreads), 0, s,
+                         w.items, w.chunks, L, u, ug, ps, stop);
   }
   check_launch();
 }

@@ -59,7 +59,8 @@ def test_smooth_with_tolerance_matches_sweep_count(gpu_lib, mk):
     pr = _pair(mk, SPECS[0])
     # a tolerance reached after a few sweeps: the early exit uses the last
     # patch's max residual of each colour (CellPoissonFACOps.C:1432-1507)
-    n_o, mr_o = pr.o.smooth("u", "f", 0, 1)
+    n_o, mr_o = pr.o.smooth("u", "f", 0, 1, 0.0)  # tol >= 0: maxres is computed
+    assert mr_o > 0
     tol = mr_o * 0.3
     pr2 = _pair(mk, SPECS[0])
     n_g, mr_g = pr2.s.smooth(pr2.u, pr2.f, 0, 200, tol)
