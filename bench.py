@@ -1,0 +1,186 @@
+#!/usr/bin/env python3
+"""FAC Poisson benchmark on one MI355X.
+
+A step is one CellPoissonFACSolver::solveSystem from u = 0 to a relative
+residual of 1e-10 (RMS residual <= 1e-10 * RMS f, FP64) on the 3-D deck:
+unit cube, 128^3 base in 64^3 patches, three levels at ratio 2 refined where
+|grad u| exceeds the 70th percentile, boxes of 16^3..64^3; D = 1,
+u = sin(pi x) sin(pi y) sin(pi z), Dirichlet 0; red-black GS 2+2 sweeps,
+coarsest level red-black (500 sweeps, tolerance 1e-12), Ewing coarse-fine
+fluxes, conservative linear prolongation.
+
+Prints one JSON line: time to solution (value, s), smoother throughput, the
+roofline of the red/black colour kernel measured with HIP events on the
+solver's stream over the timed steps, and the CPU baseline (the C oracle, a
+bounded sample of FAC cycles on the same deck, projected to the cycle count
+the GPU needed).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+PARAMS = dict(max_cycles=200, residual_tol=1e-300, relative_residual_tol=1e-10,
+              num_pre_sweeps=2, num_post_sweeps=2, coarse_solver_choice="redblack",
+              coarse_solver_tolerance=1e-12, coarse_solver_max_iterations=500,
+              cf_discretization="Ewing", prolongation_method="CONSERVATIVE_LINEAR_REFINE")
+
+
+def build_problem(fac, decks, deck):
+    h = fac.PatchHierarchy(deck["dim"], deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
+    for boxes in deck["levels"]:
+        h.add_level(deck["ratio"], boxes)
+    h.finalize()
+    u = fac.PatchData(h, 1)
+    f = fac.PatchData(h, 0)
+    for ln, boxes in enumerate(deck["levels"]):
+        for p, box in enumerate(boxes):
+            f.put(ln, p, decks.sine_rhs(deck, ln, box))
+    s = fac.CellPoissonFACSolver(h, **PARAMS)
+    s.set_d_constant(1.0)
+    s.set_dirichlet(0.0)
+    s.initialize_solver_state(u, f)
+    return h, u, f, s
+
+
+def max_error(decks, deck, u):
+    err = 0.0
+    for ln, boxes in enumerate(deck["levels"]):
+        for p, box in enumerate(boxes):
+            shp = tuple(h - l + 3 for l, h in zip(*box))
+            uu = u.get(ln, p).reshape(shp, order="F")[1:-1, 1:-1, 1:-1]
+            err = max(err, float(np.max(np.abs(uu - decks.sine_exact(deck, ln, box)))))
+    return err
+
+
+def cpu_baseline(decks, deck, cycles_gpu, ncycles, threads):
+    from oracle.oracle import OracleFAC
+    o = OracleFAC(3, deck["levels"], deck["ratio"], deck["dlo"], deck["dhi"], [0.0] * 3,
+                  [1.0] * 3)
+    o.set_poisson(False, 1.0, 0, 0.0)
+    o.set_bc([1.0] * 6, [0.0] * 6, [0.0] * 6)
+    o.set_params(max_cycles=ncycles, residual_tol=1e-300, relative_residual_tol=1e-10,
+                 pre=2, post=2, coarse_max_iterations=500, coarse_tol=1e-12, ewing=True,
+                 nthreads=threads)
+    for ln, boxes in enumerate(deck["levels"]):
+        for p, box in enumerate(boxes):
+            o.put("f", ln, p, decks.sine_rhs(deck, ln, box))
+    conv, it, res, hist = o.solve()
+    secs = o.last_solve_seconds()
+    o.close()
+    per_cycle = secs / max(it, 1)
+    return dict(value=per_cycle * cycles_gpu, unit="s", cores=threads, kind="port",
+                sample="%d FAC cycles of the same deck with the C oracle on %d host threads "
+                       "(%.2f s, %.3f s/cycle), projected to the %d cycles the GPU solve took"
+                       % (it, threads, secs, per_cycle, cycles_gpu))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cycles", type=int, default=2)
+    ap.add_argument("--smoother-sweeps", type=int, default=200)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from samrai_amd import decks, fac
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    deck = decks.deck_3d_c1()
+    ncells = decks.count_cells(deck)
+    npatch = sum(len(l) for l in deck["levels"])
+    print("bench: deck %s, %d patches, %d cells; building" % (deck["name"], npatch, ncells),
+          file=sys.stderr, flush=True)
+    h, u, f, s = build_problem(fac, decks, deck)
+
+    for w in range(args.warmup):
+        u.fill(0.0)
+        s.solve_system(u, f)
+        print("bench: warmup %d: %d cycles, %.3f s" % (w, s.number_of_iterations(),
+                                                        s.last_solve_seconds()),
+              file=sys.stderr, flush=True)
+
+    s.enable_kernel_timing(True)
+    converged = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        u.fill(0.0)
+        converged.append(s.solve_system(u, f))
+    s.synchronize()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    launches, kms, kbytes = s.kernel_timing(-1)
+    l0 = s.kernel_timing(0)
+    s.enable_kernel_timing(False)
+    cycles = s.number_of_iterations()
+    hist = s.history()
+    ms_per_step = elapsed * 1e3 / args.steps
+    print("bench: %d steps, %.3f s/step, %d cycles, final residual %.3e" %
+          (args.steps, ms_per_step / 1e3, cycles, hist[-1]), file=sys.stderr, flush=True)
+    err = max_error(decks, deck, u)
+
+    # smoother throughput on the coarsest level (the dominant operation)
+    e = s.internal_vector("e")
+    r = s.internal_vector("r")
+    n0 = sum(int(np.prod([hh - ll + 1 for ll, hh in zip(*b)])) for b in deck["levels"][0])
+    s.synchronize()
+    t1 = time.perf_counter()
+    s.smooth(e, r, 0, args.smoother_sweeps)
+    s.synchronize()
+    sm = time.perf_counter() - t1
+    smoother_gcells = n0 * args.smoother_sweeps / sm / 1e9
+
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    kernel="k_rbgs (red/black colour pass, all levels)",
+                    launches=launches, avg_launch_us=round(kms * 1e3 / max(launches, 1), 3),
+                    level0_avg_launch_us=round(l0[1] * 1e3 / max(l0[0], 1), 3),
+                    level0_achieved=round(l0[2] / (l0[1] * 1e-3) / 1e9, 1) if l0[1] else 0.0)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            threads = min(16, len(os.sched_getaffinity(0)))
+        except AttributeError:
+            threads = min(16, os.cpu_count() or 1)
+        print("bench: CPU baseline (%d cycles, %d threads)" % (args.cpu_cycles, threads),
+              file=sys.stderr, flush=True)
+        cpu = cpu_baseline(decks, deck, cycles, args.cpu_cycles, threads)
+
+    out = dict(metric="time to 1e-10 rel residual (s)", value=round(ms_per_step / 1e3, 4),
+               unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
+               ms_per_step=round(ms_per_step, 3), higher_is_better=False, scaling="strong",
+               vs_baseline=None, dtype="f64", data="synthetic",
+               config=dict(workload="%s: 3-D unit cube, 128^3 base, 3 levels ratio 2, "
+                                    "|grad u| > 70th pct, boxes 16^3..64^3, D=1, sine RHS"
+                                    % deck["name"],
+                           patches=npatch, cells=ncells, levels=len(deck["levels"]),
+                           solver="FAC V-cycle, RBGS 2+2, coarse redblack 500 sweeps tol "
+                                  "1e-12, Ewing, conservative linear refine, rel tol 1e-10"),
+               cycles=cycles, converged=all(converged), final_residual=float(hist[-1]),
+               max_error_vs_exact=err,
+               smoother_gcells_per_s=round(smoother_gcells, 2),
+               roofline=roofline, cpu_baseline=cpu)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -334,6 +334,21 @@ int sfac_solver_synchronize(sfac_solver* s) {
   });
 }
 
+int sfac_solver_enable_kernel_timing(sfac_solver* s, int on) {
+  return guarded([&] {
+    need(s, "solver");
+    s->s->ops().enableKernelTiming(on != 0);
+  });
+}
+
+int sfac_solver_kernel_timing(sfac_solver* s, int level, long* launches, double* ms,
+                              double* bytes) {
+  return guarded([&] {
+    need(s, "solver");
+    s->s->ops().kernelTiming(level, launches, ms, bytes);
+  });
+}
+
 int sfac_ops_internal_vector(sfac_solver* s, int which, sfac_patch_data** out) {
   return guarded([&] {
     need_init(s);

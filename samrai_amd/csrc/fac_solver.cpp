@@ -89,6 +89,7 @@ CellPoissonFACOps::CellPoissonFACOps(hier::PatchHierarchy& h, hipStream_t stream
 
 CellPoissonFACOps::~CellPoissonFACOps() {
   deallocateOperatorState();
+  for (hipEvent_t e : d_event_pool) (void)hipEventDestroy(e);
   (void)hipFree(d_stop);
   (void)hipFree(d_scalars);
   (void)hipHostFree(h_pinned);
@@ -241,19 +242,34 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
   // With the Ewing correction the coarse-fine face fluxes are snapshot
   // before each colour (they read a same-colour mirrored cell in 3-D).
   const int ew = ewing ? 2 : 0;
+  const double launch_bytes = (d_spec.dIsConstant() ? 12.0 : 24.0) * (double)T.ncells;
+  auto colour_pass = [&](int colour, double* mr) {
+    TimedLaunch tl{};
+    if (d_timing) {
+      tl.start = nextEvent();
+      hip_check(hipEventRecord(tl.start, d_stream), "hipEventRecord");
+    }
+    sfac::launch_rbgs(T.smooth.smooth(), T.geom, e.table(ln), e.ghost(), r.table(ln), r.ghost(),
+                      ps, ew, colour, mr, stop, d_stream);
+    if (d_timing) {
+      tl.stop = nextEvent();
+      hip_check(hipEventRecord(tl.stop, d_stream), "hipEventRecord");
+      tl.level = ln;
+      tl.bytes = launch_bytes;
+      d_timed.push_back(tl);
+    }
+  };
   for (int isweep = 0; isweep < num_sweeps; ++isweep) {
     if (ewing)
       sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, e.table(ln), e.ghost(), ps, stop,
                                d_stream);
-    sfac::launch_rbgs(T.smooth.smooth(), T.geom, e.table(ln), e.ghost(), r.table(ln), r.ghost(),
-                      ps, ew, 0, red, stop, d_stream);
+    colour_pass(0, red);
     sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
                              nullptr, 1, bc, stop, d_stream);
     if (ewing)
       sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, e.table(ln), e.ghost(), ps, stop,
                                d_stream);
-    sfac::launch_rbgs(T.smooth.smooth(), T.geom, e.table(ln), e.ghost(), r.table(ln), r.ghost(),
-                      ps, ew, 1, blk, stop, d_stream);
+    colour_pass(1, blk);
     sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
                              nullptr, 1, bc, stop, d_stream);
     if (tol >= 0.0 && np > 0)
@@ -394,6 +410,38 @@ void CellPoissonFACOps::add(PatchDataSet& y, const PatchDataSet& x, int coarse_l
 
 void CellPoissonFACOps::postprocessOneCycle(int, const SAMRAIVectorReal&,
                                             const SAMRAIVectorReal&) {}
+
+hipEvent_t CellPoissonFACOps::nextEvent() {
+  if (d_event_next == d_event_pool.size()) {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    d_event_pool.push_back(e);
+  }
+  return d_event_pool[d_event_next++];
+}
+
+void CellPoissonFACOps::enableKernelTiming(bool on) {
+  d_timing = on;
+  d_timed.clear();
+  d_event_next = 0;
+}
+
+void CellPoissonFACOps::kernelTiming(int level, long* launches, double* ms, double* bytes) {
+  hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
+  long n = 0;
+  double t = 0.0, b = 0.0;
+  for (const TimedLaunch& tl : d_timed) {
+    if (level >= 0 && tl.level != level) continue;
+    float e = 0.0f;
+    hip_check(hipEventElapsedTime(&e, tl.start, tl.stop), "hipEventElapsedTime");
+    ++n;
+    t += e;
+    b += tl.bytes;
+  }
+  if (launches) *launches = n;
+  if (ms) *ms = t;
+  if (bytes) *bytes = b;
+}
 
 }  // namespace solv
 }  // namespace samrai_amd

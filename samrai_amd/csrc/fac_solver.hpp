@@ -173,7 +173,25 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   hipStream_t stream() const { return d_stream; }
   pdat::PatchDataSet& flux() { return *d_flux; }
 
+  // HIP-event timing of every red/black colour launch (on this stream),
+  // with the algorithmic bytes of each launch: 12 B per cell of the level
+  // for constant D (u read, f read, u write per full sweep = 24 B per cell),
+  // 24 B with variable D (+ three face coefficients).
+  void enableKernelTiming(bool on);
+  // launches, summed duration (ms) and algorithmic bytes; level < 0: all.
+  void kernelTiming(int level, long* launches, double* ms, double* bytes);
+
  private:
+  struct TimedLaunch {
+    hipEvent_t start, stop;
+    int level;
+    double bytes;
+  };
+  bool d_timing = false;
+  std::vector<TimedLaunch> d_timed;
+  std::vector<hipEvent_t> d_event_pool;
+  size_t d_event_next = 0;
+  hipEvent_t nextEvent();
   sfac::PoissonDev poissonDev(int ln) const;
   sfac::RobinDev robinDev() const;
   hier::PatchHierarchy& d_h;

@@ -74,6 +74,8 @@ def lib():
     L.sfac_solver_get_history.argtypes = [_vp, _dp, C.c_int, _ip]
     L.sfac_solver_last_solve_seconds.argtypes = [_vp, _dp]
     L.sfac_solver_synchronize.argtypes = [_vp]
+    L.sfac_solver_enable_kernel_timing.argtypes = [_vp, C.c_int]
+    L.sfac_solver_kernel_timing.argtypes = [_vp, C.c_int, C.POINTER(C.c_long), _dp, _dp]
     L.sfac_ops_internal_vector.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
     L.sfac_ops_ghost_fill.argtypes = [_vp, _vp, C.c_int, C.c_int, C.c_int]
     L.sfac_ops_smooth.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp]
@@ -290,6 +292,20 @@ class CellPoissonFACSolver:
 
     def synchronize(self):
         _check(lib().sfac_solver_synchronize(self.s))
+
+    def enable_kernel_timing(self, on=True):
+        """HIP-event timing of the red/black colour launches (solver stream)."""
+        _check(lib().sfac_solver_enable_kernel_timing(self.s, int(on)))
+
+    def kernel_timing(self, level=-1):
+        """(launches, total kernel ms, algorithmic bytes) of the timed colour
+        launches on `level` (all levels when level < 0)."""
+        n = C.c_long()
+        ms = C.c_double()
+        b = C.c_double()
+        _check(lib().sfac_solver_kernel_timing(self.s, level, C.byref(n), C.byref(ms),
+                                               C.byref(b)))
+        return n.value, ms.value, b.value
 
     # FACOperatorStrategy operations (after initialize_solver_state)
     def internal_vector(self, which):
