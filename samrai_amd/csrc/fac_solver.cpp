@@ -180,8 +180,9 @@ void CellPoissonFACOps::initializeOperatorState(const SAMRAIVectorReal& u,
       hip_check(hipMemcpy(d_oflux_tab[ln], tab.data(), sizeof(double*) * tab.size(),
                           hipMemcpyHostToDevice),
                 "hipMemcpy(oflux tab)");
-    hip_check(hipMalloc(&d_maxres[ln], sizeof(double) * (2 * np + 2)), "hipMalloc(maxres)");
-    hip_check(hipMemset(d_maxres[ln], 0, sizeof(double) * (2 * np + 2)), "hipMemset");
+    const long nsc = d_h.tables(ln).smooth.nchunks;
+    hip_check(hipMalloc(&d_maxres[ln], sizeof(double) * (2 * nsc + 2)), "hipMalloc(maxres)");
+    hip_check(hipMemset(d_maxres[ln], 0, sizeof(double) * (2 * nsc + 2)), "hipMemset");
     maxchunks = std::max(maxchunks, (long)d_h.tables(ln).cells.nchunks);
   }
   d_partial_n = maxchunks;
@@ -230,12 +231,21 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
   // full fills (edges and corners included), as xeqScheduleGhostFill[NoCoarse]
   ghostFill(e, ln, false, true, false);
   if (ln > d_ln_min) ghostFill(e, ln, true, true, false);
+  const int nsc = T.smooth.nchunks;
   double* red = tol >= 0.0 ? d_maxres[ln] : nullptr;
-  double* blk = tol >= 0.0 ? d_maxres[ln] + np : nullptr;
+  double* blk = tol >= 0.0 ? d_maxres[ln] + nsc : nullptr;
   const int* stop = tol >= 0.0 ? d_stop : nullptr;
+  sfac::SweepCheck chk{};
   if (tol >= 0.0) {
-    sfac::launch_fill(d_maxres[ln], 2 * np, 0.0, d_stream);
     sfac::launch_fill_int(d_stop, 2, 0, d_stream);
+    chk.red = red;
+    chk.blk = blk;
+    chk.c0 = np > 0 ? T.smooth.first_chunk[np - 1] : 0;
+    chk.c1 = nsc;
+    chk.tol = tol;
+    chk.stop = d_stop;
+    chk.sweeps = d_stop + 1;
+    chk.maxres_out = d_scalars + 1;
   }
   // The reference refills same-level and physical ghosts before each
   // colour; the fill before sweep 0 repeats the one above, so it is elided.
@@ -270,10 +280,10 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
       sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, e.table(ln), e.ghost(), ps, stop,
                                d_stream);
     colour_pass(1, blk);
+    // the fill after black also runs the convergence test (work-group 0)
     sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
-                             nullptr, 1, bc, stop, d_stream);
-    if (tol >= 0.0 && np > 0)
-      sfac::launch_sweep_check(red, blk, np, tol, d_stop, d_stop + 1, d_scalars + 1, d_stream);
+                             nullptr, 1, bc, stop, d_stream,
+                             (tol >= 0.0 && np > 0) ? &chk : nullptr);
   }
   // edges/corners too, as the reference's last NoCoarse fill does
   ghostFill(e, ln, false, true, false);

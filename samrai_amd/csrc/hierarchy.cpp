@@ -185,10 +185,18 @@ DevList PatchHierarchy::uploadList(const std::vector<BoxItem>& items, bool colou
   std::vector<Chunk> chunks;
   for (size_t i = 0; i < items.size(); ++i) {
     const BoxItem& it = items[i];
+    L.first_chunk.push_back((int)chunks.size());
     long n;
     if (colour_space) {
+      // chunks of whole rows (j,k): a row's colour cells sit on consecutive
+      // lanes; rows per chunk as the colour kernel computes it
       int nx = it.hi[0] - it.lo[0] + 1;
-      n = (long)((nx + 1) / 2) * (it.hi[1] - it.lo[1] + 1) * (it.hi[2] - it.lo[2] + 1);
+      int half = (nx + 1) / 2;
+      if (half > sfac::kThreads) fail("PatchHierarchy: patch wider than 512 cells in x");
+      int rpc = std::max(1, sfac::kCellsPerChunk / half);
+      long rows = (long)(it.hi[1] - it.lo[1] + 1) * (it.hi[2] - it.lo[2] + 1);
+      for (long s = 0; s < rows; s += rpc) chunks.push_back(Chunk{(int)i, (int)s});
+      continue;
     } else {
       n = (long)(it.hi[0] - it.lo[0] + 1) * (it.hi[1] - it.lo[1] + 1) *
           (it.hi[2] - it.lo[2] + 1);

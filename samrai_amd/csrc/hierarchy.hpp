@@ -28,6 +28,7 @@ struct DevList {
   Chunk* chunks = nullptr;
   int nitems = 0;
   int nchunks = 0;
+  std::vector<int> first_chunk;  // host copy: first chunk of each item
   sfac::GhostLaunch launch() const { return {items, chunks, nchunks}; }
   sfac::SmoothLaunch smooth() const { return {items, chunks, nchunks}; }
 };

@@ -14,10 +14,26 @@ struct GhostLaunch {
   int nchunks;
 };
 
-// Fill ghost cells of a level vector from a ghost work list (one phase).
+// Coarse-solve convergence test data (see sweep_check in level_kernels.hip):
+// per-chunk max |residual| slots of the red and black passes, the chunk
+// range [c0, c1) of the level's last patch, the tolerance, and the device
+// stop flag / sweep counter / last max residual.
+struct SweepCheck {
+  const double* red;
+  const double* blk;
+  int c0, c1;
+  double tol;
+  int* stop;
+  int* sweeps;
+  double* maxres_out;
+};
+
+// Fill ghost cells of a level vector from a ghost work list (one phase);
+// with `check`, work-group 0 also runs the convergence test first.
 void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int dg,
                         const double* const* coarse, int cg, const Box* coarse_box,
-                        int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s);
+                        int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s,
+                        const SweepCheck* check = nullptr);
 
 // One red (colour 0) or black (colour 1) Gauss-Seidel pass with the flux
 // (and Ewing coarse-fine correction) computed on the fly.
@@ -26,6 +42,7 @@ struct SmoothLaunch {
   const Chunk* chunks;   // chunks over (half-row, row) colour space
   int nchunks;
 };
+// maxres (optional): one slot per chunk of w receives that chunk's max |residual|.
 void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, int eg,
                  const double* const* rhs, int rg, const PoissonDev& ps, int ewing,
                  int colour, double* maxres, const int* stop, hipStream_t s);
@@ -80,9 +97,7 @@ void launch_fill_int(int* p, int n, int v, hipStream_t s);
 // Set the cells of each item's box in patch item.p to v (dst ghost width g).
 void launch_set_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int g,
                       double v, hipStream_t s);
-// Coarse-solve convergence test (smoothErrorByRedBlack): not_converged =
-// max(red[last], black[last]) > tol; counts sweeps; clears the maxres slots.
-void launch_sweep_check(double* red, double* blk, int npatch, double tol, int* stop,
-                        int* sweeps, double* maxres_out, hipStream_t s);
+// The convergence test alone (levels without same-level ghost work).
+void launch_sweep_check(const SweepCheck& chk, hipStream_t s);
 
 }  // namespace sfac

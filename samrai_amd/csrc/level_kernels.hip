@@ -100,13 +100,31 @@ __device__ inline void atomic_max_nonneg(double* p, double v) {
             (unsigned long long)__double_as_longlong(v));
 }
 
+// Coarse-solve convergence test of smoothErrorByRedBlack
+// (CellPoissonFACOps.C:1486-1508): red and black keep only the last patch's
+// max |residual| (redOrBlackSmoothingOnPatch overwrites it per patch), here
+// the max over the last patch's chunk slots.
+__device__ inline void sweep_check(const SweepCheck& k) {
+  if (*k.stop) return;
+  double r = 0.0, b = 0.0;
+  for (int c = k.c0; c < k.c1; ++c) {
+    r = fmax(r, k.red[c]);
+    b = fmax(b, k.blk[c]);
+  }
+  double mx = r > b ? r : b;  // tbox::MathUtilities<double>::Max
+  if (k.maxres_out) *k.maxres_out = mx;
+  *k.sweeps += 1;
+  if (!(mx > k.tol)) *k.stop = 1;
+}
+
 // ------------------------------------------------------------ ghost fill
 
 __global__ void __launch_bounds__(kThreads)
 k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* dst, int dg,
               const double* const* coarse, int cg, const Box* cbox, int homogeneous,
-              RobinDev bc, const int* stop) {
+              RobinDev bc, const int* stop, SweepCheck chk) {
   if (stop && *stop) return;
+  if (chk.stop && blockIdx.x == 0 && threadIdx.x == 0) sweep_check(chk);
   const Chunk ch = chunks[blockIdx.x];
   const BoxItem it = items[ch.item];
   const int kind = it.tag & 15, loc = it.tag >> 4;
@@ -179,88 +197,132 @@ k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* co
 
 // ----------------------------------------------------------------- RBGS
 
-template <int DIM, bool DVAR, int CMODE>
+// One colour of red-black Gauss-Seidel on every patch of a level
+// (rbgswithfluxmax*, solv_cellpoissonfacops3d.f:136-481, with the fluxes of
+// compflux*, :10-133, formed on the fly).  Work-group = one chunk of rows
+// (j,k) of one patch; the colour cells of a row lie on consecutive lanes, so
+// neighbour loads are fixed offsets from one 32-bit row base.  EWING: the
+// coarse-fine face fluxes come from the pre-pass snapshot L.cf_flux.
+template <int DIM, bool DVAR, int CMODE, bool EWING>
 __global__ void __launch_bounds__(kThreads)
 k_rbgs(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* e, int eg,
-       const double* const* rhs, int rg, PoissonDev ps, int ewing, int colour, double* maxres,
+       const double* const* rhs, int rg, PoissonDev ps, int colour, double* maxres,
        const int* stop) {
   if (stop && *stop) return;
   const Chunk ch = chunks[blockIdx.x];
   const int p = items[ch.item].p;
   const Box b = L.box[p];
-  const View vu = vview(b, eg, DIM), vr = vview(b, rg, DIM);
-  View vd[3];
-  const double* Dp[3] = {nullptr, nullptr, nullptr};
-  if (DVAR) {
-    for (int a = 0; a < DIM; ++a) {
-      vd[a] = vview(b, ps.dghost, DIM, a);
-      Dp[a] = ps.D[a][p];
-    }
-  }
-  View vc = vview(b, ps.cghost, DIM);
-  const double* Cp = CMODE == 2 ? ps.C[p] : nullptr;
-  double* u = e[p];
-  const double* f = rhs[p];
   const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
+  const int nz = DIM == 3 ? b.hi[2] - b.lo[2] + 1 : 1;
   const int half = (nx + 1) >> 1;
+  const int rows_per_iter = kThreads / half;  // half <= kThreads (checked on the host)
+  const int nrow_total = ny * nz;
+  const int rpc = kCellsPerChunk / half > 0 ? kCellsPerChunk / half : 1;
+  const int row_end = min(ch.start + rpc, nrow_total);
+  // array strides (ghost widths eg, rg; side arrays get +1 on their axis)
+  const int us1 = nx + 2 * eg, us2 = us1 * (ny + 2 * eg);
+  const int fs1 = nx + 2 * rg, fs2 = fs1 * (ny + 2 * rg);
+  const int dgw = ps.dghost;
+  const int dxs1 = nx + 1 + 2 * dgw, dxs2 = dxs1 * (ny + 2 * dgw);
+  const int dys1 = nx + 2 * dgw, dys2 = dys1 * (ny + 1 + 2 * dgw);
+  const int dzs1 = nx + 2 * dgw, dzs2 = dzs1 * (ny + 2 * dgw);
+  const int cs1 = nx + 2 * ps.cghost, cs2 = cs1 * (ny + 2 * ps.cghost);
+  double* __restrict__ u = e[p];
+  const double* __restrict__ f = rhs[p];
+  const double* __restrict__ Dx = DVAR ? ps.D[0][p] : nullptr;
+  const double* __restrict__ Dy = DVAR ? ps.D[1][p] : nullptr;
+  const double* __restrict__ Dz = (DVAR && DIM == 3) ? ps.D[2][p] : nullptr;
+  const double* __restrict__ Cp = CMODE == 2 ? ps.C[p] : nullptr;
   const double dxi = 1.0 / L.dx[0], dyi = 1.0 / L.dx[1];
   const double dzi = DIM == 3 ? 1.0 / L.dx[2] : 0.0;
+  const double dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
+  const double dc = ps.dconst;
+  const int m = threadIdx.x % half;
   double mr = 0.0;
-  const long total = (long)half * ny * (DIM == 3 ? (b.hi[2] - b.lo[2] + 1) : 1);
-  for (int t0 = threadIdx.x; t0 < kCellsPerChunk; t0 += kThreads) {
-    long t = (long)ch.start + t0;
-    if (t >= total) break;
-    int m = (int)(t % half);
-    long row = t / half;
-    int j = b.lo[1] + (int)(row % ny);
-    int k = DIM == 3 ? b.lo[2] + (int)(row / ny) : 0;
-    int par = DIM == 3 ? fmod2(b.lo[0] + j + k) : fmod2(b.lo[1] + j);
-    int i = b.lo[0] + (par != colour ? 1 : 0) + 2 * m;
-    if (i > b.hi[0]) continue;
-    double fxl = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 0, i, j, k, dxi, ewing);
-    double fxh =
-        face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 0, i + 1, j, k, dxi, ewing);
-    double fyl = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 1, i, j, k, dyi, ewing);
-    double fyh =
-        face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 1, i, j + 1, k, dyi, ewing);
-    double scal = CMODE == 2 ? Cp[vc.at(i, j, k)] : (CMODE == 1 ? ps.cconst : 0.0);
-    double uc = u[vu.at(i, j, k)];
+  for (int row = ch.start + (int)threadIdx.x / half; row < row_end && threadIdx.x < rows_per_iter * half;
+       row += rows_per_iter) {
+    const int jj = row % ny, kk = DIM == 3 ? row / ny : 0;
+    const int j = b.lo[1] + jj, k = DIM == 3 ? b.lo[2] + kk : 0;
+    const int par = DIM == 3 ? fmod2(b.lo[0] + j + k) : fmod2(b.lo[1] + j);
+    const int ii = (par != colour ? 1 : 0) + 2 * m;
+    if (ii >= nx) continue;
+    const int i = b.lo[0] + ii;
+    const int o = (ii + eg) + us1 * (jj + eg) + us2 * (DIM == 3 ? kk + eg : 0);
+    const double uc = u[o];
+    // face fluxes: x at i and i+1, y at j and j+1, z at k and k+1
+    double Dxl, Dxh, Dyl, Dyh, Dzl = 0.0, Dzh = 0.0;
+    if (DVAR) {
+      const int ox = (ii + dgw) + dxs1 * (jj + dgw) + dxs2 * (DIM == 3 ? kk + dgw : 0);
+      const int oy = (ii + dgw) + dys1 * (jj + dgw) + dys2 * (DIM == 3 ? kk + dgw : 0);
+      Dxl = Dx[ox];
+      Dxh = Dx[ox + 1];
+      Dyl = Dy[oy];
+      Dyh = Dy[oy + dys1];
+      if (DIM == 3) {
+        const int oz = (ii + dgw) + dzs1 * (jj + dgw) + dzs2 * (kk + dgw);
+        Dzl = Dz[oz];
+        Dzh = Dz[oz + dzs2];
+      }
+    } else {
+      Dxl = Dxh = Dyl = Dyh = Dzl = Dzh = dc;
+    }
+    double fxl = dxi * Dxl * (uc - u[o - 1]);
+    double fxh = dxi * Dxh * (u[o + 1] - uc);
+    double fyl = dyi * Dyl * (uc - u[o - us1]);
+    double fyh = dyi * Dyh * (u[o + us1] - uc);
+    double fzl = 0.0, fzh = 0.0;
+    if (DIM == 3) {
+      fzl = dzi * Dzl * (uc - u[o - us2]);
+      fzh = dzi * Dzh * (u[o + us2] - uc);
+    }
+    if (EWING) {
+      int c[3] = {i, j, k};
+      if (ii == 0) { long x = cf_index(L, p, 0, c, b); if (x >= 0) fxl = L.cf_flux[x]; }
+      if (ii == nx - 1) { long x = cf_index(L, p, 1, c, b); if (x >= 0) fxh = L.cf_flux[x]; }
+      if (jj == 0) { long x = cf_index(L, p, 2, c, b); if (x >= 0) fyl = L.cf_flux[x]; }
+      if (jj == ny - 1) { long x = cf_index(L, p, 3, c, b); if (x >= 0) fyh = L.cf_flux[x]; }
+      if (DIM == 3) {
+        if (kk == 0) { long x = cf_index(L, p, 4, c, b); if (x >= 0) fzl = L.cf_flux[x]; }
+        if (kk == nz - 1) { long x = cf_index(L, p, 5, c, b); if (x >= 0) fzh = L.cf_flux[x]; }
+      }
+    }
+    double scal;
+    if (CMODE == 2)
+      scal = Cp[(ii + ps.cghost) + cs1 * (jj + ps.cghost) + cs2 * (DIM == 3 ? kk + ps.cghost : 0)];
+    else
+      scal = CMODE == 1 ? ps.cconst : 0.0;
+    const double fv = f[(ii + rg) + fs1 * (jj + rg) + fs2 * (DIM == 3 ? kk + rg : 0)];
     double residual, diag;
     if (DIM == 3) {
-      double fzl =
-          face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 2, i, j, k, dzi, ewing);
-      double fzh =
-          face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 2, i, j, k + 1, dzi, ewing);
-      residual = f[vr.at(i, j, k)] -
-                 (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
-      double dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
+      residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
       if (DVAR)
-        diag = (dxi2 * (Dp[0][vd[0].at(i + 1, j, k)] + Dp[0][vd[0].at(i, j, k)]) +
-                dyi2 * (Dp[1][vd[1].at(i, j + 1, k)] + Dp[1][vd[1].at(i, j, k)]) +
-                dzi2 * (Dp[2][vd[2].at(i, j, k + 1)] + Dp[2][vd[2].at(i, j, k)])) -
-               scal;
+        diag = (dxi2 * (Dxh + Dxl) + dyi2 * (Dyh + Dyl) + dzi2 * (Dzh + Dzl)) - scal;
       else
-        diag = (dxi2 * (ps.dconst + ps.dconst) + dyi2 * (ps.dconst + ps.dconst) +
-                dzi2 * (ps.dconst + ps.dconst)) -
-               scal;
+        diag = (dxi2 * (dc + dc) + dyi2 * (dc + dc) + dzi2 * (dc + dc)) - scal;
     } else {
-      residual = f[vr.at(i, j, k)] - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
+      residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
       if (DVAR)
-        diag = (dxi * dxi * (Dp[0][vd[0].at(i + 1, j, k)] + Dp[0][vd[0].at(i, j, k)]) +
-                dyi * dyi * (Dp[1][vd[1].at(i, j + 1, k)] + Dp[1][vd[1].at(i, j, k)])) -
-               scal;
+        diag = (dxi * dxi * (Dxh + Dxl) + dyi * dyi * (Dyh + Dyl)) - scal;
       else
-        diag = (dxi * dxi * (ps.dconst + ps.dconst) + dyi * dyi * (ps.dconst + ps.dconst)) -
-               scal;
+        diag = (dxi * dxi * (dc + dc) + dyi * dyi * (dc + dc)) - scal;
     }
-    double dudr = 1.0 / diag;
-    double du = -residual * dudr;
-    u[vu.at(i, j, k)] = uc + du * 1.0;
+    const double dudr = 1.0 / diag;
+    const double du = -residual * dudr;
+    u[o] = uc + du * 1.0;
     mr = fmax(mr, fabs(residual));
   }
   if (maxres) {
-    for (int o = 32; o > 0; o >>= 1) mr = fmax(mr, __shfl_xor(mr, o, 64));
-    if ((threadIdx.x & 63) == 0 && mr > 0.0) atomic_max_nonneg(&maxres[p], mr);
+    // work-group max into this chunk's slot (no atomics: the per-patch value
+    // the early-exit test needs is the max over the last patch's chunks)
+    __shared__ double wm[kThreads / 64];
+    for (int s = 32; s > 0; s >>= 1) mr = fmax(mr, __shfl_xor(mr, s, 64));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = mr;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double v = wm[0];
+      for (int w = 1; w < kThreads / 64; ++w) v = fmax(v, wm[w]);
+      maxres[blockIdx.x] = v;
+    }
   }
 }
 
@@ -598,26 +660,8 @@ k_set_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* cons
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_sweep_check(double* red, double* blk, int npatch, double tol, int* stop, int* sweeps,
-              double* maxres_out) {
-  __shared__ int done;
-  if (threadIdx.x == 0) {
-    done = *stop;
-    if (!done) {
-      double r = red[npatch - 1], b = blk[npatch - 1];
-      double mx = r > b ? r : b;  // tbox::MathUtilities<double>::Max
-      if (maxres_out) *maxres_out = mx;
-      *sweeps += 1;
-      if (!(mx > tol)) *stop = 1;
-    }
-  }
-  __syncthreads();
-  if (done) return;
-  for (int t = threadIdx.x; t < npatch; t += kThreads) {
-    red[t] = 0.0;
-    blk[t] = 0.0;
-  }
+__global__ void k_sweep_check(SweepCheck chk) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) sweep_check(chk);
 }
 
 inline void check_launch() {
@@ -629,11 +673,29 @@ inline void check_launch() {
 
 void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int dg,
                         const double* const* coarse, int cg, const Box* coarse_box,
-                        int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s) {
-  if (w.nchunks == 0) return;
+                        int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s,
+                        const SweepCheck* check) {
+  SweepCheck chk{};
+  if (check) chk = *check;
+  if (w.nchunks == 0) {
+    if (check) launch_sweep_check(*check, s);
+    return;
+  }
   hipLaunchKernelGGL(k_ghost_items, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L,
-                     dst, dg, coarse, cg, coarse_box, homogeneous, bc, stop);
+                     dst, dg, coarse, cg, coarse_box, homogeneous, bc, stop, chk);
   check_launch();
+}
+
+template <int DIM, bool DVAR, int CMODE>
+static void rbgs_dispatch_e(const SmoothLaunch& w, const LevelGeom& L, double* const* e, int eg,
+                            const double* const* rhs, int rg, const PoissonDev& ps, int ewing,
+                            int colour, double* maxres, const int* stop, hipStream_t s) {
+  if (ewing)
+    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, CMODE, true>), dim3(w.nchunks), dim3(kThreads), 0, s,
+                       w.items, w.chunks, L, e, eg, rhs, rg, ps, colour, maxres, stop);
+  else
+    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, CMODE, false>), dim3(w.nchunks), dim3(kThreads), 0, s,
+                       w.items, w.chunks, L, e, eg, rhs, rg, ps, colour, maxres, stop);
 }
 
 template <int DIM, bool DVAR>
@@ -641,14 +703,11 @@ static void rbgs_dispatch_c(const SmoothLaunch& w, const LevelGeom& L, double* c
                             const double* const* rhs, int rg, const PoissonDev& ps, int ewing,
                             int colour, double* maxres, const int* stop, hipStream_t s) {
   if (ps.cmode == 0)
-    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, 0>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
-                       w.chunks, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop);
+    rbgs_dispatch_e<DIM, DVAR, 0>(w, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop, s);
   else if (ps.cmode == 1)
-    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, 1>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
-                       w.chunks, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop);
+    rbgs_dispatch_e<DIM, DVAR, 1>(w, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop, s);
   else
-    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, 2>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
-                       w.chunks, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop);
+    rbgs_dispatch_e<DIM, DVAR, 2>(w, L, e, eg, rhs, rg, ps, ewing, colour, maxres, stop, s);
 }
 
 void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, int eg,
@@ -826,10 +885,8 @@ void launch_set_items(const GhostLaunch& w, const LevelGeom& L, double* const* d
   check_launch();
 }
 
-void launch_sweep_check(double* red, double* blk, int npatch, double tol, int* stop,
-                        int* sweeps, double* maxres_out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sweep_check, dim3(1), dim3(kThreads), 0, s, red, blk, npatch, tol, stop,
-                     sweeps, maxres_out);
+void launch_sweep_check(const SweepCheck& chk, hipStream_t s) {
+  hipLaunchKernelGGL(k_sweep_check, dim3(1), dim3(64), 0, s, chk);
   check_launch();
 }
 
