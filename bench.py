@@ -10,8 +10,9 @@ coarsest level red-black (500 sweeps, tolerance 1e-12), Ewing coarse-fine
 fluxes, conservative linear prolongation.
 
 Prints one JSON line: time to solution (value, s), smoother throughput, the
-roofline of the red/black colour kernel measured with HIP events on the
-solver's stream over the timed steps, and the CPU baseline (the C oracle, a
+roofline of the smoother kernel with the largest share of the time, measured
+with HIP events on the solver's stream over the timed steps, and the CPU
+baseline (the C oracle, a
 bounded sample of FAC cycles on the same deck, projected to the cycle count
 the GPU needed).
 """
@@ -126,8 +127,8 @@ def main():
     s.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    launches, kms, kbytes = s.kernel_timing(-1)
-    l0 = s.kernel_timing(0)
+    per_level = [s.kernel_timing(ln) for ln in range(len(deck["levels"]))]
+    fused = s.fused_coarsest_active()
     s.enable_kernel_timing(False)
     cycles = s.number_of_iterations()
     hist = s.history()
@@ -136,24 +137,40 @@ def main():
           (args.steps, ms_per_step / 1e3, cycles, hist[-1]), file=sys.stderr, flush=True)
     err = max_error(decks, deck, u)
 
-    # smoother throughput on the coarsest level (the dominant operation)
+    # smoother throughput (interior cells x full red+black sweeps / s) on
+    # the coarsest level (the dominant operation) and on the finest level
     e = s.internal_vector("e")
     r = s.internal_vector("r")
-    n0 = sum(int(np.prod([hh - ll + 1 for ll, hh in zip(*b)])) for b in deck["levels"][0])
-    s.synchronize()
-    t1 = time.perf_counter()
-    s.smooth(e, r, 0, args.smoother_sweeps)
-    s.synchronize()
-    sm = time.perf_counter() - t1
-    smoother_gcells = n0 * args.smoother_sweeps / sm / 1e9
 
-    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
-    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+    def smoother_rate(ln):
+        n = sum(int(np.prod([hh - ll + 1 for ll, hh in zip(*b)])) for b in deck["levels"][ln])
+        s.synchronize()
+        t1 = time.perf_counter()
+        s.smooth(e, r, ln, args.smoother_sweeps)
+        s.synchronize()
+        return n * args.smoother_sweeps / (time.perf_counter() - t1) / 1e9
+
+    smoother_gcells = smoother_rate(0)
+    finest_gcells = smoother_rate(len(deck["levels"]) - 1)
+
+    # roofline of the smoother kernel that takes the most time: algorithmic
+    # bytes of each launch (24 B per cell per full sweep at constant D: u
+    # read, f read, u write) over its HIP-event duration
+    levels = []
+    for ln, (n, ms, b) in enumerate(per_level):
+        levels.append(dict(level=ln, kernel=("k_box_rbgs (colour-fused sweep)" if ln == 0 and fused
+                                             else "k_rbgs (one colour pass)"),
+                           launches=n, total_ms=round(ms, 3),
+                           avg_launch_us=round(ms * 1e3 / max(n, 1), 3),
+                           bytes_per_launch=round(b / max(n, 1)),
+                           achieved_GBs=round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else 0.0))
+    dom = max(levels, key=lambda d: d["total_ms"])
+    achieved = dom["achieved_GBs"]
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel="k_rbgs (red/black colour pass, all levels)",
-                    launches=launches, avg_launch_us=round(kms * 1e3 / max(launches, 1), 3),
-                    level0_avg_launch_us=round(l0[1] * 1e3 / max(l0[0], 1), 3),
-                    level0_achieved=round(l0[2] / (l0[1] * 1e-3) / 1e9, 1) if l0[1] else 0.0)
+                    kernel="%s, level %d" % (dom["kernel"], dom["level"]),
+                    launches=dom["launches"], avg_launch_us=dom["avg_launch_us"],
+                    bytes_per_launch=dom["bytes_per_launch"], per_level=levels)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -178,6 +195,8 @@ def main():
                cycles=cycles, converged=all(converged), final_residual=float(hist[-1]),
                max_error_vs_exact=err,
                smoother_gcells_per_s=round(smoother_gcells, 2),
+               finest_smoother_gcells_per_s=round(finest_gcells, 2),
+               fused_coarsest=fused,
                roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)
 

@@ -240,6 +240,19 @@ int sfac_solver_enable_kernel_timing(sfac_solver* s, int on);
 int sfac_solver_kernel_timing(sfac_solver* s, int level, long* launches, double* ms,
                               double* bytes);
 
+/* Execution choice with no reference counterpart (results are unchanged):
+ * when the coarsest level of the solve tiles the physical domain exactly,
+ * its red-black sweeps run colour-fused over one contiguous copy of the
+ * level, one launch per sweep (the timing above then counts 24 / 48 B per
+ * cell per launch).  on = 0 keeps the patch-wise colour passes.
+ * tile_variant selects the LDS tile shape (0 .. sfac_fused_tile_variants-1);
+ * fused_coarsest_active reports whether the fused path is in use after
+ * sfac_solver_initialize. */
+int sfac_solver_set_fused_coarsest(sfac_solver* s, int on, int tile_variant);
+int sfac_solver_fused_coarsest_active(sfac_solver* s, int* active);
+int sfac_fused_tile_variants(int dim);
+int sfac_fused_tile_shape(int dim, int variant, int* shape3);
+
 /* FACOperatorStrategy operations of the GPU CellPoissonFACOps
  * (solv/FACOperatorStrategy.h:82-364), valid after sfac_solver_initialize.
  * which: 0 error, 1 residual, 2 temporary residual, 3 weights (borrowed). */

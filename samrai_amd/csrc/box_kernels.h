@@ -1,0 +1,86 @@
+// Colour-fused red-black sweeps on a level that is one rectangle of cells.
+//
+// The coarsest level of a FAC solve usually tiles the physical domain
+// exactly (level 0 of every benchmark hierarchy).  Its same-level ghost
+// copies are exact copies and its only other ghosts are physical Robin
+// ghosts, so the patches can be gathered once per smoothing call into
+// contiguous arrays and every sweep becomes ONE launch: each work-group
+// loads a tile plus a two-cell halo into LDS, updates the red cells of the
+// tile grown by one (the halo reds are recomputed redundantly, from the same
+// inputs and in the same expression order, so they equal the neighbour
+// tile's values bit for bit), refreshes the physical ghosts in LDS, then
+// updates the black cells of the tile and writes the tile back.  The result
+// is identical to the patch-by-patch red pass / ghost fill / black pass /
+// ghost fill sequence of CellPoissonFACOps::smoothErrorByRedBlack
+// (solv/CellPoissonFACOps.C:1338-1514).
+#pragma once
+
+#include "level_kernels.h"
+
+namespace sfac {
+
+struct BoxLevel {
+  int dim;
+  int n[3];         // cells per axis
+  int lo[3];        // level index of cell (0,0,0)
+  double dx[3];
+  int shift;        // colour shift: red <=> (i + j + k + shift) even (level indices)
+  int lp_lo[3];     // last patch box, relative to lo (early-exit test)
+  int lp_hi[3];
+};
+
+struct BoxArrays {
+  const double* ein;    // u before the sweep: n0*n1*n2, no ghosts
+  double* eout;         // u after the sweep (a second buffer: work-groups read
+                        // their halos from ein while others write eout)
+  const double* r;      // same shape
+  const double* D[3];   // side data per axis, +1 along the axis (variable D)
+  const double* C;      // cell data (C mode 2)
+  double dconst;
+  int cmode;
+  double cconst;
+};
+
+// Early-exit test of the sweep before this launch.  prev: per-tile max
+// |residual| over the last patch written by the previous sweep (nullptr for
+// the first sweep or without a tolerance); cur: this sweep's slots.
+struct BoxCheck {
+  const double* prev;
+  double* cur;
+  int nlp;              // slots (tiles meeting the last patch)
+  int tlo[3], tn[3];    // tile range of the last patch
+  int sweep;            // index of this sweep
+  double tol;
+  int* stop;
+  int* sweeps;
+  double* maxres_out;
+};
+
+// Tile shape of a fused sweep.
+struct BoxTile {
+  int tx, ty, tz;
+};
+// Tile shapes compiled in (index = variant), per dimension.
+int box_tile_variants(int dim);
+BoxTile box_tile(int dim, int variant);
+
+void launch_box_rbgs(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int homogeneous,
+                     const BoxCheck& K, int variant, hipStream_t s);
+// After the last sweep: maxres / sweep count of the last sweep when the
+// loop did not stop early.
+void launch_box_final_check(const BoxCheck& K, int nsweeps, hipStream_t s);
+
+// Copy cell data between the patches of a level (interior cells) and a
+// contiguous box array: to_box 1 gathers, 0 scatters.  The box array is
+// box[0] or box[1] by the parity of *count (count == nullptr: of count_host),
+// which lets the scatter follow a sweep count known only on the device.
+void launch_box_cells(const GhostLaunch& cells, const LevelGeom& L, double* const* x, int g,
+                      double* box0, double* box1, const int* count, int count_host,
+                      const BoxLevel& G, int to_box, hipStream_t s);
+// Gather side data of axis a into the box face array; with verify, faces
+// shared by two patches are then compared and *mismatch set to 1 if the
+// patches hold different values there.
+void launch_box_side(const LevelGeom& L, const double* const* x, int g, int a, double* box,
+                     const BoxLevel& G, int* mismatch, hipStream_t s, int verify);
+
+}  // namespace sfac

@@ -1,0 +1,408 @@
+// Colour-fused red-black sweeps over a single-rectangle level (gfx950).
+//
+// Per work-group: one TX x TY x TZ tile of owned cells.  LDS holds the tile
+// grown by two cells on every side (clipped to the level, plus the physical
+// ghost layer).  Per launch:
+//   1. early-exit test of the previous sweep (every work-group reduces the
+//      same per-tile maxima, so all of them take the same decision);
+//   2. load u; compute the face ghosts of the physical boundary in LDS;
+//   3. red update on the tile grown by one (halo reds are recomputed from the
+//      same inputs as their owning tile, so they are bit-identical);
+//   4. refresh the face ghosts from the updated reds;
+//   5. black update on the owned tile; write the owned tile back.
+// The per-cell arithmetic is rbgswithfluxmax{vardc,condc}{varsf,consf}3d /
+// 2d (solv_cellpoissonfacops3d.f:136-481, 2d :95-364) in its expression
+// order, and the ghosts are settype1cells (solv_cartesianrobinbchelper3d.f),
+// as in the level-batched kernels.  Edge and corner ghosts are not read by
+// the 7-point (5-point) stencil, so they are not formed here; the caller's
+// final ghost fill sets them in the patches.
+#include "box_kernels.h"
+
+namespace sfac {
+namespace {
+
+inline void check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw e;
+}
+
+__device__ inline int pos2(int x) { return x & 1; }  // parity of any int (two's complement)
+
+// Robin face ghost from its interior neighbour (settype1cells, homogeneous
+// or not), location loc = 2*axis + side.
+template <int DIM>
+__device__ inline double robin_ghost(const BoxLevel& G, const RobinDev& bc, int homogeneous,
+                                     int loc, double uin) {
+  const int n = loc >> 1;
+  const double h = G.dx[n], av = bc.a[loc], bv = bc.b[loc];
+  if (DIM == 3)
+    return ((homogeneous ? 0.0 : h * bc.g[loc]) + uin * (bv - 0.5 * h * av)) /
+           (bv + 0.5 * h * av);
+  if (homogeneous) return (uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
+  return (h * bc.g[loc] + uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
+}
+
+// Gauss-Seidel update of one cell held in LDS at q (strides 1, SY, SZ);
+// (i, j, k) are its box-relative indices.  Returns |residual|.
+template <int DIM, bool DVAR, int CMODE>
+__device__ inline double cell_update(double* U, int q, int SY, int SZ, int i, int j, int k,
+                                     const BoxLevel& G, const BoxArrays& A) {
+  const int n0 = G.n[0], n1 = G.n[1];
+  const long c = (long)i + (long)n0 * (j + (long)n1 * k);
+  const double dxi = 1.0 / G.dx[0], dyi = 1.0 / G.dx[1];
+  const double dzi = DIM == 3 ? 1.0 / G.dx[2] : 0.0;
+  const double dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
+  const double dc = A.dconst;
+  const double uc = U[q];
+  double Dxl, Dxh, Dyl, Dyh, Dzl = 0.0, Dzh = 0.0;
+  if (DVAR) {
+    const long ox = (long)i + (long)(n0 + 1) * (j + (long)n1 * k);
+    const long oy = (long)i + (long)n0 * (j + (long)(n1 + 1) * k);
+    Dxl = A.D[0][ox];
+    Dxh = A.D[0][ox + 1];
+    Dyl = A.D[1][oy];
+    Dyh = A.D[1][oy + n0];
+    if (DIM == 3) {
+      Dzl = A.D[2][c];
+      Dzh = A.D[2][c + (long)n0 * n1];
+    }
+  } else {
+    Dxl = Dxh = Dyl = Dyh = Dzl = Dzh = dc;
+  }
+  const double fxl = dxi * Dxl * (uc - U[q - 1]);
+  const double fxh = dxi * Dxh * (U[q + 1] - uc);
+  const double fyl = dyi * Dyl * (uc - U[q - SY]);
+  const double fyh = dyi * Dyh * (U[q + SY] - uc);
+  double fzl = 0.0, fzh = 0.0;
+  if (DIM == 3) {
+    fzl = dzi * Dzl * (uc - U[q - SZ]);
+    fzh = dzi * Dzh * (U[q + SZ] - uc);
+  }
+  double scal;
+  if (CMODE == 2) scal = A.C[c];
+  else scal = CMODE == 1 ? A.cconst : 0.0;
+  const double fv = A.r[c];
+  double residual, diag;
+  if (DIM == 3) {
+    residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
+    if (DVAR)
+      diag = (dxi2 * (Dxh + Dxl) + dyi2 * (Dyh + Dyl) + dzi2 * (Dzh + Dzl)) - scal;
+    else
+      diag = (dxi2 * (dc + dc) + dyi2 * (dc + dc) + dzi2 * (dc + dc)) - scal;
+  } else {
+    residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
+    if (DVAR)
+      diag = (dxi * dxi * (Dxh + Dxl) + dyi * dyi * (Dyh + Dyl)) - scal;
+    else
+      diag = (dxi * dxi * (dc + dc) + dyi * dyi * (dc + dc)) - scal;
+  }
+  const double dudr = 1.0 / diag;
+  const double du = -residual * dudr;
+  U[q] = uc + du * 1.0;
+  return fabs(residual);
+}
+
+// Work-group max of v (kThreads threads); result valid in thread 0.
+__device__ inline double block_max(double v, double* wm) {
+  for (int s = 32; s > 0; s >>= 1) v = fmax(v, __shfl_xor(v, s, 64));
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < kThreads / 64; ++w) v = fmax(v, wm[w]);
+  return v;
+}
+
+template <int DIM, bool DVAR, int CMODE, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(kThreads)
+k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, int ntx, int nty,
+           int ntiles, int per_xcd) {
+  constexpr int LX = TX + 4, LY = TY + 4, LZ = DIM == 3 ? TZ + 4 : 1;
+  constexpr int SY = LX, SZ = LX * LY;
+  constexpr int ZH = DIM == 3 ? 2 : 0;  // LDS halo along z
+  __shared__ double U[LX * LY * LZ];
+  __shared__ double wm[kThreads / 64];
+  __shared__ int s_stop;
+  // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
+  // so consecutive tiles (one slab of the level) share an XCD's L2.
+  const int b = blockIdx.x;
+  const int t = (b & 7) * per_xcd + (b >> 3);
+  if (t >= ntiles) return;
+  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
+  const int x0 = tx * TX, y0 = ty * TY, z0 = DIM == 3 ? tz * TZ : 0;
+  const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
+
+  // 1. early exit
+  if (K.stop) {
+    if (threadIdx.x == 0) s_stop = *K.stop;
+    __syncthreads();
+    if (!s_stop && K.prev) {
+      double m = 0.0;
+      for (int x = threadIdx.x; x < K.nlp; x += kThreads) m = fmax(m, K.prev[x]);
+      m = block_max(m, wm);
+      if (threadIdx.x == 0 && !(m > K.tol)) {
+        s_stop = 1;
+        if (b == 0) {
+          *K.stop = 1;
+          *K.sweeps = K.sweep;
+          if (K.maxres_out) *K.maxres_out = m;
+        }
+      }
+    }
+    __syncthreads();
+    if (s_stop) return;
+  }
+
+  // 2. load the grown tile (outside the level: zero, ghosts set below)
+  for (int q = threadIdx.x; q < LX * LY * LZ; q += kThreads) {
+    const int lx = q % LX, ly = (q / LX) % LY, lz = q / (LX * LY);
+    const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
+    const bool in = i >= 0 && i < n0 && j >= 0 && j < n1 && (DIM == 2 || (k >= 0 && k < n2));
+    U[q] = in ? A.ein[(long)i + (long)n0 * (j + (long)n1 * k)] : 0.0;
+  }
+  __syncthreads();
+
+  // face ghosts of the physical boundary that fall inside the LDS tile
+  auto ghosts = [&]() {
+    for (int a = 0; a < DIM; ++a)
+      for (int side = 0; side < 2; ++side) {
+        const int org[3] = {x0 - 2, y0 - 2, z0 - ZH};
+        const int gidx = side ? G.n[a] : -1;     // ghost index along a
+        const int l = gidx - org[a];
+        const int lext[3] = {LX, LY, LZ};
+        if (l < 0 || l >= lext[a]) continue;
+        const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+        const int e1 = lext[t1], e2 = DIM == 3 ? lext[t2] : 1;
+        for (int w = threadIdx.x; w < e1 * e2; w += kThreads) {
+          int lc[3];
+          lc[a] = l;
+          lc[t1] = w % e1;
+          lc[t2] = w / e1;
+          const int g1 = org[t1] + lc[t1], g2 = org[t2] + lc[t2];
+          if (g1 < 0 || g1 >= G.n[t1]) continue;
+          if (DIM == 3 && (g2 < 0 || g2 >= G.n[t2])) continue;
+          const int q = lc[0] + SY * lc[1] + SZ * lc[2];
+          const int stride = a == 0 ? 1 : (a == 1 ? SY : SZ);
+          const double uin = U[side ? q - stride : q + stride];
+          U[q] = robin_ghost<DIM>(G, bc, homogeneous, 2 * a + side, uin);
+        }
+      }
+  };
+  ghosts();
+  __syncthreads();
+
+  // colour parity: red <=> (ilev + jlev + klev + shift) even; in LDS
+  // coordinates the sum is base + lx + ly + lz
+  const int base = G.lo[0] + G.lo[1] + (DIM == 3 ? G.lo[2] : 0) + x0 + y0 + z0 - 4 - ZH + G.shift;
+  const int pb = pos2(base);
+  const bool lp_tile = K.cur != nullptr;
+  double mr = 0.0;
+  auto in_lp = [&](int i, int j, int k) {
+    return i >= G.lp_lo[0] && i <= G.lp_hi[0] && j >= G.lp_lo[1] && j <= G.lp_hi[1] &&
+           (DIM == 2 || (k >= G.lp_lo[2] && k <= G.lp_hi[2]));
+  };
+
+  // 3. red cells of the tile grown by one
+  {
+    constexpr int HX = (TX + 2) / 2;
+    constexpr int RY = TY + 2, RZ = DIM == 3 ? TZ + 2 : 1;
+    for (int w = threadIdx.x; w < HX * RY * RZ; w += kThreads) {
+      const int m = w % HX, ly = 1 + (w / HX) % RY, lz = DIM == 3 ? 1 + w / (HX * RY) : 0;
+      const int lx = 1 + pos2(pb + 1 + ly + lz) + 2 * m;
+      const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
+      if (i < 0 || i >= n0 || j < 0 || j >= n1) continue;
+      if (DIM == 3 && (k < 0 || k >= n2)) continue;
+      const double ra =
+          cell_update<DIM, DVAR, CMODE>(U, lx + SY * ly + SZ * lz, SY, SZ, i, j, k, G, A);
+      const bool owned = lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2 &&
+                         (DIM == 2 || (lz >= 2 && lz < TZ + 2));
+      if (lp_tile && owned && in_lp(i, j, k)) mr = fmax(mr, ra);
+    }
+  }
+  __syncthreads();
+  ghosts();
+  __syncthreads();
+
+  // 5. black cells of the owned tile
+  {
+    constexpr int HB = TX / 2;
+    constexpr int BZ = DIM == 3 ? TZ : 1;
+    for (int w = threadIdx.x; w < HB * TY * BZ; w += kThreads) {
+      const int m = w % HB, ly = 2 + (w / HB) % TY, lz = DIM == 3 ? 2 + w / (HB * TY) : 0;
+      const int lx = 3 - pos2(pb + ly + lz) + 2 * m;
+      const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
+      if (i >= n0 || j >= n1 || (DIM == 3 && k >= n2)) continue;
+      const double ra =
+          cell_update<DIM, DVAR, CMODE>(U, lx + SY * ly + SZ * lz, SY, SZ, i, j, k, G, A);
+      if (lp_tile && in_lp(i, j, k)) mr = fmax(mr, ra);
+    }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < TX * TY * (DIM == 3 ? TZ : 1); w += kThreads) {
+    const int ox = w % TX, oy = (w / TX) % TY, oz = DIM == 3 ? w / (TX * TY) : 0;
+    const int i = x0 + ox, j = y0 + oy, k = z0 + oz;
+    if (i >= n0 || j >= n1 || (DIM == 3 && k >= n2)) continue;
+    A.eout[(long)i + (long)n0 * (j + (long)n1 * k)] =
+        U[(ox + 2) + SY * (oy + 2) + SZ * (oz + ZH)];
+  }
+  if (lp_tile) {
+    const int ti[3] = {tx - K.tlo[0], ty - K.tlo[1], DIM == 3 ? tz - K.tlo[2] : 0};
+    const bool meets = ti[0] >= 0 && ti[0] < K.tn[0] && ti[1] >= 0 && ti[1] < K.tn[1] &&
+                       ti[2] >= 0 && ti[2] < K.tn[2];
+    if (meets) {  // uniform over the work-group
+      mr = block_max(mr, wm);
+      if (threadIdx.x == 0) K.cur[ti[0] + K.tn[0] * (ti[1] + K.tn[1] * ti[2])] = mr;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_box_final_check(BoxCheck K, int nsweeps) {
+  __shared__ double wm[kThreads / 64];
+  if (*K.stop) return;  // stopped early: already recorded
+  double m = 0.0;
+  for (int x = threadIdx.x; x < K.nlp; x += kThreads) m = fmax(m, K.cur[x]);
+  m = block_max(m, wm);
+  if (threadIdx.x == 0) {
+    *K.sweeps = nsweeps;
+    if (K.maxres_out) *K.maxres_out = m;
+    if (!(m > K.tol)) *K.stop = 1;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_box_cells(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* x, int g,
+            double* box0, double* box1, const int* count, int count_host, BoxLevel G,
+            int to_box) {
+  double* box = ((count ? *count : count_host) & 1) ? box1 : box0;
+  const Chunk ch = chunks[blockIdx.x];
+  const BoxItem it = items[ch.item];
+  const Box pb = L.box[it.p];
+  const View v = make_view(pb.lo, pb.hi, g, L.dim);
+  double* xp = x[it.p];
+  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
+    int c[3];
+    if (!decode_box(it, (long)ch.start + t, c)) break;
+    const long o = (long)(c[0] - G.lo[0]) +
+                   (long)G.n[0] * ((c[1] - G.lo[1]) + (long)G.n[1] * (L.dim == 3 ? c[2] - G.lo[2] : 0));
+    if (to_box) box[o] = xp[v.at(c[0], c[1], c[2])];
+    else xp[v.at(c[0], c[1], c[2])] = box[o];
+  }
+}
+
+// grid (npatch, slices): faces of axis a of patch blockIdx.x
+__global__ void __launch_bounds__(kThreads)
+k_box_side(LevelGeom L, const double* const* x, int g, int a, double* box, BoxLevel G,
+           int* mismatch, int verify) {
+  const int p = blockIdx.x;
+  const Box pb = L.box[p];
+  int e[3] = {1, 1, 1};
+  for (int d = 0; d < L.dim; ++d) e[d] = pb.hi[d] - pb.lo[d] + 1 + (d == a ? 1 : 0);
+  int bn[3] = {G.n[0], G.n[1], L.dim == 3 ? G.n[2] : 1};
+  bn[a] += 1;
+  const View v = make_view(pb.lo, pb.hi, g, L.dim, a);
+  const double* xp = x[p];
+  const long nf = (long)e[0] * e[1] * e[2];
+  for (long t = (long)blockIdx.y * kThreads + threadIdx.x; t < nf;
+       t += (long)gridDim.y * kThreads) {
+    const int i = pb.lo[0] + (int)(t % e[0]);
+    const int j = pb.lo[1] + (int)((t / e[0]) % e[1]);
+    const int k = L.dim == 3 ? pb.lo[2] + (int)(t / ((long)e[0] * e[1])) : 0;
+    const long o = (long)(i - G.lo[0]) +
+                   (long)bn[0] * ((j - G.lo[1]) + (long)bn[1] * (L.dim == 3 ? k - G.lo[2] : 0));
+    const double val = xp[v.at(i, j, k)];
+    if (!verify) box[o] = val;
+    else if (__double_as_longlong(box[o]) != __double_as_longlong(val))
+      *mismatch = 1;
+  }
+}
+
+template <int DIM, bool DVAR, int CMODE, int TX, int TY, int TZ>
+void launch_tile(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int homogeneous,
+                 const BoxCheck& K, hipStream_t s) {
+  const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
+  const int ntz = DIM == 3 ? (G.n[2] + TZ - 1) / TZ : 1;
+  const int ntiles = ntx * nty * ntz;
+  const int per_xcd = (ntiles + 7) / 8;
+  hipLaunchKernelGGL((k_box_rbgs<DIM, DVAR, CMODE, TX, TY, TZ>), dim3(8 * per_xcd),
+                     dim3(kThreads), 0, s, G, A, bc, homogeneous, K, ntx, nty, ntiles, per_xcd);
+  check_launch();
+}
+
+constexpr BoxTile kTiles3[] = {{32, 8, 8}, {16, 8, 8}, {32, 16, 4}, {64, 4, 4}};
+constexpr BoxTile kTiles2[] = {{64, 16, 1}, {32, 16, 1}, {32, 32, 1}, {128, 8, 1}};
+
+template <int DIM, bool DVAR, int CMODE>
+void launch_variant(int variant, const BoxLevel& G, const BoxArrays& A, const RobinDev& bc,
+                    int homogeneous, const BoxCheck& K, hipStream_t s) {
+  if (DIM == 3) {
+    switch (variant) {
+      case 1: launch_tile<3, DVAR, CMODE, 16, 8, 8>(G, A, bc, homogeneous, K, s); break;
+      case 2: launch_tile<3, DVAR, CMODE, 32, 16, 4>(G, A, bc, homogeneous, K, s); break;
+      case 3: launch_tile<3, DVAR, CMODE, 64, 4, 4>(G, A, bc, homogeneous, K, s); break;
+      default: launch_tile<3, DVAR, CMODE, 32, 8, 8>(G, A, bc, homogeneous, K, s); break;
+    }
+  } else {
+    switch (variant) {
+      case 1: launch_tile<2, DVAR, CMODE, 32, 16, 1>(G, A, bc, homogeneous, K, s); break;
+      case 2: launch_tile<2, DVAR, CMODE, 32, 32, 1>(G, A, bc, homogeneous, K, s); break;
+      case 3: launch_tile<2, DVAR, CMODE, 128, 8, 1>(G, A, bc, homogeneous, K, s); break;
+      default: launch_tile<2, DVAR, CMODE, 64, 16, 1>(G, A, bc, homogeneous, K, s); break;
+    }
+  }
+}
+
+template <int DIM>
+void launch_dim(int variant, const BoxLevel& G, const BoxArrays& A, const RobinDev& bc,
+                int homogeneous, const BoxCheck& K, hipStream_t s) {
+  const bool dvar = A.D[0] != nullptr;
+  if (dvar) {
+    if (A.cmode == 2) launch_variant<DIM, true, 2>(variant, G, A, bc, homogeneous, K, s);
+    else if (A.cmode == 1) launch_variant<DIM, true, 1>(variant, G, A, bc, homogeneous, K, s);
+    else launch_variant<DIM, true, 0>(variant, G, A, bc, homogeneous, K, s);
+  } else {
+    if (A.cmode == 2) launch_variant<DIM, false, 2>(variant, G, A, bc, homogeneous, K, s);
+    else if (A.cmode == 1) launch_variant<DIM, false, 1>(variant, G, A, bc, homogeneous, K, s);
+    else launch_variant<DIM, false, 0>(variant, G, A, bc, homogeneous, K, s);
+  }
+}
+
+}  // namespace
+
+int box_tile_variants(int) { return 4; }
+
+BoxTile box_tile(int dim, int variant) {
+  if (variant < 0 || variant > 3) variant = 0;
+  return dim == 3 ? kTiles3[variant] : kTiles2[variant];
+}
+
+void launch_box_rbgs(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int homogeneous,
+                     const BoxCheck& K, int variant, hipStream_t s) {
+  if (G.dim == 3) launch_dim<3>(variant, G, A, bc, homogeneous, K, s);
+  else launch_dim<2>(variant, G, A, bc, homogeneous, K, s);
+}
+
+void launch_box_final_check(const BoxCheck& K, int nsweeps, hipStream_t s) {
+  hipLaunchKernelGGL(k_box_final_check, dim3(1), dim3(kThreads), 0, s, K, nsweeps);
+  check_launch();
+}
+
+void launch_box_cells(const GhostLaunch& cells, const LevelGeom& L, double* const* x, int g,
+                      double* box0, double* box1, const int* count, int count_host,
+                      const BoxLevel& G, int to_box, hipStream_t s) {
+  if (cells.nchunks == 0) return;
+  hipLaunchKernelGGL(k_box_cells, dim3(cells.nchunks), dim3(kThreads), 0, s, cells.items,
+                     cells.chunks, L, x, g, box0, box1, count, count_host, G, to_box);
+  check_launch();
+}
+
+void launch_box_side(const LevelGeom& L, const double* const* x, int g, int a, double* box,
+                     const BoxLevel& G, int* mismatch, hipStream_t s, int verify) {
+  if (L.npatch == 0) return;
+  dim3 grid(L.npatch, 64);
+  hipLaunchKernelGGL(k_box_side, grid, dim3(kThreads), 0, s, L, x, g, a, box, G, mismatch, 0);
+  check_launch();
+  if (!verify) return;
+  hipLaunchKernelGGL(k_box_side, grid, dim3(kThreads), 0, s, L, x, g, a, box, G, mismatch, 1);
+  check_launch();
+}
+
+}  // namespace sfac

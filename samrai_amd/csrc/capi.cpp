@@ -341,6 +341,36 @@ int sfac_solver_enable_kernel_timing(sfac_solver* s, int on) {
   });
 }
 
+int sfac_solver_set_fused_coarsest(sfac_solver* s, int on, int tile_variant) {
+  return guarded([&] {
+    need(s, "solver");
+    s->s->ops().setFusedCoarsest(on != 0, tile_variant);
+  });
+}
+
+int sfac_solver_fused_coarsest_active(sfac_solver* s, int* active) {
+  return guarded([&] {
+    need(s, "solver");
+    need(active, "active");
+    *active = s->s->ops().fusedCoarsestActive() ? 1 : 0;
+  });
+}
+
+int sfac_fused_tile_variants(int dim) { return sfac::box_tile_variants(dim); }
+
+int sfac_fused_tile_shape(int dim, int variant, int* shape3) {
+  return guarded([&] {
+    need(shape3, "shape");
+    if (dim != 2 && dim != 3) fail("sfac_fused_tile_shape: dim must be 2 or 3");
+    if (variant < 0 || variant >= sfac::box_tile_variants(dim))
+      fail("sfac_fused_tile_shape: variant out of range");
+    sfac::BoxTile t = sfac::box_tile(dim, variant);
+    shape3[0] = t.tx;
+    shape3[1] = t.ty;
+    shape3[2] = t.tz;
+  });
+}
+
 int sfac_solver_kernel_timing(sfac_solver* s, int level, long* launches, double* ms,
                               double* bytes) {
   return guarded([&] {

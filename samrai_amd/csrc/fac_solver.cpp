@@ -7,6 +7,7 @@
 // level operation is one or a few level-batched launches on one stream.
 #include "fac_solver.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -187,6 +188,7 @@ void CellPoissonFACOps::initializeOperatorState(const SAMRAIVectorReal& u,
   }
   d_partial_n = maxchunks;
   hip_check(hipMalloc(&d_partial, sizeof(double) * d_partial_n), "hipMalloc(partial)");
+  setupBoxPath();
 }
 
 void CellPoissonFACOps::deallocateOperatorState() {
@@ -199,7 +201,226 @@ void CellPoissonFACOps::deallocateOperatorState() {
   d_maxres.clear();
   if (d_partial) (void)hipFree(d_partial);
   d_partial = nullptr;
+  releaseBoxPath();
   d_ln_min = d_ln_max = -1;
+}
+
+// ---------------------------------------------- single-rectangle coarsest level
+
+void CellPoissonFACOps::setFusedCoarsest(bool on, int tile_variant) {
+  if (tile_variant < 0 || tile_variant >= sfac::box_tile_variants(d_h.dim()))
+    fail("CellPoissonFACOps: tile variant out of range");
+  d_box_enabled = on;
+  d_box_variant = tile_variant;
+}
+
+void CellPoissonFACOps::releaseBoxPath() {
+  for (double* p : {d_box.e[0], d_box.e[1], d_box.r, d_box.D[0], d_box.D[1], d_box.D[2], d_box.C,
+                    d_box.slots})
+    if (p) (void)hipFree(p);
+  if (d_box.mismatch) (void)hipFree(d_box.mismatch);
+  d_box = BoxPath{};
+}
+
+// The coarsest level qualifies when its patches tile its bounding box
+// exactly and that box is the physical domain: then every ghost of the
+// level is a same-level copy or a physical ghost.  In 2-D the colouring is
+// patch-relative (solv_cellpoissonfacops2d.f), so all patches must agree
+// on the parity of lo0 + lo1.
+void CellPoissonFACOps::setupBoxPath() {
+  releaseBoxPath();
+  const int ln = d_ln_min;
+  if (ln < 0) return;
+  const auto& L = d_h.level(ln);
+  const int dim = d_h.dim();
+  const int np = (int)L.boxes.size();
+  if (np == 0) return;
+  sfac::Box bb = L.boxes[0];
+  for (const auto& b : L.boxes)
+    for (int d = 0; d < dim; ++d) {
+      bb.lo[d] = std::min(bb.lo[d], b.lo[d]);
+      bb.hi[d] = std::max(bb.hi[d], b.hi[d]);
+    }
+  long n[3] = {1, 1, 1}, vol = 1, sum = 0;
+  for (int d = 0; d < dim; ++d) {
+    if (bb.lo[d] != L.dlo[d] || bb.hi[d] != L.dhi[d]) return;
+    n[d] = bb.hi[d] - bb.lo[d] + 1;
+    vol *= n[d];
+  }
+  if (vol > (1L << 30)) return;
+  for (const auto& b : L.boxes) {
+    long c = 1;
+    for (int d = 0; d < dim; ++d) c *= b.hi[d] - b.lo[d] + 1;
+    sum += c;
+  }
+  if (sum != vol) return;
+  // no overlaps (then the equal volume means the patches tile the box)
+  std::vector<unsigned char> cover((size_t)vol, 0);
+  for (const auto& b : L.boxes)
+    for (int k = (dim == 3 ? b.lo[2] : 0); k <= (dim == 3 ? b.hi[2] : 0); ++k)
+      for (int j = b.lo[1]; j <= b.hi[1]; ++j)
+        for (int i = b.lo[0]; i <= b.hi[0]; ++i) {
+          size_t o = (size_t)(i - bb.lo[0]) +
+                     (size_t)n[0] * ((j - bb.lo[1]) + (size_t)n[1] * (dim == 3 ? k - bb.lo[2] : 0));
+          if (cover[o]++) return;
+        }
+  int shift = 0;
+  if (dim == 2) {
+    shift = (L.boxes[0].lo[0] + L.boxes[0].lo[1]) & 1;
+    for (const auto& b : L.boxes)
+      if (((b.lo[0] + b.lo[1]) & 1) != shift) return;
+  }
+  BoxPath& B = d_box;
+  B.ln = ln;
+  B.G.dim = dim;
+  for (int d = 0; d < 3; ++d) {
+    B.G.n[d] = (int)n[d];
+    B.G.lo[d] = d < dim ? bb.lo[d] : 0;
+    B.G.dx[d] = L.dx[d];
+    const auto& last = L.boxes[np - 1];
+    B.G.lp_lo[d] = d < dim ? last.lo[d] - bb.lo[d] : 0;
+    B.G.lp_hi[d] = d < dim ? last.hi[d] - bb.lo[d] : 0;
+  }
+  B.G.shift = shift;
+  hip_check(hipMalloc(&B.e[0], sizeof(double) * vol), "hipMalloc(box e)");
+  hip_check(hipMalloc(&B.e[1], sizeof(double) * vol), "hipMalloc(box e)");
+  hip_check(hipMalloc(&B.r, sizeof(double) * vol), "hipMalloc(box r)");
+  // per-tile early-exit slots, double buffered, sized for the smallest tile
+  long maxtiles = 1;
+  for (int v = 0; v < sfac::box_tile_variants(dim); ++v) {
+    sfac::BoxTile t = sfac::box_tile(dim, v);
+    long nt = ((n[0] + t.tx - 1) / t.tx) * ((n[1] + t.ty - 1) / t.ty) *
+              (dim == 3 ? (n[2] + t.tz - 1) / t.tz : 1);
+    maxtiles = std::max(maxtiles, nt);
+  }
+  B.nslots = maxtiles;
+  hip_check(hipMalloc(&B.slots, sizeof(double) * 2 * maxtiles), "hipMalloc(box slots)");
+  hip_check(hipMalloc(&B.mismatch, sizeof(int)), "hipMalloc(box flag)");
+  B.ok = true;
+}
+
+int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num_sweeps,
+                                 double tol, double* maxres_out) {
+  BoxPath& B = d_box;
+  const int ln = B.ln;
+  const int dim = d_h.dim();
+  const auto& T = d_h.tables(ln);
+  const sfac::BoxLevel& G = B.G;
+  const long vol = (long)G.n[0] * G.n[1] * (dim == 3 ? G.n[2] : 1);
+  // coefficients: gathered on every call (the caller owns them); shared
+  // faces of variable D are checked once for agreement between patches
+  sfac::BoxArrays A{};
+  A.dconst = d_spec.getDConstant();
+  A.cmode = d_spec.cMode();
+  A.cconst = d_spec.getCConstant();
+  if (!d_spec.dIsConstant()) {
+    const PatchDataSet& D = *d_spec.getD();
+    for (int a = 0; a < dim; ++a) {
+      if (!B.D[a]) {
+        long nf = vol / G.n[a] * (G.n[a] + 1);
+        hip_check(hipMalloc(&B.D[a], sizeof(double) * nf), "hipMalloc(box D)");
+      }
+    }
+    if (!B.verified) sfac::launch_fill_int(B.mismatch, 1, 0, d_stream);
+    for (int a = 0; a < dim; ++a)
+      sfac::launch_box_side(T.geom, D.table(ln, a), D.ghost(), a, B.D[a], G,
+                            B.verified ? nullptr : B.mismatch, d_stream, !B.verified);
+    if (!B.verified) {
+      int bad = 0;
+      hip_check(hipMemcpyAsync(&bad, B.mismatch, sizeof(int), hipMemcpyDeviceToHost, d_stream),
+                "hipMemcpyAsync");
+      hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
+      if (bad) {  // patches disagree on shared faces: keep the patch-wise path
+        releaseBoxPath();
+        return -1;
+      }
+      B.verified = true;
+    }
+    for (int a = 0; a < dim; ++a) A.D[a] = B.D[a];
+  }
+  if (A.cmode == 2) {
+    if (!B.C) hip_check(hipMalloc(&B.C, sizeof(double) * vol), "hipMalloc(box C)");
+    const PatchDataSet& Cd = *d_spec.getC();
+    sfac::launch_box_cells(T.cells.launch(), T.geom, const_cast<double* const*>(Cd.table(ln)),
+                           Cd.ghost(), B.C, B.C, nullptr, 0, G, 1, d_stream);
+    A.C = B.C;
+  }
+  sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), B.e[0], B.e[0],
+                         nullptr, 0, G, 1, d_stream);
+  sfac::launch_box_cells(T.cells.launch(), T.geom, const_cast<double* const*>(r.table(ln)),
+                         r.ghost(), B.r, B.r, nullptr, 0, G, 1, d_stream);
+  A.r = B.r;
+  sfac::BoxCheck K{};
+  const sfac::BoxTile tile = sfac::box_tile(dim, d_box_variant);
+  const int tsz[3] = {tile.tx, tile.ty, dim == 3 ? tile.tz : 1};
+  K.nlp = 1;
+  for (int d = 0; d < 3; ++d) {
+    if (d < dim) {
+      K.tlo[d] = G.lp_lo[d] / tsz[d];
+      K.tn[d] = G.lp_hi[d] / tsz[d] - K.tlo[d] + 1;
+    } else {
+      K.tlo[d] = 0;
+      K.tn[d] = 1;
+    }
+    K.nlp *= K.tn[d];
+  }
+  K.tol = tol;
+  const bool check = tol >= 0.0;
+  if (check) {
+    sfac::launch_fill_int(d_stop, 2, 0, d_stream);
+    K.stop = d_stop;
+    K.sweeps = d_stop + 1;
+    K.maxres_out = d_scalars + 1;
+  }
+  const double launch_bytes = (d_spec.dIsConstant() ? 24.0 : 48.0) * (double)vol;
+  const sfac::RobinDev bc = robinDev();
+  for (int s = 0; s < num_sweeps; ++s) {
+    K.sweep = s;
+    A.ein = B.e[s & 1];
+    A.eout = B.e[(s + 1) & 1];
+    K.prev = (check && s > 0) ? B.slots + (long)((s - 1) & 1) * B.nslots : nullptr;
+    K.cur = check ? B.slots + (long)(s & 1) * B.nslots : nullptr;
+    TimedLaunch tl{};
+    if (d_timing) {
+      tl.start = nextEvent();
+      hip_check(hipEventRecord(tl.start, d_stream), "hipEventRecord");
+    }
+    sfac::launch_box_rbgs(G, A, bc, 1, K, d_box_variant, d_stream);
+    if (d_timing) {
+      tl.stop = nextEvent();
+      hip_check(hipEventRecord(tl.stop, d_stream), "hipEventRecord");
+      tl.level = ln;
+      tl.bytes = launch_bytes;
+      d_timed.push_back(tl);
+    }
+  }
+  if (check && num_sweeps > 0) {
+    K.cur = B.slots + (long)((num_sweeps - 1) & 1) * B.nslots;
+    sfac::launch_box_final_check(K, num_sweeps, d_stream);
+  }
+  // the sweeps done (known on the device when the loop may stop early)
+  // select the buffer holding the result
+  sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), B.e[0], B.e[1],
+                         (check && num_sweeps > 0) ? d_stop + 1 : nullptr, num_sweeps, G, 0,
+                         d_stream);
+  // edges/corners too, as the reference's last NoCoarse fill does
+  ghostFill(e, ln, false, true, false);
+  if (check && (maxres_out || d_logging)) {
+    int sweeps = 0;
+    double mr = 0.0;
+    if (num_sweeps > 0) {
+      hip_check(hipMemcpyAsync(&sweeps, d_stop + 1, sizeof(int), hipMemcpyDeviceToHost, d_stream),
+                "hipMemcpyAsync");
+      hip_check(hipMemcpyAsync(&mr, d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost,
+                               d_stream),
+                "hipMemcpyAsync");
+      hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
+    }
+    if (maxres_out) *maxres_out = mr;
+    if (d_logging) std::printf("CellPoissonFACOps RBGS smoothing maxres = %g\n  after %d sweeps.\n", mr, sweeps);
+    return sweeps;
+  }
+  return num_sweeps;
 }
 
 void CellPoissonFACOps::ghostFill(PatchDataSet& x, int ln, bool with_coarse, bool homogeneous,
@@ -223,6 +444,10 @@ void CellPoissonFACOps::ghostFill(PatchDataSet& x, int ln, bool with_coarse, boo
 
 int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet& r, int ln,
                                              int num_sweeps, double tol, double* maxres_out) {
+  if (d_box.ok && d_box_enabled && ln == d_box.ln) {
+    int n = smoothBox(e, r, num_sweeps, tol, maxres_out);
+    if (n >= 0) return n;
+  }
   const auto& T = d_h.tables(ln);
   const sfac::PoissonDev ps = poissonDev(ln);
   const int ewing = (d_ewing && ln > d_ln_min) ? 1 : 0;

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "box_kernels.h"
 #include "hierarchy.hpp"
 
 namespace samrai_amd {
@@ -173,6 +174,11 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   hipStream_t stream() const { return d_stream; }
   pdat::PatchDataSet& flux() { return *d_flux; }
 
+  // Colour-fused sweeps on the coarsest level when it is one rectangle
+  // (box_kernels.h): on by default; tile_variant picks the tile shape.
+  void setFusedCoarsest(bool on, int tile_variant);
+  bool fusedCoarsestActive() const { return d_box.ok && d_box_enabled; }
+
   // HIP-event timing of every red/black colour launch (on this stream),
   // with the algorithmic bytes of each launch: 12 B per cell of the level
   // for constant D (u read, f read, u write per full sweep = 24 B per cell),
@@ -194,6 +200,26 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   hipEvent_t nextEvent();
   sfac::PoissonDev poissonDev(int ln) const;
   sfac::RobinDev robinDev() const;
+  // single-rectangle coarsest level
+  struct BoxPath {
+    bool ok = false;
+    int ln = -1;
+    sfac::BoxLevel G{};
+    double* e[2] = {nullptr, nullptr};  // ping-pong sweep buffers
+    double* r = nullptr;
+    double* D[3] = {nullptr, nullptr, nullptr};
+    double* C = nullptr;
+    double* slots = nullptr;
+    long nslots = 0;
+    int* mismatch = nullptr;
+    bool verified = false;
+  } d_box;
+  bool d_box_enabled = true;
+  int d_box_variant = 0;
+  void setupBoxPath();
+  void releaseBoxPath();
+  int smoothBox(pdat::PatchDataSet& e, const pdat::PatchDataSet& r, int num_sweeps, double tol,
+                double* maxres_out);
   hier::PatchHierarchy& d_h;
   hipStream_t d_stream;
   std::string d_coarse_choice = "redblack";

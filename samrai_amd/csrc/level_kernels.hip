@@ -104,17 +104,35 @@ __device__ inline void atomic_max_nonneg(double* p, double v) {
 // (CellPoissonFACOps.C:1486-1508): red and black keep only the last patch's
 // max |residual| (redOrBlackSmoothingOnPatch overwrites it per patch), here
 // the max over the last patch's chunk slots.
+// Called by every thread of one work-group of kThreads threads.
 __device__ inline void sweep_check(const SweepCheck& k) {
-  if (*k.stop) return;
+  __shared__ double rs[kThreads / 64], bs[kThreads / 64];
+  const int done = *k.stop;
   double r = 0.0, b = 0.0;
-  for (int c = k.c0; c < k.c1; ++c) {
-    r = fmax(r, k.red[c]);
-    b = fmax(b, k.blk[c]);
+  if (!done)
+    for (int c = k.c0 + (int)threadIdx.x; c < k.c1; c += kThreads) {
+      r = fmax(r, k.red[c]);
+      b = fmax(b, k.blk[c]);
+    }
+  for (int s = 32; s > 0; s >>= 1) {
+    r = fmax(r, __shfl_xor(r, s, 64));
+    b = fmax(b, __shfl_xor(b, s, 64));
   }
-  double mx = r > b ? r : b;  // tbox::MathUtilities<double>::Max
-  if (k.maxres_out) *k.maxres_out = mx;
-  *k.sweeps += 1;
-  if (!(mx > k.tol)) *k.stop = 1;
+  if ((threadIdx.x & 63) == 0) {
+    rs[threadIdx.x >> 6] = r;
+    bs[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && !done) {
+    for (int w = 1; w < kThreads / 64; ++w) {
+      r = fmax(r, rs[w]);
+      b = fmax(b, bs[w]);
+    }
+    double mx = r > b ? r : b;  // tbox::MathUtilities<double>::Max
+    if (k.maxres_out) *k.maxres_out = mx;
+    *k.sweeps += 1;
+    if (!(mx > k.tol)) *k.stop = 1;
+  }
 }
 
 // ------------------------------------------------------------ ghost fill
@@ -124,7 +142,7 @@ k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* co
               const double* const* coarse, int cg, const Box* cbox, int homogeneous,
               RobinDev bc, const int* stop, SweepCheck chk) {
   if (stop && *stop) return;
-  if (chk.stop && blockIdx.x == 0 && threadIdx.x == 0) sweep_check(chk);
+  if (chk.stop && blockIdx.x == 0) sweep_check(chk);
   const Chunk ch = chunks[blockIdx.x];
   const BoxItem it = items[ch.item];
   const int kind = it.tag & 15, loc = it.tag >> 4;
@@ -660,9 +678,7 @@ k_set_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* cons
   }
 }
 
-__global__ void k_sweep_check(SweepCheck chk) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) sweep_check(chk);
-}
+__global__ void __launch_bounds__(kThreads) k_sweep_check(SweepCheck chk) { sweep_check(chk); }
 
 inline void check_launch() {
   hipError_t e = hipGetLastError();
@@ -886,7 +902,7 @@ void launch_set_items(const GhostLaunch& w, const LevelGeom& L, double* const* d
 }
 
 void launch_sweep_check(const SweepCheck& chk, hipStream_t s) {
-  hipLaunchKernelGGL(k_sweep_check, dim3(1), dim3(64), 0, s, chk);
+  hipLaunchKernelGGL(k_sweep_check, dim3(1), dim3(kThreads), 0, s, chk);
   check_launch();
 }
 

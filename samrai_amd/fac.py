@@ -76,6 +76,10 @@ def lib():
     L.sfac_solver_synchronize.argtypes = [_vp]
     L.sfac_solver_enable_kernel_timing.argtypes = [_vp, C.c_int]
     L.sfac_solver_kernel_timing.argtypes = [_vp, C.c_int, C.POINTER(C.c_long), _dp, _dp]
+    L.sfac_solver_set_fused_coarsest.argtypes = [_vp, C.c_int, C.c_int]
+    L.sfac_solver_fused_coarsest_active.argtypes = [_vp, _ip]
+    L.sfac_fused_tile_variants.argtypes = [C.c_int]
+    L.sfac_fused_tile_shape.argtypes = [C.c_int, C.c_int, _ip]
     L.sfac_ops_internal_vector.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
     L.sfac_ops_ghost_fill.argtypes = [_vp, _vp, C.c_int, C.c_int, C.c_int]
     L.sfac_ops_smooth.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int, C.c_double, _ip, _dp]
@@ -297,6 +301,16 @@ class CellPoissonFACSolver:
         """HIP-event timing of the red/black colour launches (solver stream)."""
         _check(lib().sfac_solver_enable_kernel_timing(self.s, int(on)))
 
+    def set_fused_coarsest(self, on=True, tile_variant=0):
+        """Colour-fused sweeps on a coarsest level that tiles the domain
+        (one launch per sweep, same results); off: patch-wise colour passes."""
+        _check(lib().sfac_solver_set_fused_coarsest(self.s, int(on), int(tile_variant)))
+
+    def fused_coarsest_active(self):
+        a = C.c_int()
+        _check(lib().sfac_solver_fused_coarsest_active(self.s, C.byref(a)))
+        return bool(a.value)
+
     def kernel_timing(self, level=-1):
         """(launches, total kernel ms, algorithmic bytes) of the timed colour
         launches on `level` (all levels when level < 0)."""
@@ -351,3 +365,14 @@ class CellPoissonFACSolver:
             self.close()
         except Exception:
             pass
+
+
+def fused_tile_shapes(dim):
+    """Tile shapes (tx, ty, tz) of the colour-fused coarsest-level sweep."""
+    L = lib()
+    out = []
+    for v in range(L.sfac_fused_tile_variants(dim)):
+        a = (C.c_int * 3)()
+        _check(L.sfac_fused_tile_shape(dim, v, a))
+        out.append(tuple(a))
+    return out

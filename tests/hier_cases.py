@@ -33,6 +33,14 @@ def case_2d_c0(n=32):
     return dict(name="2dc0", dim=2, levels=[L0, L1], ratio=2, dlo=(0, 0), dhi=(n - 1, n - 1))
 
 
+def case_2d_odd_split():
+    """Level 0 split at an odd x: the 2-D patch-relative colourings of the
+    two patches differ, so the coarsest level cannot be swept as one box."""
+    L0 = [((0, 0), (6, 15)), ((7, 0), (15, 15))]
+    L1 = [((8, 8), (15, 23))]
+    return dict(name="2doddsplit", dim=2, levels=[L0, L1], ratio=2, dlo=(0, 0), dhi=(15, 15))
+
+
 ALL = [case_2d, case_3d, case_3d_boundary]
 
 

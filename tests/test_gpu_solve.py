@@ -45,3 +45,39 @@ def test_solve_matches_oracle(gpu_lib, mk, spec):
     assert pr.s.number_of_iterations() == it_o
     np.testing.assert_allclose(hist_g, hist_o, rtol=1e-12, atol=0)
     pr.compare(pr.u, "u", interior_only=True, rtol=1e-12)
+
+
+FIXED = dict(PARAMS, max_cycles=4, relative_residual_tol=1e-300, coarse_solver_max_iterations=60)
+
+
+@pytest.mark.parametrize("spec", [dict(dvar=False), dict(dvar=True, cmode=2, robin=True)],
+                         ids=["constD", "varD-varC-robin"])
+@pytest.mark.parametrize("mk", hc.ALL + [hc.case_2d_c0, hc.case_2d_odd_split],
+                         ids=lambda f: f.__name__)
+def test_fused_coarsest_sweeps_are_bit_identical(gpu_lib, mk, spec):
+    """The colour-fused coarsest-level sweeps (every tile shape) give the
+    same bits as the patch-wise red pass / fill / black pass / fill."""
+    from samrai_amd import fac
+    case = mk()
+    spec = dict(spec)
+    if spec.pop("robin", False):
+        spec["bc"] = hc.bc_robin(case["dim"])
+    data = hc.make_data(case, seed=7)
+    shapes = fac.fused_tile_shapes(case["dim"])
+    ref = None
+    for on, v in [(False, 0)] + [(True, v) for v in range(len(shapes))]:
+        print("fused", case["name"], on, v, flush=True)
+        pr = hc.Pair(case, FIXED, data, **spec)
+        pr.s.set_fused_coarsest(on, v)
+        pr.s.solve_system(pr.u, pr.f)
+        assert pr.s.fused_coarsest_active() == (on and case["name"] != "2doddsplit")
+        u = [pr.u.get(ln, p) for ln in range(len(case["levels"]))
+             for p in range(len(case["levels"][ln]))]
+        got = (pr.s.number_of_iterations(), np.array(pr.s.history()), u)
+        if ref is None:
+            ref = got
+            continue
+        assert got[0] == ref[0]
+        np.testing.assert_array_equal(got[1], ref[1])
+        for a, b in zip(got[2], ref[2]):
+            np.testing.assert_array_equal(a, b)
