@@ -11,8 +11,8 @@ fluxes, conservative linear prolongation.
 
 Prints one JSON line: time to solution (value, s), smoother throughput, the
 roofline of the smoother kernel with the largest share of the time, measured
-with HIP events on the solver's stream over the timed steps, and the CPU
-baseline (the C oracle, a
+with HIP events on the solver's stream over one more (untimed) solve, and the
+CPU baseline (the C oracle, a
 bounded sample of FAC cycles on the same deck, projected to the cycle count
 the GPU needed).
 """
@@ -117,7 +117,6 @@ def main():
                                                         s.last_solve_seconds()),
               file=sys.stderr, flush=True)
 
-    s.enable_kernel_timing(True)
     converged = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -127,11 +126,16 @@ def main():
     s.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    per_level = [s.kernel_timing(ln) for ln in range(len(deck["levels"]))]
-    fused = s.fused_coarsest_active()
-    s.enable_kernel_timing(False)
     cycles = s.number_of_iterations()
     hist = s.history()
+    fused = s.fused_coarsest_active()
+    # one more solve with HIP events around every smoother launch (kept out
+    # of the timed steps: the events add host work between launches)
+    s.enable_kernel_timing(True)
+    u.fill(0.0)
+    s.solve_system(u, f)
+    per_level = [s.kernel_timing(ln) for ln in range(len(deck["levels"]))]
+    s.enable_kernel_timing(False)
     ms_per_step = elapsed * 1e3 / args.steps
     print("bench: %d steps, %.3f s/step, %d cycles, final residual %.3e" %
           (args.steps, ms_per_step / 1e3, cycles, hist[-1]), file=sys.stderr, flush=True)

@@ -253,6 +253,12 @@ int sfac_solver_fused_coarsest_active(sfac_solver* s, int* active);
 int sfac_fused_tile_variants(int dim);
 int sfac_fused_tile_shape(int dim, int variant, int* shape3);
 
+/* Execution choice with no reference counterpart (results are unchanged):
+ * on = 1 (default) captures one FACPreconditioner cycle (e = 0, V-cycle,
+ * u += e, restriction, composite residual and its norm reductions) as a HIP
+ * graph and replays it every cycle; kernel timing or logging turn it off. */
+int sfac_solver_set_graphs(sfac_solver* s, int on);
+
 /* FACOperatorStrategy operations of the GPU CellPoissonFACOps
  * (solv/FACOperatorStrategy.h:82-364), valid after sfac_solver_initialize.
  * which: 0 error, 1 residual, 2 temporary residual, 3 weights (borrowed). */

@@ -42,13 +42,39 @@ __device__ inline double robin_ghost(const BoxLevel& G, const RobinDev& bc, int 
   return (h * bc.g[loc] + uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
 }
 
-// Gauss-Seidel update of one cell held in LDS at q (strides 1, SY, SZ);
-// (i, j, k) are its box-relative indices.  Returns |residual|.
+// Per-cell inputs of one Gauss-Seidel update that do not change during a
+// sweep: right-hand side, the six face coefficients (variable D) and C.
+struct Coef {
+  double f, xl, xh, yl, yh, zl, zh, c;
+};
+
 template <int DIM, bool DVAR, int CMODE>
-__device__ inline double cell_update(double* U, int q, int SY, int SZ, int i, int j, int k,
-                                     const BoxLevel& G, const BoxArrays& A) {
+__device__ inline Coef load_coef(const BoxLevel& G, const BoxArrays& A, int i, int j, int k) {
+  Coef q{};
   const int n0 = G.n[0], n1 = G.n[1];
   const long c = (long)i + (long)n0 * (j + (long)n1 * k);
+  q.f = A.r[c];
+  if (DVAR) {
+    const long ox = (long)i + (long)(n0 + 1) * (j + (long)n1 * k);
+    const long oy = (long)i + (long)n0 * (j + (long)(n1 + 1) * k);
+    q.xl = A.D[0][ox];
+    q.xh = A.D[0][ox + 1];
+    q.yl = A.D[1][oy];
+    q.yh = A.D[1][oy + n0];
+    if (DIM == 3) {
+      q.zl = A.D[2][c];
+      q.zh = A.D[2][c + (long)n0 * n1];
+    }
+  }
+  if (CMODE == 2) q.c = A.C[c];
+  return q;
+}
+
+// Gauss-Seidel update of one cell held in LDS at q (strides 1, SY, SZ),
+// in the expression order of rbgswithfluxmax*.  Returns |residual|.
+template <int DIM, bool DVAR, int CMODE>
+__device__ inline double cell_update(double* U, int q, int SY, int SZ, const Coef& k,
+                                     const BoxLevel& G, const BoxArrays& A) {
   const double dxi = 1.0 / G.dx[0], dyi = 1.0 / G.dx[1];
   const double dzi = DIM == 3 ? 1.0 / G.dx[2] : 0.0;
   const double dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
@@ -56,15 +82,13 @@ __device__ inline double cell_update(double* U, int q, int SY, int SZ, int i, in
   const double uc = U[q];
   double Dxl, Dxh, Dyl, Dyh, Dzl = 0.0, Dzh = 0.0;
   if (DVAR) {
-    const long ox = (long)i + (long)(n0 + 1) * (j + (long)n1 * k);
-    const long oy = (long)i + (long)n0 * (j + (long)(n1 + 1) * k);
-    Dxl = A.D[0][ox];
-    Dxh = A.D[0][ox + 1];
-    Dyl = A.D[1][oy];
-    Dyh = A.D[1][oy + n0];
+    Dxl = k.xl;
+    Dxh = k.xh;
+    Dyl = k.yl;
+    Dyh = k.yh;
     if (DIM == 3) {
-      Dzl = A.D[2][c];
-      Dzh = A.D[2][c + (long)n0 * n1];
+      Dzl = k.zl;
+      Dzh = k.zh;
     }
   } else {
     Dxl = Dxh = Dyl = Dyh = Dzl = Dzh = dc;
@@ -79,9 +103,9 @@ __device__ inline double cell_update(double* U, int q, int SY, int SZ, int i, in
     fzh = dzi * Dzh * (U[q + SZ] - uc);
   }
   double scal;
-  if (CMODE == 2) scal = A.C[c];
+  if (CMODE == 2) scal = k.c;
   else scal = CMODE == 1 ? A.cconst : 0.0;
-  const double fv = A.r[c];
+  const double fv = k.f;
   double residual, diag;
   if (DIM == 3) {
     residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
@@ -152,12 +176,63 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
     if (s_stop) return;
   }
 
-  // 2. load the grown tile (outside the level: zero, ghosts set below)
-  for (int q = threadIdx.x; q < LX * LY * LZ; q += kThreads) {
+  // colour parity: red <=> (ilev + jlev + klev + shift) even; in LDS
+  // coordinates the sum is base + lx + ly + lz
+  const int base = G.lo[0] + G.lo[1] + (DIM == 3 ? G.lo[2] : 0) + x0 + y0 + z0 - 4 - ZH + G.shift;
+  const int pb = pos2(base);
+  // red cells: half-rows of the tile grown by one; black: of the owned tile
+  constexpr int HX = (TX + 2) / 2, RY = TY + 2, RZ = DIM == 3 ? TZ + 2 : 1;
+  constexpr int NRW = HX * RY * RZ, NR = (NRW + kThreads - 1) / kThreads;
+  constexpr int HB = TX / 2, BZ = DIM == 3 ? TZ : 1;
+  constexpr int NBW = HB * TY * BZ, NB = (NBW + kThreads - 1) / kThreads;
+  constexpr int NTOT = LX * LY * LZ, NL = (NTOT + kThreads - 1) / kThreads;
+  auto red_cell = [&](int w, int& lx, int& ly, int& lz) {
+    const int m = w % HX;
+    ly = 1 + (w / HX) % RY;
+    lz = DIM == 3 ? 1 + w / (HX * RY) : 0;
+    lx = 1 + pos2(pb + 1 + ly + lz) + 2 * m;
+  };
+  auto black_cell = [&](int w, int& lx, int& ly, int& lz) {
+    const int m = w % HB;
+    ly = 2 + (w / HB) % TY;
+    lz = DIM == 3 ? 2 + w / (HB * TY) : 0;
+    lx = 3 - pos2(pb + ly + lz) + 2 * m;
+  };
+  auto inside = [&](int i, int j, int k) {
+    return i >= 0 && i < n0 && j >= 0 && j < n1 && (DIM == 2 || (k >= 0 && k < n2));
+  };
+
+  // 2. issue every global load of the sweep up front: the grown tile of u
+  // and the coefficients of each cell this thread updates
+  double v[NL];
+#pragma unroll
+  for (int u = 0; u < NL; ++u) {
+    const int q = threadIdx.x + u * kThreads;
     const int lx = q % LX, ly = (q / LX) % LY, lz = q / (LX * LY);
     const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
-    const bool in = i >= 0 && i < n0 && j >= 0 && j < n1 && (DIM == 2 || (k >= 0 && k < n2));
-    U[q] = in ? A.ein[(long)i + (long)n0 * (j + (long)n1 * k)] : 0.0;
+    v[u] = (q < NTOT && inside(i, j, k)) ? A.ein[(long)i + (long)n0 * (j + (long)n1 * k)] : 0.0;
+  }
+  Coef cr[NR], cb[NB];
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    const int w = threadIdx.x + u * kThreads;
+    int lx, ly, lz;
+    red_cell(w, lx, ly, lz);
+    const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
+    cr[u] = (w < NRW && inside(i, j, k)) ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef{};
+  }
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int w = threadIdx.x + u * kThreads;
+    int lx, ly, lz;
+    black_cell(w, lx, ly, lz);
+    const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
+    cb[u] = (w < NBW && inside(i, j, k)) ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef{};
+  }
+#pragma unroll
+  for (int u = 0; u < NL; ++u) {
+    const int q = threadIdx.x + u * kThreads;
+    if (q < NTOT) U[q] = v[u];
   }
   __syncthreads();
 
@@ -190,10 +265,6 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
   ghosts();
   __syncthreads();
 
-  // colour parity: red <=> (ilev + jlev + klev + shift) even; in LDS
-  // coordinates the sum is base + lx + ly + lz
-  const int base = G.lo[0] + G.lo[1] + (DIM == 3 ? G.lo[2] : 0) + x0 + y0 + z0 - 4 - ZH + G.shift;
-  const int pb = pos2(base);
   const bool lp_tile = K.cur != nullptr;
   double mr = 0.0;
   auto in_lp = [&](int i, int j, int k) {
@@ -202,17 +273,15 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
   };
 
   // 3. red cells of the tile grown by one
-  {
-    constexpr int HX = (TX + 2) / 2;
-    constexpr int RY = TY + 2, RZ = DIM == 3 ? TZ + 2 : 1;
-    for (int w = threadIdx.x; w < HX * RY * RZ; w += kThreads) {
-      const int m = w % HX, ly = 1 + (w / HX) % RY, lz = DIM == 3 ? 1 + w / (HX * RY) : 0;
-      const int lx = 1 + pos2(pb + 1 + ly + lz) + 2 * m;
-      const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
-      if (i < 0 || i >= n0 || j < 0 || j >= n1) continue;
-      if (DIM == 3 && (k < 0 || k >= n2)) continue;
+#pragma unroll
+  for (int u = 0; u < NR; ++u) {
+    const int w = threadIdx.x + u * kThreads;
+    int lx, ly, lz;
+    red_cell(w, lx, ly, lz);
+    const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
+    if (w < NRW && inside(i, j, k)) {
       const double ra =
-          cell_update<DIM, DVAR, CMODE>(U, lx + SY * ly + SZ * lz, SY, SZ, i, j, k, G, A);
+          cell_update<DIM, DVAR, CMODE>(U, lx + SY * ly + SZ * lz, SY, SZ, cr[u], G, A);
       const bool owned = lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2 &&
                          (DIM == 2 || (lz >= 2 && lz < TZ + 2));
       if (lp_tile && owned && in_lp(i, j, k)) mr = fmax(mr, ra);
@@ -223,16 +292,15 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
   __syncthreads();
 
   // 5. black cells of the owned tile
-  {
-    constexpr int HB = TX / 2;
-    constexpr int BZ = DIM == 3 ? TZ : 1;
-    for (int w = threadIdx.x; w < HB * TY * BZ; w += kThreads) {
-      const int m = w % HB, ly = 2 + (w / HB) % TY, lz = DIM == 3 ? 2 + w / (HB * TY) : 0;
-      const int lx = 3 - pos2(pb + ly + lz) + 2 * m;
-      const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
-      if (i >= n0 || j >= n1 || (DIM == 3 && k >= n2)) continue;
+#pragma unroll
+  for (int u = 0; u < NB; ++u) {
+    const int w = threadIdx.x + u * kThreads;
+    int lx, ly, lz;
+    black_cell(w, lx, ly, lz);
+    const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
+    if (w < NBW && inside(i, j, k)) {
       const double ra =
-          cell_update<DIM, DVAR, CMODE>(U, lx + SY * ly + SZ * lz, SY, SZ, i, j, k, G, A);
+          cell_update<DIM, DVAR, CMODE>(U, lx + SY * ly + SZ * lz, SY, SZ, cb[u], G, A);
       if (lp_tile && in_lp(i, j, k)) mr = fmax(mr, ra);
     }
   }

@@ -356,6 +356,13 @@ int sfac_solver_fused_coarsest_active(sfac_solver* s, int* active) {
   });
 }
 
+int sfac_solver_set_graphs(sfac_solver* s, int on) {
+  return guarded([&] {
+    need(s, "solver");
+    s->s->ops().setUseGraphs(on != 0);
+  });
+}
+
 int sfac_fused_tile_variants(int dim) { return sfac::box_tile_variants(dim); }
 
 int sfac_fused_tile_shape(int dim, int variant, int* shape3) {

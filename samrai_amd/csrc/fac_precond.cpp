@@ -20,6 +20,11 @@ FACPreconditioner::FACPreconditioner(FACOperatorStrategy& ops, CellPoissonFACOps
 
 FACPreconditioner::~FACPreconditioner() { deallocateSolverState(); }
 
+void FACPreconditioner::releaseCycleGraph() {
+  if (d_cycle_exec) (void)hipGraphExecDestroy(d_cycle_exec);
+  d_cycle_exec = nullptr;
+}
+
 void FACPreconditioner::setMaxCycles(int n) {
   if (!(n >= 1)) fail("FACPreconditioner: max_cycles out of range");
   d_max_cycles = n;
@@ -61,6 +66,7 @@ void FACPreconditioner::initializeSolverState(const SAMRAIVectorReal& u,
 }
 
 void FACPreconditioner::deallocateSolverState() {
+  releaseCycleGraph();
   if (d_initialized) d_ops.deallocateOperatorState();
   d_e_data.reset();
   d_r_data.reset();
@@ -93,6 +99,25 @@ void FACPreconditioner::facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal
   d_ops.smoothError(e, r, ln, d_post);
 }
 
+// One pass of the solveSystem loop body up to the residual norm's device
+// reductions (the host read-back is rmsNormFinish): e = 0, V-cycle, u += e,
+// restrict u, full composite residual (FACPreconditioner.C:378-420).  It
+// enqueues work on one stream only, allocates nothing and never waits, so
+// it can be captured as one HIP graph and replayed every cycle.
+void FACPreconditioner::enqueueCycle(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
+  d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
+  facCycle_Recursive(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
+  d_gpu->add(*u.data, *d_e.data, d_lmin, d_lmax);
+  for (int ln = d_lmax - 1; ln >= d_lmin; --ln) d_ops.restrictSolution(u, u, ln);
+  d_ops.computeCompositeResidualOnLevel(d_r, u, f, d_lmax, false);
+  for (int ln = d_lmax - 1; ln >= d_lmin; --ln) {
+    d_ops.restrictSolution(u, u, ln);
+    d_ops.computeCompositeResidualOnLevel(d_r, u, f, ln, false);
+    d_ops.restrictResidual(d_r, d_r, ln);
+  }
+  d_gpu->rmsNormEnqueue(*d_r.data, *d_r.weights, d_lmin, d_lmax);
+}
+
 void FACPreconditioner::runOneCycle(SAMRAIVectorReal& u) {
   d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
   facCycle_Recursive(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
@@ -116,13 +141,37 @@ bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
   if ((int)d_factors.size() < d_max_cycles) d_factors.resize(d_max_cycles);
   d_history.clear();
   d_iterations = 0;
+  const hipStream_t stream = d_gpu->stream();
   do {
-    d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
-    facCycle_Recursive(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
-    d_gpu->add(*u.data, *d_e.data, d_lmin, d_lmax);
-    for (int ln = d_lmax - 1; ln >= d_lmin; --ln) d_ops.restrictSolution(u, u, ln);
     d_factors[d_iterations] = d_residual_norm;
-    d_residual_norm = computeFullCompositeResidual(d_r, u, f);
+    if (d_gpu->graphsUsable()) {
+      if (!d_cycle_exec || d_cycle_gen != d_gpu->generation() || d_cycle_u != u.data ||
+          d_cycle_f != f.data) {
+        releaseCycleGraph();
+        hipGraph_t g = nullptr;
+        hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+        try {
+          enqueueCycle(u, f);
+        } catch (...) {
+          (void)hipStreamEndCapture(stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+        hipError_t err = hipGraphInstantiate(&d_cycle_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (err != hipSuccess) d_cycle_exec = nullptr;
+        hip_check(err, "hipGraphInstantiate");
+        d_cycle_gen = d_gpu->generation();
+        d_cycle_u = u.data;
+        d_cycle_f = f.data;
+      }
+      hip_check(hipGraphLaunch(d_cycle_exec, stream), "hipGraphLaunch");
+    } else {
+      enqueueCycle(u, f);
+    }
+    d_residual_norm = d_gpu->rmsNormFinish();
     d_history.push_back(d_residual_norm);
     if (d_factors[d_iterations] != 0)
       d_factors[d_iterations] = d_residual_norm / d_factors[d_iterations];

@@ -100,17 +100,20 @@ void CellPoissonFACOps::setCoarsestLevelSolverChoice(const std::string& c) {
   if (c != "redblack" && c != "jacobi" && c != "hypre")
     fail("CellPoissonFACOps: bad coarse_solver_choice '" + c + "'");
   d_coarse_choice = c;
+  ++d_generation;
 }
 
 void CellPoissonFACOps::setCoarseFineDiscretization(const std::string& c) {
   if (c != "Ewing") fail("CellPoissonFACOps: cf_discretization '" + c + "' not available");
   d_ewing = true;
+  ++d_generation;
 }
 
 void CellPoissonFACOps::setProlongationMethod(const std::string& m) {
   if (m != "CONSERVATIVE_LINEAR_REFINE" && m != "CONSTANT_REFINE")
     fail("CellPoissonFACOps: prolongation_method '" + m + "' not available");
   d_prolongation = m;
+  ++d_generation;
 }
 
 sfac::PoissonDev CellPoissonFACOps::poissonDev(int ln) const {
@@ -142,6 +145,7 @@ sfac::RobinDev CellPoissonFACOps::robinDev() const {
 void CellPoissonFACOps::initializeOperatorState(const SAMRAIVectorReal& u,
                                                 const SAMRAIVectorReal& f) {
   deallocateOperatorState();
+  ++d_generation;
   d_ln_min = u.coarsest;
   d_ln_max = u.finest;
   if (f.coarsest != d_ln_min || f.finest != d_ln_max)
@@ -192,6 +196,7 @@ void CellPoissonFACOps::initializeOperatorState(const SAMRAIVectorReal& u,
 }
 
 void CellPoissonFACOps::deallocateOperatorState() {
+  ++d_generation;
   d_flux.reset();
   for (double* p : d_oflux_slab) (void)hipFree(p);
   for (double** p : d_oflux_tab) (void)hipFree(p);
@@ -212,6 +217,7 @@ void CellPoissonFACOps::setFusedCoarsest(bool on, int tile_variant) {
     fail("CellPoissonFACOps: tile variant out of range");
   d_box_enabled = on;
   d_box_variant = tile_variant;
+  ++d_generation;
 }
 
 void CellPoissonFACOps::releaseBoxPath() {
@@ -296,7 +302,27 @@ void CellPoissonFACOps::setupBoxPath() {
   B.nslots = maxtiles;
   hip_check(hipMalloc(&B.slots, sizeof(double) * 2 * maxtiles), "hipMalloc(box slots)");
   hip_check(hipMalloc(&B.mismatch, sizeof(int)), "hipMalloc(box flag)");
+  if (d_spec.cMode() == 2) hip_check(hipMalloc(&B.C, sizeof(double) * vol), "hipMalloc(box C)");
   B.ok = true;
+  if (!d_spec.dIsConstant()) {
+    // variable D: the patches must agree on the faces they share (each
+    // patch's cells read its own copy in the patch-wise path)
+    const PatchDataSet& D = *d_spec.getD();
+    const auto& T = d_h.tables(ln);
+    for (int a = 0; a < dim; ++a) {
+      long nf = vol / n[a] * (n[a] + 1);
+      hip_check(hipMalloc(&B.D[a], sizeof(double) * nf), "hipMalloc(box D)");
+    }
+    sfac::launch_fill_int(B.mismatch, 1, 0, d_stream);
+    for (int a = 0; a < dim; ++a)
+      sfac::launch_box_side(T.geom, D.table(ln, a), D.ghost(), a, B.D[a], B.G, B.mismatch,
+                            d_stream, 1);
+    int bad = 0;
+    hip_check(hipMemcpyAsync(&bad, B.mismatch, sizeof(int), hipMemcpyDeviceToHost, d_stream),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
+    if (bad) releaseBoxPath();
+  }
 }
 
 int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num_sweeps,
@@ -307,8 +333,9 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
   const auto& T = d_h.tables(ln);
   const sfac::BoxLevel& G = B.G;
   const long vol = (long)G.n[0] * G.n[1] * (dim == 3 ? G.n[2] : 1);
-  // coefficients: gathered on every call (the caller owns them); shared
-  // faces of variable D are checked once for agreement between patches
+  // coefficients: gathered on every call (the caller owns them and may
+  // change their values between solves); no allocation or host wait here,
+  // so the sequence can be captured in a graph
   sfac::BoxArrays A{};
   A.dconst = d_spec.getDConstant();
   A.cmode = d_spec.cMode();
@@ -316,30 +343,12 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
   if (!d_spec.dIsConstant()) {
     const PatchDataSet& D = *d_spec.getD();
     for (int a = 0; a < dim; ++a) {
-      if (!B.D[a]) {
-        long nf = vol / G.n[a] * (G.n[a] + 1);
-        hip_check(hipMalloc(&B.D[a], sizeof(double) * nf), "hipMalloc(box D)");
-      }
+      sfac::launch_box_side(T.geom, D.table(ln, a), D.ghost(), a, B.D[a], G, nullptr, d_stream,
+                            0);
+      A.D[a] = B.D[a];
     }
-    if (!B.verified) sfac::launch_fill_int(B.mismatch, 1, 0, d_stream);
-    for (int a = 0; a < dim; ++a)
-      sfac::launch_box_side(T.geom, D.table(ln, a), D.ghost(), a, B.D[a], G,
-                            B.verified ? nullptr : B.mismatch, d_stream, !B.verified);
-    if (!B.verified) {
-      int bad = 0;
-      hip_check(hipMemcpyAsync(&bad, B.mismatch, sizeof(int), hipMemcpyDeviceToHost, d_stream),
-                "hipMemcpyAsync");
-      hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
-      if (bad) {  // patches disagree on shared faces: keep the patch-wise path
-        releaseBoxPath();
-        return -1;
-      }
-      B.verified = true;
-    }
-    for (int a = 0; a < dim; ++a) A.D[a] = B.D[a];
   }
   if (A.cmode == 2) {
-    if (!B.C) hip_check(hipMalloc(&B.C, sizeof(double) * vol), "hipMalloc(box C)");
     const PatchDataSet& Cd = *d_spec.getC();
     sfac::launch_box_cells(T.cells.launch(), T.geom, const_cast<double* const*>(Cd.table(ln)),
                            Cd.ghost(), B.C, B.C, nullptr, 0, G, 1, d_stream);
@@ -444,10 +453,8 @@ void CellPoissonFACOps::ghostFill(PatchDataSet& x, int ln, bool with_coarse, boo
 
 int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet& r, int ln,
                                              int num_sweeps, double tol, double* maxres_out) {
-  if (d_box.ok && d_box_enabled && ln == d_box.ln) {
-    int n = smoothBox(e, r, num_sweeps, tol, maxres_out);
-    if (n >= 0) return n;
-  }
+  if (d_box.ok && d_box_enabled && ln == d_box.ln)
+    return smoothBox(e, r, num_sweeps, tol, maxres_out);
   const auto& T = d_h.tables(ln);
   const sfac::PoissonDev ps = poissonDev(ln);
   const int ewing = (d_ewing && ln > d_ln_min) ? 1 : 0;
@@ -596,6 +603,12 @@ void CellPoissonFACOps::prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAI
 
 double CellPoissonFACOps::rmsNorm(const PatchDataSet& x, const PatchDataSet& w, int coarse_ln,
                                   int fine_ln) {
+  rmsNormEnqueue(x, w, coarse_ln, fine_ln);
+  return rmsNormFinish();
+}
+
+void CellPoissonFACOps::rmsNormEnqueue(const PatchDataSet& x, const PatchDataSet& w,
+                                       int coarse_ln, int fine_ln) {
   for (int pass = 0; pass < 2; ++pass)
     for (int ln = coarse_ln; ln <= fine_ln; ++ln) {
       const auto& T = d_h.tables(ln);
@@ -605,6 +618,9 @@ double CellPoissonFACOps::rmsNorm(const PatchDataSet& x, const PatchDataSet& w, 
       sfac::launch_sum(d_partial, T.cells.nchunks, d_scalars + 2 * pass, ln > coarse_ln,
                        d_stream);
     }
+}
+
+double CellPoissonFACOps::rmsNormFinish() {
   hip_check(hipMemcpyAsync(h_pinned, d_scalars, sizeof(double) * 3, hipMemcpyDeviceToHost,
                            d_stream),
             "hipMemcpyAsync");
@@ -657,6 +673,7 @@ hipEvent_t CellPoissonFACOps::nextEvent() {
 
 void CellPoissonFACOps::enableKernelTiming(bool on) {
   d_timing = on;
+  ++d_generation;
   d_timed.clear();
   d_event_next = 0;
 }

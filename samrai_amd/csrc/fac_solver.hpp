@@ -130,13 +130,19 @@ class CellPoissonFACOps : public FACOperatorStrategy {
 
   // input keys of CellPoissonFACOps::getFromInput (CellPoissonFACOps.C:691-739)
   void setCoarsestLevelSolverChoice(const std::string& c);
-  void setCoarsestLevelSolverTolerance(double t) { d_coarse_tol = t; }
-  void setCoarsestLevelSolverMaxIterations(int n) { d_coarse_max = n; }
+  void setCoarsestLevelSolverTolerance(double t) { d_coarse_tol = t; ++d_generation; }
+  void setCoarsestLevelSolverMaxIterations(int n) { d_coarse_max = n; ++d_generation; }
   void setCoarseFineDiscretization(const std::string& c);
   void setProlongationMethod(const std::string& m);
-  void setPoissonSpecifications(const PoissonSpecifications& s) { d_spec = s; }
-  void setPhysicalBcCoefObject(const LocationIndexRobinBcCoefs& bc) { d_bc = bc; }
-  void enableLogging(bool on) { d_logging = on; }
+  void setPoissonSpecifications(const PoissonSpecifications& s) { d_spec = s; ++d_generation; }
+  void setPhysicalBcCoefObject(const LocationIndexRobinBcCoefs& bc) { d_bc = bc; ++d_generation; }
+  void enableLogging(bool on) { d_logging = on; ++d_generation; }
+  // Whole FAC cycles may be captured in HIP graphs and replayed (no kernel
+  // timing or logging then).  generation() changes whenever anything a
+  // captured cycle depends on changes.
+  void setUseGraphs(bool on) { d_use_graphs = on; ++d_generation; }
+  bool graphsUsable() const { return d_use_graphs && !d_timing && !d_logging; }
+  unsigned long generation() const { return d_generation; }
 
   void computeVectorWeights(pdat::PatchDataSet& w, int coarsest_ln, int finest_ln);
 
@@ -168,6 +174,10 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   // weighted RMS over levels [coarse_ln, fine_ln]
   double rmsNorm(const pdat::PatchDataSet& x, const pdat::PatchDataSet& w, int coarse_ln,
                  int fine_ln);
+  // the same in two halves: the device reductions, then the host read-back
+  void rmsNormEnqueue(const pdat::PatchDataSet& x, const pdat::PatchDataSet& w, int coarse_ln,
+                      int fine_ln);
+  double rmsNormFinish();
   // HierarchyCellDataOpsReal helpers (interior unless noted)
   void setToScalar(pdat::PatchDataSet& x, double v, int coarse_ln, int fine_ln);  // incl. ghosts
   void add(pdat::PatchDataSet& y, const pdat::PatchDataSet& x, int coarse_ln, int fine_ln);
@@ -215,6 +225,8 @@ class CellPoissonFACOps : public FACOperatorStrategy {
     bool verified = false;
   } d_box;
   bool d_box_enabled = true;
+  bool d_use_graphs = true;
+  unsigned long d_generation = 0;
   int d_box_variant = 0;
   void setupBoxPath();
   void releaseBoxPath();
@@ -250,8 +262,8 @@ class FACPreconditioner {
   void setMaxCycles(int n);
   void setResidualTolerance(double t);
   void setRelativeResidualTolerance(double t);
-  void setPresmoothingSweeps(int n) { d_pre = n; }
-  void setPostsmoothingSweeps(int n) { d_post = n; }
+  void setPresmoothingSweeps(int n) { d_pre = n; releaseCycleGraph(); }
+  void setPostsmoothingSweeps(int n) { d_post = n; releaseCycleGraph(); }
   void setAlgorithmChoice(const std::string& c);
 
   void initializeSolverState(const SAMRAIVectorReal& u, const SAMRAIVectorReal& f);
@@ -276,6 +288,13 @@ class FACPreconditioner {
  private:
   void facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal& r, SAMRAIVectorReal& u,
                           int lmax, int lmin, int ln);
+  // one cycle of solveSystem up to the residual norm's device reductions
+  void enqueueCycle(SAMRAIVectorReal& u, SAMRAIVectorReal& f);
+  void releaseCycleGraph();
+  hipGraphExec_t d_cycle_exec = nullptr;
+  unsigned long d_cycle_gen = 0;
+  const void* d_cycle_u = nullptr;
+  const void* d_cycle_f = nullptr;
   FACOperatorStrategy& d_ops;
   CellPoissonFACOps* d_gpu;
   int d_max_cycles = 10;

@@ -78,6 +78,7 @@ def lib():
     L.sfac_solver_kernel_timing.argtypes = [_vp, C.c_int, C.POINTER(C.c_long), _dp, _dp]
     L.sfac_solver_set_fused_coarsest.argtypes = [_vp, C.c_int, C.c_int]
     L.sfac_solver_fused_coarsest_active.argtypes = [_vp, _ip]
+    L.sfac_solver_set_graphs.argtypes = [_vp, C.c_int]
     L.sfac_fused_tile_variants.argtypes = [C.c_int]
     L.sfac_fused_tile_shape.argtypes = [C.c_int, C.c_int, _ip]
     L.sfac_ops_internal_vector.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
@@ -305,6 +306,10 @@ class CellPoissonFACSolver:
         """Colour-fused sweeps on a coarsest level that tiles the domain
         (one launch per sweep, same results); off: patch-wise colour passes."""
         _check(lib().sfac_solver_set_fused_coarsest(self.s, int(on), int(tile_variant)))
+
+    def set_graphs(self, on=True):
+        """Capture each FAC cycle as a HIP graph and replay it (same results)."""
+        _check(lib().sfac_solver_set_graphs(self.s, int(on)))
 
     def fused_coarsest_active(self):
         a = C.c_int()

@@ -55,8 +55,10 @@ FIXED = dict(PARAMS, max_cycles=4, relative_residual_tol=1e-300, coarse_solver_m
 @pytest.mark.parametrize("mk", hc.ALL + [hc.case_2d_c0, hc.case_2d_odd_split],
                          ids=lambda f: f.__name__)
 def test_fused_coarsest_sweeps_are_bit_identical(gpu_lib, mk, spec):
-    """The colour-fused coarsest-level sweeps (every tile shape) give the
-    same bits as the patch-wise red pass / fill / black pass / fill."""
+    """The colour-fused coarsest-level sweeps (every tile shape), with and
+    without whole-cycle HIP graphs, give the same bits as the patch-wise red
+    pass / fill / black pass / fill launched eagerly.  A graph is replayed
+    once per cycle, with the host's norm read-back between replays."""
     from samrai_amd import fac
     case = mk()
     spec = dict(spec)
@@ -65,10 +67,13 @@ def test_fused_coarsest_sweeps_are_bit_identical(gpu_lib, mk, spec):
     data = hc.make_data(case, seed=7)
     shapes = fac.fused_tile_shapes(case["dim"])
     ref = None
-    for on, v in [(False, 0)] + [(True, v) for v in range(len(shapes))]:
-        print("fused", case["name"], on, v, flush=True)
+    runs = [(False, 0, False), (False, 0, True), (True, 0, False)]
+    runs += [(True, v, True) for v in range(len(shapes))]
+    for on, v, graphs in runs:
+        print("fused", case["name"], on, v, graphs, flush=True)
         pr = hc.Pair(case, FIXED, data, **spec)
         pr.s.set_fused_coarsest(on, v)
+        pr.s.set_graphs(graphs)
         pr.s.solve_system(pr.u, pr.f)
         assert pr.s.fused_coarsest_active() == (on and case["name"] != "2doddsplit")
         u = [pr.u.get(ln, p) for ln in range(len(case["levels"]))
