@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--smoother-sweeps", type=int, default=200)
+    ap.add_argument("--mode", choices=["solve", "smoother"], default="solve",
+                    help="smoother: only a few smoother launches and a 256 MiB copy of known "
+                         "size (for hardware-counter passes); prints no benchmark line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -109,6 +112,21 @@ def main():
     print("bench: deck %s, %d patches, %d cells; building" % (deck["name"], npatch, ncells),
           file=sys.stderr, flush=True)
     h, u, f, s = build_problem(fac, decks, deck)
+    if args.mode == "smoother":
+        # a plain copy of known size calibrates the byte counters
+        x = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        e = s.internal_vector("e")
+        r = s.internal_vector("r")
+        r.fill(1.0)
+        s.smooth(e, r, 0, 20)
+        s.smooth(e, r, len(deck["levels"]) - 1, 5)
+        s.synchronize()
+        print("bench: smoother mode done (copy 256 MiB, 20 coarsest sweeps, 5 finest sweeps)",
+              file=sys.stderr, flush=True)
+        return
 
     for w in range(args.warmup):
         u.fill(0.0)

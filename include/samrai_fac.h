@@ -232,10 +232,12 @@ int sfac_solver_get_history(sfac_solver* s, double* hist, int max, int* n);
 int sfac_solver_last_solve_seconds(sfac_solver* s, double* seconds);
 int sfac_solver_synchronize(sfac_solver* s);
 /* Instrumentation (no reference counterpart; the tbox::Timer analogue for the
- * smoother kernel): HIP events around every red/black colour launch on the
- * solver's stream.  kernel_timing returns, for one level (or all: level < 0),
- * the launch count, summed kernel milliseconds and summed algorithmic bytes
- * (12 B per level cell per colour launch at constant D, 24 B at variable D). */
+ * smoother kernel): HIP events on the solver's stream around every red/black
+ * colour launch, and around each batch of back-to-back colour-fused sweeps.
+ * kernel_timing returns, for one level (or all: level < 0), the launch count,
+ * summed milliseconds and summed algorithmic bytes (per level cell: 12 B per
+ * colour launch, 24 B per fused sweep at constant D; twice that at variable
+ * D).  While timing is on, cycles are not captured in graphs. */
 int sfac_solver_enable_kernel_timing(sfac_solver* s, int on);
 int sfac_solver_kernel_timing(sfac_solver* s, int level, long* launches, double* ms,
                               double* bytes);

@@ -189,10 +189,12 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   void setFusedCoarsest(bool on, int tile_variant);
   bool fusedCoarsestActive() const { return d_box.ok && d_box_enabled; }
 
-  // HIP-event timing of every red/black colour launch (on this stream),
-  // with the algorithmic bytes of each launch: 12 B per cell of the level
-  // for constant D (u read, f read, u write per full sweep = 24 B per cell),
-  // 24 B with variable D (+ three face coefficients).
+  // HIP-event timing of the smoother launches (on this stream), with the
+  // algorithmic bytes of each launch: a colour pass moves 12 B per cell of
+  // the level at constant D (u read, f read, u write per full sweep = 24 B
+  // per cell), 24 B with variable D (+ three face coefficients); a fused
+  // sweep moves the full-sweep figure.  Fused sweeps run back to back and
+  // are timed as one batch per smoothing call.
   void enableKernelTiming(bool on);
   // launches, summed duration (ms) and algorithmic bytes; level < 0: all.
   void kernelTiming(int level, long* launches, double* ms, double* bytes);
@@ -201,7 +203,8 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   struct TimedLaunch {
     hipEvent_t start, stop;
     int level;
-    double bytes;
+    int count;     // launches between the two events
+    double bytes;  // algorithmic bytes of those launches
   };
   bool d_timing = false;
   std::vector<TimedLaunch> d_timed;
