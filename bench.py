@@ -147,6 +147,7 @@ def main():
     cycles = s.number_of_iterations()
     hist = s.history()
     fused = s.fused_coarsest_active()
+    tile = fac.fused_tile_shapes(3)[s.fused_tile_variant()] if fused else None
     # one more solve with HIP events around every smoother launch (kept out
     # of the timed steps: the events add host work between launches)
     s.enable_kernel_timing(True)
@@ -218,7 +219,7 @@ def main():
                max_error_vs_exact=err,
                smoother_gcells_per_s=round(smoother_gcells, 2),
                finest_smoother_gcells_per_s=round(finest_gcells, 2),
-               fused_coarsest=fused,
+               fused_coarsest=fused, fused_tile=tile,
                roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(out), flush=True)
 

@@ -247,11 +247,13 @@ int sfac_solver_kernel_timing(sfac_solver* s, int level, long* launches, double*
  * its red-black sweeps run colour-fused over one contiguous copy of the
  * level, one launch per sweep (the timing above then counts 24 / 48 B per
  * cell per launch).  on = 0 keeps the patch-wise colour passes.
- * tile_variant selects the LDS tile shape (0 .. sfac_fused_tile_variants-1);
- * fused_coarsest_active reports whether the fused path is in use after
- * sfac_solver_initialize. */
+ * tile_variant selects the LDS tile shape (0 .. sfac_fused_tile_variants-1),
+ * or -1 (default) for the autotuner, which times every shape once per level
+ * shape and keeps the fastest.  fused_coarsest_active / fused_tile_variant
+ * report the path and shape in use after sfac_solver_initialize. */
 int sfac_solver_set_fused_coarsest(sfac_solver* s, int on, int tile_variant);
 int sfac_solver_fused_coarsest_active(sfac_solver* s, int* active);
+int sfac_solver_fused_tile_variant(sfac_solver* s, int* variant);
 int sfac_fused_tile_variants(int dim);
 int sfac_fused_tile_shape(int dim, int variant, int* shape3);
 

@@ -395,8 +395,11 @@ void launch_tile(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int 
   check_launch();
 }
 
-constexpr BoxTile kTiles3[] = {{32, 8, 8}, {16, 8, 8}, {32, 16, 4}, {64, 4, 4}};
-constexpr BoxTile kTiles2[] = {{64, 16, 1}, {32, 16, 1}, {32, 32, 1}, {128, 8, 1}};
+constexpr int kVariants = 6;
+constexpr BoxTile kTiles3[kVariants] = {{32, 8, 8}, {16, 8, 8}, {32, 16, 4},
+                                        {64, 4, 4}, {32, 8, 4}, {16, 16, 4}};
+constexpr BoxTile kTiles2[kVariants] = {{64, 16, 1}, {32, 16, 1}, {32, 32, 1},
+                                        {128, 8, 1}, {64, 8, 1}, {16, 16, 1}};
 
 template <int DIM, bool DVAR, int CMODE>
 void launch_variant(int variant, const BoxLevel& G, const BoxArrays& A, const RobinDev& bc,
@@ -406,6 +409,8 @@ void launch_variant(int variant, const BoxLevel& G, const BoxArrays& A, const Ro
       case 1: launch_tile<3, DVAR, CMODE, 16, 8, 8>(G, A, bc, homogeneous, K, s); break;
       case 2: launch_tile<3, DVAR, CMODE, 32, 16, 4>(G, A, bc, homogeneous, K, s); break;
       case 3: launch_tile<3, DVAR, CMODE, 64, 4, 4>(G, A, bc, homogeneous, K, s); break;
+      case 4: launch_tile<3, DVAR, CMODE, 32, 8, 4>(G, A, bc, homogeneous, K, s); break;
+      case 5: launch_tile<3, DVAR, CMODE, 16, 16, 4>(G, A, bc, homogeneous, K, s); break;
       default: launch_tile<3, DVAR, CMODE, 32, 8, 8>(G, A, bc, homogeneous, K, s); break;
     }
   } else {
@@ -413,6 +418,8 @@ void launch_variant(int variant, const BoxLevel& G, const BoxArrays& A, const Ro
       case 1: launch_tile<2, DVAR, CMODE, 32, 16, 1>(G, A, bc, homogeneous, K, s); break;
       case 2: launch_tile<2, DVAR, CMODE, 32, 32, 1>(G, A, bc, homogeneous, K, s); break;
       case 3: launch_tile<2, DVAR, CMODE, 128, 8, 1>(G, A, bc, homogeneous, K, s); break;
+      case 4: launch_tile<2, DVAR, CMODE, 64, 8, 1>(G, A, bc, homogeneous, K, s); break;
+      case 5: launch_tile<2, DVAR, CMODE, 16, 16, 1>(G, A, bc, homogeneous, K, s); break;
       default: launch_tile<2, DVAR, CMODE, 64, 16, 1>(G, A, bc, homogeneous, K, s); break;
     }
   }
@@ -435,10 +442,10 @@ void launch_dim(int variant, const BoxLevel& G, const BoxArrays& A, const RobinD
 
 }  // namespace
 
-int box_tile_variants(int) { return 4; }
+int box_tile_variants(int) { return kVariants; }
 
 BoxTile box_tile(int dim, int variant) {
-  if (variant < 0 || variant > 3) variant = 0;
+  if (variant < 0 || variant >= kVariants) variant = 0;
   return dim == 3 ? kTiles3[variant] : kTiles2[variant];
 }
 

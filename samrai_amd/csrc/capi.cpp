@@ -363,6 +363,14 @@ int sfac_solver_set_graphs(sfac_solver* s, int on) {
   });
 }
 
+int sfac_solver_fused_tile_variant(sfac_solver* s, int* variant) {
+  return guarded([&] {
+    need(s, "solver");
+    need(variant, "variant");
+    *variant = s->s->ops().fusedTileVariant();
+  });
+}
+
 int sfac_fused_tile_variants(int dim) { return sfac::box_tile_variants(dim); }
 
 int sfac_fused_tile_shape(int dim, int variant, int* shape3) {

@@ -8,6 +8,9 @@
 #include "fac_solver.hpp"
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -213,11 +216,72 @@ void CellPoissonFACOps::deallocateOperatorState() {
 // ---------------------------------------------- single-rectangle coarsest level
 
 void CellPoissonFACOps::setFusedCoarsest(bool on, int tile_variant) {
-  if (tile_variant < 0 || tile_variant >= sfac::box_tile_variants(d_h.dim()))
+  if (tile_variant < -1 || tile_variant >= sfac::box_tile_variants(d_h.dim()))
     fail("CellPoissonFACOps: tile variant out of range");
   d_box_enabled = on;
   d_box_variant = tile_variant;
+  if (d_box.ok) d_box.variant = tile_variant >= 0 ? tile_variant : tuneBoxVariant();
   ++d_generation;
+}
+
+// Autotuner: time 8 back-to-back fused sweeps of every tile shape on the
+// level's own arrays (zeroed; timing does not depend on the values) and keep
+// the fastest.  The choice is cached per (dim, level shape, D and C kind).
+int CellPoissonFACOps::tuneBoxVariant() {
+  BoxPath& B = d_box;
+  const int dim = d_h.dim();
+  const sfac::BoxLevel& G = B.G;
+  const long vol = (long)G.n[0] * G.n[1] * (dim == 3 ? G.n[2] : 1);
+  const std::array<long, 6> key = {dim, G.n[0], G.n[1], G.n[2], d_spec.dIsConstant() ? 0 : 1,
+                                   d_spec.cMode()};
+  static std::mutex mu;
+  static std::map<std::array<long, 6>, int> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  sfac::BoxArrays A{};
+  A.dconst = d_spec.getDConstant();
+  A.cmode = d_spec.cMode();
+  A.cconst = d_spec.getCConstant();
+  for (int a = 0; a < dim; ++a) A.D[a] = B.D[a];
+  if (A.cmode == 2) {
+    sfac::launch_fill(B.C, vol, 0.0, d_stream);
+    A.C = B.C;
+  }
+  sfac::launch_fill(B.e[0], vol, 0.0, d_stream);
+  sfac::launch_fill(B.e[1], vol, 0.0, d_stream);
+  sfac::launch_fill(B.r, vol, 0.0, d_stream);
+  A.r = B.r;
+  const sfac::BoxCheck K{};
+  const sfac::RobinDev bc = robinDev();
+  hipEvent_t t0, t1;
+  hip_check(hipEventCreate(&t0), "hipEventCreate");
+  hip_check(hipEventCreate(&t1), "hipEventCreate");
+  int best = 0;
+  float best_ms = 0.0f;
+  for (int v = 0; v < sfac::box_tile_variants(dim); ++v) {
+    for (int i = 0; i < 10; ++i) {
+      if (i == 2) hip_check(hipEventRecord(t0, d_stream), "hipEventRecord");
+      A.ein = B.e[i & 1];
+      A.eout = B.e[(i + 1) & 1];
+      sfac::launch_box_rbgs(G, A, bc, 1, K, v, d_stream);
+    }
+    hip_check(hipEventRecord(t1, d_stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+    if (v == 0 || ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  std::lock_guard<std::mutex> lk(mu);
+  cache[key] = best;
+  return best;
 }
 
 void CellPoissonFACOps::releaseBoxPath() {
@@ -321,8 +385,12 @@ void CellPoissonFACOps::setupBoxPath() {
     hip_check(hipMemcpyAsync(&bad, B.mismatch, sizeof(int), hipMemcpyDeviceToHost, d_stream),
               "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
-    if (bad) releaseBoxPath();
+    if (bad) {
+      releaseBoxPath();
+      return;
+    }
   }
+  B.variant = d_box_variant >= 0 ? d_box_variant : tuneBoxVariant();
 }
 
 int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num_sweeps,
@@ -360,7 +428,7 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
                          r.ghost(), B.r, B.r, nullptr, 0, G, 1, d_stream);
   A.r = B.r;
   sfac::BoxCheck K{};
-  const sfac::BoxTile tile = sfac::box_tile(dim, d_box_variant);
+  const sfac::BoxTile tile = sfac::box_tile(dim, B.variant);
   const int tsz[3] = {tile.tx, tile.ty, dim == 3 ? tile.tz : 1};
   K.nlp = 1;
   for (int d = 0; d < 3; ++d) {
@@ -396,7 +464,7 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
     A.eout = B.e[(s + 1) & 1];
     K.prev = (check && s > 0) ? B.slots + (long)((s - 1) & 1) * B.nslots : nullptr;
     K.cur = check ? B.slots + (long)(s & 1) * B.nslots : nullptr;
-    sfac::launch_box_rbgs(G, A, bc, 1, K, d_box_variant, d_stream);
+    sfac::launch_box_rbgs(G, A, bc, 1, K, B.variant, d_stream);
   }
   if (d_timing && num_sweeps > 0) {
     tl.stop = nextEvent();

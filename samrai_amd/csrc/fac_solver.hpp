@@ -185,9 +185,12 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   pdat::PatchDataSet& flux() { return *d_flux; }
 
   // Colour-fused sweeps on the coarsest level when it is one rectangle
-  // (box_kernels.h): on by default; tile_variant picks the tile shape.
+  // (box_kernels.h): on by default; tile_variant picks the tile shape, -1
+  // lets the autotuner time every shape once per level shape and keep the
+  // fastest.
   void setFusedCoarsest(bool on, int tile_variant);
   bool fusedCoarsestActive() const { return d_box.ok && d_box_enabled; }
+  int fusedTileVariant() const { return d_box.ok ? d_box.variant : -1; }
 
   // HIP-event timing of the smoother launches (on this stream), with the
   // algorithmic bytes of each launch: a colour pass moves 12 B per cell of
@@ -225,12 +228,13 @@ class CellPoissonFACOps : public FACOperatorStrategy {
     double* slots = nullptr;
     long nslots = 0;
     int* mismatch = nullptr;
-    bool verified = false;
+    int variant = 0;
   } d_box;
   bool d_box_enabled = true;
+  int tuneBoxVariant();
   bool d_use_graphs = true;
   unsigned long d_generation = 0;
-  int d_box_variant = 0;
+  int d_box_variant = -1;
   void setupBoxPath();
   void releaseBoxPath();
   int smoothBox(pdat::PatchDataSet& e, const pdat::PatchDataSet& r, int num_sweeps, double tol,

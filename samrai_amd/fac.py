@@ -78,6 +78,7 @@ def lib():
     L.sfac_solver_kernel_timing.argtypes = [_vp, C.c_int, C.POINTER(C.c_long), _dp, _dp]
     L.sfac_solver_set_fused_coarsest.argtypes = [_vp, C.c_int, C.c_int]
     L.sfac_solver_fused_coarsest_active.argtypes = [_vp, _ip]
+    L.sfac_solver_fused_tile_variant.argtypes = [_vp, _ip]
     L.sfac_solver_set_graphs.argtypes = [_vp, C.c_int]
     L.sfac_fused_tile_variants.argtypes = [C.c_int]
     L.sfac_fused_tile_shape.argtypes = [C.c_int, C.c_int, _ip]
@@ -302,14 +303,20 @@ class CellPoissonFACSolver:
         """HIP-event timing of the red/black colour launches (solver stream)."""
         _check(lib().sfac_solver_enable_kernel_timing(self.s, int(on)))
 
-    def set_fused_coarsest(self, on=True, tile_variant=0):
+    def set_fused_coarsest(self, on=True, tile_variant=-1):
         """Colour-fused sweeps on a coarsest level that tiles the domain
-        (one launch per sweep, same results); off: patch-wise colour passes."""
+        (one launch per sweep, same results); off: patch-wise colour passes.
+        tile_variant -1: autotuned tile shape."""
         _check(lib().sfac_solver_set_fused_coarsest(self.s, int(on), int(tile_variant)))
 
     def set_graphs(self, on=True):
         """Capture each FAC cycle as a HIP graph and replay it (same results)."""
         _check(lib().sfac_solver_set_graphs(self.s, int(on)))
+
+    def fused_tile_variant(self):
+        v = C.c_int()
+        _check(lib().sfac_solver_fused_tile_variant(self.s, C.byref(v)))
+        return v.value
 
     def fused_coarsest_active(self):
         a = C.c_int()
