@@ -35,7 +35,7 @@ PARAMS = dict(max_cycles=200, residual_tol=1e-300, relative_residual_tol=1e-10,
               cf_discretization="Ewing", prolongation_method="CONSERVATIVE_LINEAR_REFINE")
 
 
-def build_problem(fac, decks, deck):
+def build_problem(fac, decks, deck, params=None):
     h = fac.PatchHierarchy(deck["dim"], deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
     for boxes in deck["levels"]:
         h.add_level(deck["ratio"], boxes)
@@ -45,11 +45,15 @@ def build_problem(fac, decks, deck):
     for ln, boxes in enumerate(deck["levels"]):
         for p, box in enumerate(boxes):
             f.put(ln, p, decks.sine_rhs(deck, ln, box))
-    s = fac.CellPoissonFACSolver(h, **PARAMS)
+    s = fac.CellPoissonFACSolver(h, **(params or PARAMS))
     s.set_d_constant(1.0)
     s.set_dirichlet(0.0)
     s.initialize_solver_state(u, f)
     return h, u, f, s
+
+
+MG_PARAMS = dict(PARAMS, coarse_solver_choice="mg", coarse_solver_tolerance=1e-8,
+                 coarse_solver_max_iterations=10)
 
 
 def max_error(decks, deck, u):
@@ -62,15 +66,17 @@ def max_error(decks, deck, u):
     return err
 
 
-def cpu_baseline(decks, deck, cycles_gpu, ncycles, threads):
+def cpu_baseline(decks, deck, cycles_gpu, ncycles, threads, mg=False):
     from oracle.oracle import OracleFAC
     o = OracleFAC(3, deck["levels"], deck["ratio"], deck["dlo"], deck["dhi"], [0.0] * 3,
                   [1.0] * 3)
     o.set_poisson(False, 1.0, 0, 0.0)
     o.set_bc([1.0] * 6, [0.0] * 6, [0.0] * 6)
+    P = MG_PARAMS if mg else PARAMS
     o.set_params(max_cycles=ncycles, residual_tol=1e-300, relative_residual_tol=1e-10,
-                 pre=2, post=2, coarse_max_iterations=500, coarse_tol=1e-12, ewing=True,
-                 nthreads=threads)
+                 pre=2, post=2, coarse_max_iterations=P["coarse_solver_max_iterations"],
+                 coarse_tol=P["coarse_solver_tolerance"], ewing=True, nthreads=threads)
+    o.set_coarse_mg(mg)
     for ln, boxes in enumerate(deck["levels"]):
         for p, box in enumerate(boxes):
             o.put("f", ln, p, decks.sine_rhs(deck, ln, box))
@@ -148,6 +154,8 @@ def main():
     hist = s.history()
     fused = s.fused_coarsest_active()
     tile = fac.fused_tile_shapes(3)[s.fused_tile_variant()] if fused else None
+    tuning = {"x".join(map(str, sh)): round(ms * 1e3, 2)
+              for sh, ms in zip(fac.fused_tile_shapes(3), s.fused_tile_timings())}
     # one more solve with HIP events around every smoother launch (kept out
     # of the timed steps: the events add host work between launches)
     s.enable_kernel_timing(True)
@@ -195,6 +203,27 @@ def main():
                     launches=dom["launches"], avg_launch_us=dom["avg_launch_us"],
                     bytes_per_launch=dom["bytes_per_launch"], per_level=levels)
 
+    # the same deck with the multigrid coarsest-level solver (the reference's
+    # default coarse solver is hypre; "mg" is this library's stand-in for it)
+    h2, u2, f2, s2 = build_problem(fac, decks, deck, MG_PARAMS)
+    u2.fill(0.0)
+    s2.solve_system(u2, f2)
+    mg_times = []
+    for k in range(args.steps):
+        u2.fill(0.0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        ok2 = s2.solve_system(u2, f2)
+        s2.synchronize()
+        mg_times.append(time.perf_counter() - t2)
+    mg = dict(s_per_solve=round(sum(mg_times) / len(mg_times), 4), cycles=s2.number_of_iterations(),
+              converged=bool(ok2), final_residual=float(s2.history()[-1]),
+              max_error_vs_exact=max_error(decks, deck, u2),
+              coarse="mg V(2,2), rel tol 1e-8, <= 10 V-cycles",
+              last_coarse_vcycles=s2.coarse_mg_stats()[0])
+    print("bench: mg coarse solver: %.4f s/solve, %d cycles" % (mg["s_per_solve"], mg["cycles"]),
+          file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -204,6 +233,8 @@ def main():
         print("bench: CPU baseline (%d cycles, %d threads)" % (args.cpu_cycles, threads),
               file=sys.stderr, flush=True)
         cpu = cpu_baseline(decks, deck, cycles, args.cpu_cycles, threads)
+        mg["cpu_baseline"] = cpu_baseline(decks, deck, mg["cycles"], args.cpu_cycles, threads,
+                                          mg=True)
 
     out = dict(metric="time to 1e-10 rel residual (s)", value=round(ms_per_step / 1e3, 4),
                unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
@@ -219,8 +250,8 @@ def main():
                max_error_vs_exact=err,
                smoother_gcells_per_s=round(smoother_gcells, 2),
                finest_smoother_gcells_per_s=round(finest_gcells, 2),
-               fused_coarsest=fused, fused_tile=tile,
-               roofline=roofline, cpu_baseline=cpu)
+               fused_coarsest=fused, fused_tile=tile, fused_tile_autotune_us_per_sweep=tuning,
+               roofline=roofline, cpu_baseline=cpu, with_mg_coarse_solver=mg)
     print(json.dumps(out), flush=True)
 
 

@@ -193,7 +193,7 @@ typedef struct {
   double relative_residual_tol;
   int num_pre_sweeps;
   int num_post_sweeps;
-  const char* coarse_solver_choice;  /* "redblack" | "jacobi" */
+  const char* coarse_solver_choice;  /* "redblack" | "jacobi" | "mg" | "hypre" (= "mg") */
   double coarse_solver_tolerance;
   int coarse_solver_max_iterations;
   const char* cf_discretization;     /* "Ewing" */
@@ -254,8 +254,20 @@ int sfac_solver_kernel_timing(sfac_solver* s, int level, long* launches, double*
 int sfac_solver_set_fused_coarsest(sfac_solver* s, int on, int tile_variant);
 int sfac_solver_fused_coarsest_active(sfac_solver* s, int* active);
 int sfac_solver_fused_tile_variant(sfac_solver* s, int* variant);
+/* The autotuner's measured milliseconds per sweep of each tile shape (n = 0
+ * when the shape was set explicitly). */
+int sfac_solver_fused_tile_timings(sfac_solver* s, double* ms_per_sweep, int max, int* n);
 int sfac_fused_tile_variants(int dim);
 int sfac_fused_tile_shape(int dim, int variant, int* shape3);
+
+/* coarse_solver_choice "mg" (no reference counterpart; "hypre" selects it
+ * too, hypre not being part of this build): geometric multigrid on a
+ * coarsest level that tiles the physical domain, V(2,2) red-black cycles
+ * until ||r||_2 <= coarse_solver_tolerance ||b||_2 or
+ * coarse_solver_max_iterations cycles -- the stopping test of the reference's
+ * hypre path (CellPoissonFACOps::solveCoarsestLevel_HYPRE, .C:1740-1780).
+ * coarse_mg_stats: V-cycles and relative residual of the last coarse solve. */
+int sfac_solver_coarse_mg_stats(sfac_solver* s, int* iterations, double* relres);
 
 /* Execution choice with no reference counterpart (results are unchanged):
  * on = 1 (default) captures one FACPreconditioner cycle (e = 0, V-cycle,

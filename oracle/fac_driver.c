@@ -74,6 +74,9 @@ struct orc_fac {
   int max_cycles; double residual_tol, rel_tol; int pre, post;
   int cmax; double ctol; int ewing; int nthreads;
   double last_seconds;
+  int coarse_mg;                 /* coarsest level by multigrid (mg_solve) */
+  int mg_iterations;             /* V-cycles of the last coarse solve */
+  double mg_relres;              /* its final relative residual */
 };
 
 /* ------------------------------------------------------------ helpers */
@@ -781,9 +784,341 @@ static double full_composite_residual(orc_fac* s) {
   return rms(s, F_R);
 }
 
+/* ------------------------------------------------ multigrid coarse solve
+ *
+ * Geometric multigrid for the coarsest level (coarse_solver_choice "mg",
+ * the stand-in for hypre PFMG): the level is gathered into one box array;
+ * MG level m+1 halves every extent; V(2,2) cycles of red-black Gauss-Seidel
+ * (red: i+j+k even in box indices), full-weighting restriction of the
+ * residual (mean of the children), separable linear prolongation (3/4, 1/4
+ * per axis, homogeneous Robin ghosts beyond the domain), and 50 sweeps on a
+ * grid of at most 8 cells per axis; stop when ||r||_2 <= ctol ||b||_2 or
+ * after cmax cycles (hypre's relative-residual test).  Per-cell arithmetic
+ * follows the red-black kernel (rbgswithfluxmax*, expression order kept).
+ */
+typedef struct {
+  int n[3];
+  double dx[3];
+  double *e, *f, *D[3], *C;
+} MGL;
+
+#define MG_PRE 2
+#define MG_POST 2
+#define MG_SMALL_EDGE 8
+#define MG_SMALL_SWEEPS 50
+
+static double mg_ghost(const orc_fac* s, const MGL* G, int loc, double uin) {
+  const int n = loc >> 1;
+  const double h = G->dx[n], av = s->bca[loc], bv = s->bcb[loc];
+  if (s->dim == 3) return (0.0 + uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
+  return (uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
+}
+
+static long mg_idx(const MGL* G, int i, int j, int k) {
+  return (long)i + (long)G->n[0] * (j + (long)G->n[1] * k);
+}
+
+/* u with homogeneous Robin ghosts one cell beyond the box along any axis */
+static double mg_at(const orc_fac* s, const MGL* G, const double* u, int i, int j, int k) {
+  int c[3] = {i, j, k}, l[3] = {0, 0, 0};
+  for (int a = 0; a < s->dim; ++a) l[a] = c[a] < 0 ? 0 : (c[a] >= G->n[a] ? G->n[a] - 1 : c[a]);
+  double v = u[mg_idx(G, l[0], l[1], l[2])];
+  for (int a = 0; a < s->dim; ++a) {
+    if (c[a] < 0) v = mg_ghost(s, G, 2 * a, v);
+    else if (c[a] >= G->n[a]) v = mg_ghost(s, G, 2 * a + 1, v);
+  }
+  return v;
+}
+
+/* face coefficients and C of one cell */
+static void mg_coef(const orc_fac* s, const MGL* G, int i, int j, int k, double* D6,
+                    double* scal) {
+  const int n0 = G->n[0], n1 = G->n[1];
+  if (s->dvar) {
+    long ox = (long)i + (long)(n0 + 1) * (j + (long)n1 * k);
+    long oy = (long)i + (long)n0 * (j + (long)(n1 + 1) * k);
+    D6[0] = G->D[0][ox]; D6[1] = G->D[0][ox + 1];
+    D6[2] = G->D[1][oy]; D6[3] = G->D[1][oy + n0];
+    if (s->dim == 3) {
+      long c = mg_idx(G, i, j, k);
+      D6[4] = G->D[2][c]; D6[5] = G->D[2][c + (long)n0 * n1];
+    } else {
+      D6[4] = D6[5] = 0.0;
+    }
+  } else {
+    for (int q = 0; q < 6; ++q) D6[q] = s->dconst;
+    if (s->dim == 2) D6[4] = D6[5] = 0.0;
+  }
+  if (s->cmode == 2) *scal = G->C[mg_idx(G, i, j, k)];
+  else *scal = s->cmode == 1 ? s->cconst : 0.0;
+}
+
+/* residual (update = 0) or Gauss-Seidel update (update = 1) of one cell */
+static double mg_cell(const orc_fac* s, const MGL* G, double* u, int i, int j, int k,
+                      int update) {
+  const int dim = s->dim;
+  double D6[6], scal;
+  mg_coef(s, G, i, j, k, D6, &scal);
+  const double dxi = 1.0 / G->dx[0], dyi = 1.0 / G->dx[1];
+  const double dzi = dim == 3 ? 1.0 / G->dx[2] : 0.0;
+  const double dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
+  const long c = mg_idx(G, i, j, k);
+  const double uc = u[c];
+  const double uxl = mg_at(s, G, u, i - 1, j, k), uxh = mg_at(s, G, u, i + 1, j, k);
+  const double uyl = mg_at(s, G, u, i, j - 1, k), uyh = mg_at(s, G, u, i, j + 1, k);
+  const double fxl = dxi * D6[0] * (uc - uxl);
+  const double fxh = dxi * D6[1] * (uxh - uc);
+  const double fyl = dyi * D6[2] * (uc - uyl);
+  const double fyh = dyi * D6[3] * (uyh - uc);
+  double fzl = 0.0, fzh = 0.0;
+  if (dim == 3) {
+    const double uzl = mg_at(s, G, u, i, j, k - 1), uzh = mg_at(s, G, u, i, j, k + 1);
+    fzl = dzi * D6[4] * (uc - uzl);
+    fzh = dzi * D6[5] * (uzh - uc);
+  }
+  const double fv = G->f[c];
+  double residual, diag;
+  const double dc = s->dconst;
+  if (dim == 3) {
+    residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
+    if (s->dvar)
+      diag = (dxi2 * (D6[1] + D6[0]) + dyi2 * (D6[3] + D6[2]) + dzi2 * (D6[5] + D6[4])) - scal;
+    else
+      diag = (dxi2 * (dc + dc) + dyi2 * (dc + dc) + dzi2 * (dc + dc)) - scal;
+  } else {
+    residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
+    if (s->dvar)
+      diag = (dxi * dxi * (D6[1] + D6[0]) + dyi * dyi * (D6[3] + D6[2])) - scal;
+    else
+      diag = (dxi * dxi * (dc + dc) + dyi * dyi * (dc + dc)) - scal;
+  }
+  if (update) {
+    const double dudr = 1.0 / diag;
+    const double du = -residual * dudr;
+    u[c] = uc + du * 1.0;
+  }
+  return residual;
+}
+
+static void mg_sweeps(const orc_fac* s, const MGL* G, double* u, int sweeps) {
+  const int n2 = s->dim == 3 ? G->n[2] : 1;
+  for (int sw = 0; sw < sweeps; ++sw)
+    for (int colour = 0; colour < 2; ++colour)
+      for (int k = 0; k < n2; ++k)
+        for (int j = 0; j < G->n[1]; ++j)
+          for (int i = 0; i < G->n[0]; ++i)
+            if (((i + j + k) & 1) == colour) mg_cell(s, G, u, i, j, k, 1);
+}
+
+static void mg_vcycle(const orc_fac* s, MGL* M, int nm, int m, int small_direct) {
+  MGL* G = &M[m];
+  const int dim = s->dim;
+  const long ncell = (long)G->n[0] * G->n[1] * (dim == 3 ? G->n[2] : 1);
+  if (m == nm - 1) {
+    if (m > 0) memset(G->e, 0, sizeof(double) * (size_t)ncell);
+    mg_sweeps(s, G, G->e, small_direct ? MG_SMALL_SWEEPS : 64);
+    return;
+  }
+  if (m > 0) memset(G->e, 0, sizeof(double) * (size_t)ncell);
+  mg_sweeps(s, G, G->e, MG_PRE);
+  MGL* Cg = &M[m + 1];
+  const int c2n = dim == 3 ? Cg->n[2] : 1;
+  for (int K = 0; K < c2n; ++K)
+    for (int J = 0; J < Cg->n[1]; ++J)
+      for (int I = 0; I < Cg->n[0]; ++I) {
+        double sum = 0.0;
+        for (int q2 = 0; q2 < (dim == 3 ? 2 : 1); ++q2)
+          for (int q1 = 0; q1 < 2; ++q1)
+            for (int q0 = 0; q0 < 2; ++q0)
+              sum += mg_cell(s, G, G->e, 2 * I + q0, 2 * J + q1, dim == 3 ? 2 * K + q2 : 0, 0);
+        Cg->f[mg_idx(Cg, I, J, K)] = sum * (dim == 3 ? 0.125 : 0.25);
+      }
+  mg_vcycle(s, M, nm, m + 1, small_direct);
+  const int n2 = dim == 3 ? G->n[2] : 1;
+  for (int k = 0; k < n2; ++k)
+    for (int j = 0; j < G->n[1]; ++j)
+      for (int i = 0; i < G->n[0]; ++i) {
+        const int I = i >> 1, J = j >> 1, K = k >> 1;
+        const int sx = (i & 1) ? 1 : -1, sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
+        double v;
+        if (dim == 3) {
+          double z0 = 0.75 * (0.75 * mg_at(s, Cg, Cg->e, I, J, K) +
+                              0.25 * mg_at(s, Cg, Cg->e, I + sx, J, K)) +
+                      0.25 * (0.75 * mg_at(s, Cg, Cg->e, I, J + sy, K) +
+                              0.25 * mg_at(s, Cg, Cg->e, I + sx, J + sy, K));
+          double z1 = 0.75 * (0.75 * mg_at(s, Cg, Cg->e, I, J, K + sz) +
+                              0.25 * mg_at(s, Cg, Cg->e, I + sx, J, K + sz)) +
+                      0.25 * (0.75 * mg_at(s, Cg, Cg->e, I, J + sy, K + sz) +
+                              0.25 * mg_at(s, Cg, Cg->e, I + sx, J + sy, K + sz));
+          v = 0.75 * z0 + 0.25 * z1;
+        } else {
+          v = 0.75 * (0.75 * mg_at(s, Cg, Cg->e, I, J, 0) + 0.25 * mg_at(s, Cg, Cg->e, I + sx, J, 0)) +
+              0.25 * (0.75 * mg_at(s, Cg, Cg->e, I, J + sy, 0) +
+                      0.25 * mg_at(s, Cg, Cg->e, I + sx, J + sy, 0));
+        }
+        long t = mg_idx(G, i, j, k);
+        G->e[t] = G->e[t] + v;
+      }
+  mg_sweeps(s, G, G->e, MG_POST);
+}
+
+static double mg_sumsq(const orc_fac* s, const MGL* G, int residual) {
+  const int n2 = s->dim == 3 ? G->n[2] : 1;
+  double acc = 0.0;
+  for (int k = 0; k < n2; ++k)
+    for (int j = 0; j < G->n[1]; ++j)
+      for (int i = 0; i < G->n[0]; ++i) {
+        double v = residual ? mg_cell(s, G, G->e, i, j, k, 0) : G->f[mg_idx(G, i, j, k)];
+        acc += v * v;
+      }
+  return acc;
+}
+
+/* Level 0 tiles the domain (checked at creation); gather, solve, scatter. */
+static void mg_solve(orc_fac* s) {
+  const int dim = s->dim;
+  Level* L0 = &s->L[0];
+  MGL M[32];
+  int nm = 0;
+  MGL* G = &M[0];
+  for (int d = 0; d < 3; ++d) {
+    G->n[d] = d < dim ? L0->dhi[d] - L0->dlo[d] + 1 : 1;
+    G->dx[d] = L0->dx[d];
+  }
+  nm = 1;
+  for (;;) {
+    MGL* P = &M[nm - 1];
+    int small = 1, can = 1;
+    for (int d = 0; d < dim; ++d) {
+      if (P->n[d] > MG_SMALL_EDGE) small = 0;
+      if (P->n[d] % 2 != 0 || P->n[d] < 2) can = 0;
+    }
+    if (small || !can || nm == 32) break;
+    MGL* C = &M[nm++];
+    for (int d = 0; d < 3; ++d) {
+      C->n[d] = d < dim ? P->n[d] / 2 : 1;
+      C->dx[d] = d < dim ? P->dx[d] * 2 : P->dx[d];
+    }
+  }
+  int small_direct = 1;
+  for (int d = 0; d < dim; ++d)
+    if (M[nm - 1].n[d] > MG_SMALL_EDGE) small_direct = 0;
+  if (nm == 1) small_direct = 0;
+  for (int m = 0; m < nm; ++m) {
+    MGL* X = &M[m];
+    long nc = (long)X->n[0] * X->n[1] * X->n[2];
+    X->e = dalloc(nc);
+    X->f = dalloc(nc);
+    X->C = s->cmode == 2 ? dalloc(nc) : NULL;
+    for (int a = 0; a < 3; ++a) X->D[a] = NULL;
+    if (s->dvar)
+      for (int a = 0; a < dim; ++a) X->D[a] = dalloc(nc / X->n[a] * (X->n[a] + 1));
+  }
+  /* gather e, r, D, C of level 0 */
+  G = &M[0];
+  for (int p = 0; p < L0->np; ++p) {
+    Patch* P = &L0->p[p];
+    for (int k = (dim == 3 ? P->b.lo[2] : 0); k <= (dim == 3 ? P->b.hi[2] : 0); ++k)
+      for (int j = P->b.lo[1]; j <= P->b.hi[1]; ++j)
+        for (int i = P->b.lo[0]; i <= P->b.hi[0]; ++i) {
+          long t = mg_idx(G, i - L0->dlo[0], j - L0->dlo[1], dim == 3 ? k - L0->dlo[2] : 0);
+          long pi = (long)(i - P->b.lo[0]) +
+                    (long)(P->b.hi[0] - P->b.lo[0] + 1) *
+                        ((j - P->b.lo[1]) + (long)(P->b.hi[1] - P->b.lo[1] + 1) *
+                                                (dim == 3 ? k - P->b.lo[2] : 0));
+          G->e[t] = P->e[gidx(&P->gb, i, j, k)];
+          G->f[t] = P->r[pi];
+          if (G->C) G->C[t] = P->C[pi];
+        }
+    if (s->dvar)
+      for (int a = 0; a < dim; ++a) {
+        int e[3] = {1, 1, 1}, bn[3] = {G->n[0], G->n[1], G->n[2]};
+        for (int d = 0; d < dim; ++d) e[d] = P->b.hi[d] - P->b.lo[d] + 1 + (d == a);
+        bn[a] += 1;
+        for (int k = 0; k < e[2]; ++k)
+          for (int j = 0; j < e[1]; ++j)
+            for (int i = 0; i < e[0]; ++i) {
+              long src = (long)i + (long)e[0] * (j + (long)e[1] * k);
+              int gi = P->b.lo[0] + i - L0->dlo[0], gj = P->b.lo[1] + j - L0->dlo[1];
+              int gk = dim == 3 ? P->b.lo[2] + k - L0->dlo[2] : 0;
+              G->D[a][(long)gi + (long)bn[0] * (gj + (long)bn[1] * gk)] = P->D[a][src];
+            }
+      }
+  }
+  /* coarse coefficients */
+  for (int m = 1; m < nm; ++m) {
+    MGL* F = &M[m - 1];
+    MGL* C = &M[m];
+    if (s->dvar)
+      for (int a = 0; a < dim; ++a) {
+        int e[3] = {C->n[0], C->n[1], C->n[2]}, ef[3] = {F->n[0], F->n[1], F->n[2]};
+        e[a] += 1;
+        ef[a] += 1;
+        const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+        for (int z = 0; z < e[2]; ++z)
+          for (int y = 0; y < e[1]; ++y)
+            for (int x = 0; x < e[0]; ++x) {
+              int c[3] = {x, y, z};
+              double sum = 0.0;
+              for (int q2 = 0; q2 < (dim == 3 ? 2 : 1); ++q2)
+                for (int q1 = 0; q1 < 2; ++q1) {
+                  int f[3];
+                  f[a] = 2 * c[a];
+                  f[t1] = 2 * c[t1] + q1;
+                  f[t2] = dim == 3 ? 2 * c[t2] + q2 : 0;
+                  sum += F->D[a][(long)f[0] + (long)ef[0] * (f[1] + (long)ef[1] * f[2])];
+                }
+              C->D[a][(long)x + (long)e[0] * (y + (long)e[1] * z)] =
+                  sum * (dim == 3 ? 0.25 : 0.5);
+            }
+      }
+    if (s->cmode == 2)
+      for (int K = 0; K < (dim == 3 ? C->n[2] : 1); ++K)
+        for (int J = 0; J < C->n[1]; ++J)
+          for (int I = 0; I < C->n[0]; ++I) {
+            double sum = 0.0;
+            for (int c2 = 0; c2 < (dim == 3 ? 2 : 1); ++c2)
+              for (int c1 = 0; c1 < 2; ++c1)
+                for (int c0 = 0; c0 < 2; ++c0)
+                  sum += F->C[mg_idx(F, 2 * I + c0, 2 * J + c1, dim == 3 ? 2 * K + c2 : 0)];
+            C->C[mg_idx(C, I, J, K)] = sum * (dim == 3 ? 0.125 : 0.25);
+          }
+  }
+  const double bb = mg_sumsq(s, G, 0);
+  s->mg_iterations = 0;
+  s->mg_relres = 0.0;
+  if (bb > 0.0) {
+    for (int it = 0; it < s->cmax; ++it) {
+      mg_vcycle(s, M, nm, 0, small_direct);
+      const double rel = sqrt(mg_sumsq(s, G, 1)) / sqrt(bb);
+      s->mg_iterations = it + 1;
+      s->mg_relres = rel;
+      if (!(rel > s->ctol)) break;
+    }
+  }
+  /* scatter e */
+  for (int p = 0; p < L0->np; ++p) {
+    Patch* P = &L0->p[p];
+    for (int k = (dim == 3 ? P->b.lo[2] : 0); k <= (dim == 3 ? P->b.hi[2] : 0); ++k)
+      for (int j = P->b.lo[1]; j <= P->b.hi[1]; ++j)
+        for (int i = P->b.lo[0]; i <= P->b.hi[0]; ++i)
+          P->e[gidx(&P->gb, i, j, k)] =
+              G->e[mg_idx(G, i - L0->dlo[0], j - L0->dlo[1], dim == 3 ? k - L0->dlo[2] : 0)];
+  }
+  for (int m = 0; m < nm; ++m) {
+    free(M[m].e); free(M[m].f); free(M[m].C);
+    for (int a = 0; a < 3; ++a) free(M[m].D[a]);
+  }
+}
+
+void orc_fac_set_coarse_mg(orc_fac* s, int on) { s->coarse_mg = on; }
+int orc_fac_mg_iterations(orc_fac* s) { return s->mg_iterations; }
+double orc_fac_mg_relres(orc_fac* s) { return s->mg_relres; }
+
 static void fac_cycle(orc_fac* s, int ln) {
   if (ln == 0) {
-    smooth(s, F_E, F_R, ln, s->cmax, s->ctol, NULL);
+    if (s->coarse_mg) mg_solve(s);
+    else smooth(s, F_E, F_R, ln, s->cmax, s->ctol, NULL);
     ghost_fill(s, F_E, ln, 0, 1);
     return;
   }

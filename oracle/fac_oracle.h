@@ -242,6 +242,13 @@ double orc_fac_op_rms(orc_fac* s, int field);
 double orc_fac_op_dot(orc_fac* s, int f1, int f2);
 double orc_fac_op_sum_cv(orc_fac* s);
 void orc_fac_op_vcycle(orc_fac* s);
+/* Coarsest level by geometric multigrid instead of red-black smoothing
+ * (coarse_solver_choice "mg"; restates this library's own multigrid, so
+ * its parity is pinned by the converged FAC solution, not by a reference
+ * routine).  mg_iterations / mg_relres report the last coarse solve. */
+void orc_fac_set_coarse_mg(orc_fac* s, int on);
+int orc_fac_mg_iterations(orc_fac* s);
+double orc_fac_mg_relres(orc_fac* s);
 void orc_fac_op_get_flux(orc_fac* s, int ln, int p, int axis, double* dst);
 
 #ifdef __cplusplus

@@ -79,6 +79,11 @@ def oracle_lib():
         lib.orc_fac_op_sum_cv.restype = C.c_double
         lib.orc_fac_op_sum_cv.argtypes = [C.c_void_p]
         lib.orc_fac_op_get_flux.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
+        lib.orc_fac_set_coarse_mg.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_fac_mg_iterations.argtypes = [C.c_void_p]
+        lib.orc_fac_mg_iterations.restype = C.c_int
+        lib.orc_fac_mg_relres.argtypes = [C.c_void_p]
+        lib.orc_fac_mg_relres.restype = C.c_double
         _oracle = lib
     return _oracle
 
@@ -149,6 +154,13 @@ class OracleFAC:
         self.lib.orc_fac_set_params(self.h, max_cycles, residual_tol, relative_residual_tol,
                                     pre, post, coarse_max_iterations, coarse_tol, int(ewing),
                                     nthreads)
+
+    def set_coarse_mg(self, on=True):
+        """Coarsest level by geometric multigrid (cmax V-cycles, relative tol ctol)."""
+        self.lib.orc_fac_set_coarse_mg(self.h, int(on))
+
+    def mg_stats(self):
+        return self.lib.orc_fac_mg_iterations(self.h), self.lib.orc_fac_mg_relres(self.h)
 
     def size(self, field, ln, p, axis=0):
         return self.lib.orc_fac_array_size(self.h, FIELDS[field], ln, p, axis)

@@ -83,4 +83,30 @@ void launch_box_cells(const GhostLaunch& cells, const LevelGeom& L, double* cons
 void launch_box_side(const LevelGeom& L, const double* const* x, int g, int a, double* box,
                      const BoxLevel& G, int* mismatch, hipStream_t s, int verify);
 
+// ---- geometric multigrid on a single-rectangle level ----
+// Largest edge of the grid the one-work-group LDS solve accepts, and the
+// number of partial sums launch_mg_sumsq writes.
+int mg_small_edge();
+int mg_partials();
+// fc = mean over the 2^dim children of f - (div D grad u + C u) on Gf.
+void launch_mg_restrict_residual(const BoxLevel& Gf, const BoxArrays& A, const RobinDev& bc,
+                                 const double* u, const BoxLevel& Gc, double* fc,
+                                 const int* stop, hipStream_t s);
+// ef += separable linear interpolation of ec (homogeneous Robin beyond the domain).
+void launch_mg_prolong_add(const BoxLevel& Gc, const RobinDev& bc, const double* ec,
+                           const BoxLevel& Gf, double* ef, const int* stop, hipStream_t s);
+// Coarse D faces (mean of the fine faces they cover) and C (mean of children).
+void launch_mg_coarsen_coef(const BoxLevel& Gf, const BoxArrays& Af, const BoxLevel& Gc,
+                            double* const* Dc, double* Cc, hipStream_t s);
+// u = 0, then `sweeps` red-black sweeps on a grid of at most mg_small_edge()
+// cells per axis, in one work-group.
+void launch_mg_small_solve(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, double* u,
+                           int sweeps, const int* stop, hipStream_t s);
+// mg_partials() partial sums of x^2 (x != nullptr) or of the squared residual of u.
+void launch_mg_sumsq(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, const double* u,
+                     const double* x, double* partial, const int* stop, hipStream_t s);
+// Relative-residual test after iteration it (it < 0: b = 0 test only).
+void launch_mg_check(const double* rr, const double* bb, double tol, int it, int* stop,
+                     int* iters, double* relres, hipStream_t s);
+
 }  // namespace sfac

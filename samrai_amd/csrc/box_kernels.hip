@@ -440,6 +440,280 @@ void launch_dim(int variant, const BoxLevel& G, const BoxArrays& A, const RobinD
   }
 }
 
+// ------------------------------------------------------------------------
+// Geometric multigrid on a single-rectangle level (the coarsest level's
+// solver when coarse_solver_choice is "mg"/"hypre").  Level m+1 halves every
+// extent of level m; cell-centred full-weighting restriction of the residual
+// (mean of the 2^dim children), separable linear prolongation (weights 3/4
+// and 1/4 per axis, homogeneous Robin ghosts beyond the domain), red-black
+// Gauss-Seidel smoothing by k_box_rbgs, and a one-work-group LDS solve on
+// the smallest grid.  All kernels return at once when *stop is set.
+
+// u at box index (i, j, k) with homogeneous Robin ghosts one cell beyond the
+// domain along any axis (applied axis by axis).
+template <int DIM>
+__device__ inline double mg_at(const BoxLevel& G, const RobinDev& bc, const double* u, int i,
+                               int j, int k) {
+  int c[3] = {i, j, k}, lo[3] = {0, 0, 0};
+  for (int a = 0; a < DIM; ++a) {
+    lo[a] = c[a] < 0 ? 0 : (c[a] >= G.n[a] ? G.n[a] - 1 : c[a]);
+  }
+  double v = u[(long)lo[0] + (long)G.n[0] * (lo[1] + (long)G.n[1] * (DIM == 3 ? lo[2] : 0))];
+  for (int a = 0; a < DIM; ++a) {
+    if (c[a] < 0) v = robin_ghost<DIM>(G, bc, 1, 2 * a, v);
+    else if (c[a] >= G.n[a]) v = robin_ghost<DIM>(G, bc, 1, 2 * a + 1, v);
+  }
+  return v;
+}
+
+// Residual f - (div D grad u + C u) of one cell, in the smoother's
+// expression order (global-memory neighbours, Robin ghosts at the domain).
+template <int DIM, bool DVAR, int CMODE>
+__device__ inline double mg_residual(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc,
+                                     const double* u, int i, int j, int k) {
+  const int n0 = G.n[0], n1 = G.n[1];
+  const long c = (long)i + (long)n0 * (j + (long)n1 * k);
+  const Coef q = load_coef<DIM, DVAR, CMODE>(G, A, i, j, k);
+  const double dxi = 1.0 / G.dx[0], dyi = 1.0 / G.dx[1];
+  const double dzi = DIM == 3 ? 1.0 / G.dx[2] : 0.0;
+  const double dc = A.dconst;
+  const double uc = u[c];
+  double Dxl, Dxh, Dyl, Dyh, Dzl = 0.0, Dzh = 0.0;
+  if (DVAR) {
+    Dxl = q.xl; Dxh = q.xh; Dyl = q.yl; Dyh = q.yh;
+    if (DIM == 3) { Dzl = q.zl; Dzh = q.zh; }
+  } else {
+    Dxl = Dxh = Dyl = Dyh = Dzl = Dzh = dc;
+  }
+  const double uxl = i > 0 ? u[c - 1] : robin_ghost<DIM>(G, bc, 1, 0, uc);
+  const double uxh = i < n0 - 1 ? u[c + 1] : robin_ghost<DIM>(G, bc, 1, 1, uc);
+  const double uyl = j > 0 ? u[c - n0] : robin_ghost<DIM>(G, bc, 1, 2, uc);
+  const double uyh = j < n1 - 1 ? u[c + n0] : robin_ghost<DIM>(G, bc, 1, 3, uc);
+  const double fxl = dxi * Dxl * (uc - uxl);
+  const double fxh = dxi * Dxh * (uxh - uc);
+  const double fyl = dyi * Dyl * (uc - uyl);
+  const double fyh = dyi * Dyh * (uyh - uc);
+  double fzl = 0.0, fzh = 0.0;
+  if (DIM == 3) {
+    const long s2 = (long)n0 * n1;
+    const double uzl = k > 0 ? u[c - s2] : robin_ghost<DIM>(G, bc, 1, 4, uc);
+    const double uzh = k < G.n[2] - 1 ? u[c + s2] : robin_ghost<DIM>(G, bc, 1, 5, uc);
+    fzl = dzi * Dzl * (uc - uzl);
+    fzh = dzi * Dzh * (uzh - uc);
+  }
+  double scal;
+  if (CMODE == 2) scal = q.c;
+  else scal = CMODE == 1 ? A.cconst : 0.0;
+  if (DIM == 3)
+    return q.f - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
+  return q.f - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
+}
+
+// fc = mean over the 2^dim children of the fine residual (one thread per
+// coarse cell; children summed in i, j, k order).
+template <int DIM, bool DVAR, int CMODE>
+__global__ void __launch_bounds__(kThreads)
+k_mg_restrict_residual(BoxLevel Gf, BoxArrays A, RobinDev bc, const double* u, BoxLevel Gc,
+                       double* fc, const int* stop) {
+  if (*stop) return;
+  const long nc = (long)Gc.n[0] * Gc.n[1] * (DIM == 3 ? Gc.n[2] : 1);
+  for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < nc;
+       t += (long)gridDim.x * kThreads) {
+    const int I = (int)(t % Gc.n[0]);
+    const int J = (int)((t / Gc.n[0]) % Gc.n[1]);
+    const int K = DIM == 3 ? (int)(t / ((long)Gc.n[0] * Gc.n[1])) : 0;
+    double sum = 0.0;
+    for (int c2 = 0; c2 < (DIM == 3 ? 2 : 1); ++c2)
+      for (int c1 = 0; c1 < 2; ++c1)
+        for (int c0 = 0; c0 < 2; ++c0)
+          sum += mg_residual<DIM, DVAR, CMODE>(Gf, A, bc, u, 2 * I + c0, 2 * J + c1,
+                                               DIM == 3 ? 2 * K + c2 : 0);
+    fc[t] = sum * (DIM == 3 ? 0.125 : 0.25);
+  }
+}
+
+// ef += linear interpolation of ec (one thread per fine cell).
+template <int DIM>
+__global__ void __launch_bounds__(kThreads)
+k_mg_prolong_add(BoxLevel Gc, RobinDev bc, const double* ec, BoxLevel Gf, double* ef,
+                 const int* stop) {
+  if (*stop) return;
+  const long nf = (long)Gf.n[0] * Gf.n[1] * (DIM == 3 ? Gf.n[2] : 1);
+  for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < nf;
+       t += (long)gridDim.x * kThreads) {
+    const int i = (int)(t % Gf.n[0]);
+    const int j = (int)((t / Gf.n[0]) % Gf.n[1]);
+    const int k = DIM == 3 ? (int)(t / ((long)Gf.n[0] * Gf.n[1])) : 0;
+    const int I = i >> 1, J = j >> 1, K = k >> 1;
+    const int sx = (i & 1) ? 1 : -1, sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
+    double v;
+    if (DIM == 3) {
+      double z0 = 0.75 * (0.75 * mg_at<3>(Gc, bc, ec, I, J, K) +
+                          0.25 * mg_at<3>(Gc, bc, ec, I + sx, J, K)) +
+                  0.25 * (0.75 * mg_at<3>(Gc, bc, ec, I, J + sy, K) +
+                          0.25 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K));
+      double z1 = 0.75 * (0.75 * mg_at<3>(Gc, bc, ec, I, J, K + sz) +
+                          0.25 * mg_at<3>(Gc, bc, ec, I + sx, J, K + sz)) +
+                  0.25 * (0.75 * mg_at<3>(Gc, bc, ec, I, J + sy, K + sz) +
+                          0.25 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K + sz));
+      v = 0.75 * z0 + 0.25 * z1;
+    } else {
+      v = 0.75 * (0.75 * mg_at<2>(Gc, bc, ec, I, J, 0) + 0.25 * mg_at<2>(Gc, bc, ec, I + sx, J, 0)) +
+          0.25 * (0.75 * mg_at<2>(Gc, bc, ec, I, J + sy, 0) +
+                  0.25 * mg_at<2>(Gc, bc, ec, I + sx, J + sy, 0));
+    }
+    ef[t] = ef[t] + v;
+  }
+}
+
+// Coefficients of level m+1 from level m: a coarse face takes the mean of
+// the 2^(dim-1) fine faces it covers; C the mean of the 2^dim children.
+template <int DIM>
+__global__ void __launch_bounds__(kThreads)
+k_mg_coarsen_coef(BoxLevel Gf, BoxArrays Af, BoxLevel Gc, double* Dc0, double* Dc1, double* Dc2,
+                  double* Cc) {
+  const int nc[3] = {Gc.n[0], Gc.n[1], DIM == 3 ? Gc.n[2] : 1};
+  const int nf[3] = {Gf.n[0], Gf.n[1], DIM == 3 ? Gf.n[2] : 1};
+  double* Dc[3] = {Dc0, Dc1, Dc2};
+  for (int a = 0; a < DIM; ++a) {
+    if (!Dc[a]) continue;
+    int e[3] = {nc[0], nc[1], nc[2]};
+    e[a] += 1;
+    int ef[3] = {nf[0], nf[1], nf[2]};
+    ef[a] += 1;
+    const long n = (long)e[0] * e[1] * e[2];
+    const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+    for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < n;
+         t += (long)gridDim.x * kThreads) {
+      int c[3] = {(int)(t % e[0]), (int)((t / e[0]) % e[1]), (int)(t / ((long)e[0] * e[1]))};
+      double sum = 0.0;
+      for (int q2 = 0; q2 < (DIM == 3 ? 2 : 1); ++q2)
+        for (int q1 = 0; q1 < 2; ++q1) {
+          int f[3];
+          f[a] = 2 * c[a];
+          f[t1] = 2 * c[t1] + q1;
+          f[t2] = DIM == 3 ? 2 * c[t2] + q2 : 0;
+          sum += Af.D[a][(long)f[0] + (long)ef[0] * (f[1] + (long)ef[1] * f[2])];
+        }
+      Dc[a][t] = sum * (DIM == 3 ? 0.25 : 0.5);
+    }
+  }
+  if (Cc) {
+    const long n = (long)nc[0] * nc[1] * nc[2];
+    for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < n;
+         t += (long)gridDim.x * kThreads) {
+      const int I = (int)(t % nc[0]), J = (int)((t / nc[0]) % nc[1]);
+      const int K = (int)(t / ((long)nc[0] * nc[1]));
+      double sum = 0.0;
+      for (int c2 = 0; c2 < (DIM == 3 ? 2 : 1); ++c2)
+        for (int c1 = 0; c1 < 2; ++c1)
+          for (int c0 = 0; c0 < 2; ++c0)
+            sum += Af.C[(long)(2 * I + c0) +
+                        (long)nf[0] * ((2 * J + c1) + (long)nf[1] * (DIM == 3 ? 2 * K + c2 : 0))];
+      Cc[t] = sum * (DIM == 3 ? 0.125 : 0.25);
+    }
+  }
+}
+
+// The smallest grid (at most kMGSmall cells and 8 per axis): u = 0, then
+// `sweeps` red-black sweeps with the whole grid and its ghosts in LDS; one
+// work-group.
+constexpr int kMGSmallEdge = 8;
+template <int DIM, bool DVAR, int CMODE>
+__global__ void __launch_bounds__(kThreads)
+k_mg_small_solve(BoxLevel G, BoxArrays A, RobinDev bc, double* u, int sweeps, const int* stop) {
+  constexpr int L = kMGSmallEdge + 2;
+  constexpr int SY = L, SZ = L * L;
+  __shared__ double U[DIM == 3 ? L * L * L : L * L];
+  if (*stop) return;
+  const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
+  for (int q = threadIdx.x; q < (DIM == 3 ? L * L * L : L * L); q += kThreads) U[q] = 0.0;
+  __syncthreads();
+  auto ghosts = [&]() {
+    const long ncell = (long)n0 * n1 * n2;
+    for (long t = threadIdx.x; t < ncell; t += kThreads) {
+      const int i = (int)(t % n0), j = (int)((t / n0) % n1), k = DIM == 3 ? (int)(t / ((long)n0 * n1)) : 0;
+      const int q = (i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0);
+      if (i == 0) U[q - 1] = robin_ghost<DIM>(G, bc, 1, 0, U[q]);
+      if (i == n0 - 1) U[q + 1] = robin_ghost<DIM>(G, bc, 1, 1, U[q]);
+      if (j == 0) U[q - SY] = robin_ghost<DIM>(G, bc, 1, 2, U[q]);
+      if (j == n1 - 1) U[q + SY] = robin_ghost<DIM>(G, bc, 1, 3, U[q]);
+      if (DIM == 3 && k == 0) U[q - SZ] = robin_ghost<DIM>(G, bc, 1, 4, U[q]);
+      if (DIM == 3 && k == n2 - 1) U[q + SZ] = robin_ghost<DIM>(G, bc, 1, 5, U[q]);
+    }
+  };
+  const long ncell = (long)n0 * n1 * n2;
+  for (int s = 0; s < sweeps; ++s)
+    for (int colour = 0; colour < 2; ++colour) {
+      ghosts();
+      __syncthreads();
+      for (long t = threadIdx.x; t < ncell; t += kThreads) {
+        const int i = (int)(t % n0), j = (int)((t / n0) % n1), k = DIM == 3 ? (int)(t / ((long)n0 * n1)) : 0;
+        if (((i + j + k) & 1) != colour) continue;
+        const int q = (i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0);
+        cell_update<DIM, DVAR, CMODE>(U, q, SY, SZ, load_coef<DIM, DVAR, CMODE>(G, A, i, j, k), G,
+                                      A);
+      }
+      __syncthreads();
+    }
+  for (long t = threadIdx.x; t < ncell; t += kThreads) {
+    const int i = (int)(t % n0), j = (int)((t / n0) % n1), k = DIM == 3 ? (int)(t / ((long)n0 * n1)) : 0;
+    u[t] = U[(i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0)];
+  }
+}
+
+// Partial sums of the squared residual (x == nullptr) or of x^2 over a
+// level, one per work-group of a fixed grid.
+template <int DIM, bool DVAR, int CMODE>
+__global__ void __launch_bounds__(kThreads)
+k_mg_sumsq(BoxLevel G, BoxArrays A, RobinDev bc, const double* u, const double* x,
+           double* partial, const int* stop) {
+  __shared__ double sm[kThreads / 64];
+  if (*stop) return;
+  const long n = (long)G.n[0] * G.n[1] * (DIM == 3 ? G.n[2] : 1);
+  double acc = 0.0;
+  for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < n;
+       t += (long)gridDim.x * kThreads) {
+    double v;
+    if (x) {
+      v = x[t];
+    } else {
+      const int i = (int)(t % G.n[0]), j = (int)((t / G.n[0]) % G.n[1]);
+      const int k = DIM == 3 ? (int)(t / ((long)G.n[0] * G.n[1])) : 0;
+      v = mg_residual<DIM, DVAR, CMODE>(G, A, bc, u, i, j, k);
+    }
+    acc += v * v;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum = 0.0;
+    for (int w = 0; w < kThreads / 64; ++w) sum += sm[w];
+    partial[blockIdx.x] = sum;
+  }
+}
+
+// Convergence test after iteration `it` (it < 0: before the first): stop
+// when ||r|| <= tol ||b|| (or b = 0), as hypre's relative-residual test.
+__global__ void k_mg_check(const double* rr, const double* bb, double tol, int it, int* stop,
+                           int* iters, double* relres) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || *stop) return;
+  const double b = sqrt(*bb);
+  if (it < 0) {
+    if (!(b > 0.0)) {
+      *stop = 1;
+      *iters = 0;
+      *relres = 0.0;
+    }
+    return;
+  }
+  const double rel = sqrt(*rr) / b;
+  *iters = it + 1;
+  *relres = rel;
+  if (!(rel > tol)) *stop = 1;
+}
+
 }  // namespace
 
 int box_tile_variants(int) { return kVariants; }
@@ -477,6 +751,124 @@ void launch_box_side(const LevelGeom& L, const double* const* x, int g, int a, d
   check_launch();
   if (!verify) return;
   hipLaunchKernelGGL(k_box_side, grid, dim3(kThreads), 0, s, L, x, g, a, box, G, mismatch, 1);
+  check_launch();
+}
+
+
+// ------------------------------------------------------------ multigrid
+
+namespace {
+constexpr int kMGGrid = 1024;  // work-groups of the grid-stride MG kernels
+
+inline int mg_blocks(long n) {
+  long b = (n + kThreads - 1) / kThreads;
+  return (int)(b < kMGGrid ? (b > 0 ? b : 1) : kMGGrid);
+}
+
+template <template <int, bool, int> class F, class... Args>
+void dispatch3(const BoxLevel& G, const BoxArrays& A, Args&&... args) {
+  const bool dvar = A.D[0] != nullptr;
+  if (G.dim == 3) {
+    if (dvar) {
+      if (A.cmode == 2) F<3, true, 2>::run(G, A, args...);
+      else if (A.cmode == 1) F<3, true, 1>::run(G, A, args...);
+      else F<3, true, 0>::run(G, A, args...);
+    } else {
+      if (A.cmode == 2) F<3, false, 2>::run(G, A, args...);
+      else if (A.cmode == 1) F<3, false, 1>::run(G, A, args...);
+      else F<3, false, 0>::run(G, A, args...);
+    }
+  } else {
+    if (dvar) {
+      if (A.cmode == 2) F<2, true, 2>::run(G, A, args...);
+      else if (A.cmode == 1) F<2, true, 1>::run(G, A, args...);
+      else F<2, true, 0>::run(G, A, args...);
+    } else {
+      if (A.cmode == 2) F<2, false, 2>::run(G, A, args...);
+      else if (A.cmode == 1) F<2, false, 1>::run(G, A, args...);
+      else F<2, false, 0>::run(G, A, args...);
+    }
+  }
+}
+
+template <int DIM, bool DVAR, int CMODE>
+struct RestrictRun {
+  static void run(const BoxLevel& Gf, const BoxArrays& A, const RobinDev& bc, const double* u,
+                  const BoxLevel& Gc, double* fc, const int* stop, hipStream_t s) {
+    const long nc = (long)Gc.n[0] * Gc.n[1] * (DIM == 3 ? Gc.n[2] : 1);
+    hipLaunchKernelGGL((k_mg_restrict_residual<DIM, DVAR, CMODE>), dim3(mg_blocks(nc)),
+                       dim3(kThreads), 0, s, Gf, A, bc, u, Gc, fc, stop);
+    check_launch();
+  }
+};
+
+template <int DIM, bool DVAR, int CMODE>
+struct SmallRun {
+  static void run(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, double* u,
+                  int sweeps, const int* stop, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_small_solve<DIM, DVAR, CMODE>), dim3(1), dim3(kThreads), 0, s, G, A,
+                       bc, u, sweeps, stop);
+    check_launch();
+  }
+};
+
+template <int DIM, bool DVAR, int CMODE>
+struct SumsqRun {
+  static void run(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, const double* u,
+                  const double* x, double* partial, const int* stop, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_sumsq<DIM, DVAR, CMODE>), dim3(kMGGrid), dim3(kThreads), 0, s, G, A,
+                       bc, u, x, partial, stop);
+    check_launch();
+  }
+};
+}  // namespace
+
+int mg_small_edge() { return kMGSmallEdge; }
+int mg_partials() { return kMGGrid; }
+
+void launch_mg_restrict_residual(const BoxLevel& Gf, const BoxArrays& A, const RobinDev& bc,
+                                 const double* u, const BoxLevel& Gc, double* fc,
+                                 const int* stop, hipStream_t s) {
+  dispatch3<RestrictRun>(Gf, A, bc, u, Gc, fc, stop, s);
+}
+
+void launch_mg_prolong_add(const BoxLevel& Gc, const RobinDev& bc, const double* ec,
+                           const BoxLevel& Gf, double* ef, const int* stop, hipStream_t s) {
+  const long nf = (long)Gf.n[0] * Gf.n[1] * (Gf.dim == 3 ? Gf.n[2] : 1);
+  if (Gf.dim == 3)
+    hipLaunchKernelGGL(k_mg_prolong_add<3>, dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gc, bc, ec,
+                       Gf, ef, stop);
+  else
+    hipLaunchKernelGGL(k_mg_prolong_add<2>, dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gc, bc, ec,
+                       Gf, ef, stop);
+  check_launch();
+}
+
+void launch_mg_coarsen_coef(const BoxLevel& Gf, const BoxArrays& Af, const BoxLevel& Gc,
+                            double* const* Dc, double* Cc, hipStream_t s) {
+  const long nf = (long)Gf.n[0] * Gf.n[1] * (Gf.dim == 3 ? Gf.n[2] : 1);
+  if (Gf.dim == 3)
+    hipLaunchKernelGGL(k_mg_coarsen_coef<3>, dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gf, Af, Gc,
+                       Dc[0], Dc[1], Dc[2], Cc);
+  else
+    hipLaunchKernelGGL(k_mg_coarsen_coef<2>, dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gf, Af, Gc,
+                       Dc[0], Dc[1], nullptr, Cc);
+  check_launch();
+}
+
+void launch_mg_small_solve(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, double* u,
+                           int sweeps, const int* stop, hipStream_t s) {
+  dispatch3<SmallRun>(G, A, bc, u, sweeps, stop, s);
+}
+
+void launch_mg_sumsq(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, const double* u,
+                     const double* x, double* partial, const int* stop, hipStream_t s) {
+  dispatch3<SumsqRun>(G, A, bc, u, x, partial, stop, s);
+}
+
+void launch_mg_check(const double* rr, const double* bb, double tol, int it, int* stop,
+                     int* iters, double* relres, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(64), 0, s, rr, bb, tol, it, stop, iters, relres);
   check_launch();
 }
 

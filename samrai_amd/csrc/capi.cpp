@@ -371,6 +371,23 @@ int sfac_solver_fused_tile_variant(sfac_solver* s, int* variant) {
   });
 }
 
+int sfac_solver_fused_tile_timings(sfac_solver* s, double* ms_per_sweep, int max, int* n) {
+  return guarded([&] {
+    need(s, "solver");
+    need(n, "n");
+    const auto& t = s->s->ops().fusedTileTimings();
+    *n = (int)t.size();
+    for (int i = 0; i < (int)t.size() && i < max; ++i) ms_per_sweep[i] = t[i];
+  });
+}
+
+int sfac_solver_coarse_mg_stats(sfac_solver* s, int* iterations, double* relres) {
+  return guarded([&] {
+    need(s, "solver");
+    s->s->ops().mgStats(iterations, relres);
+  });
+}
+
 int sfac_fused_tile_variants(int dim) { return sfac::box_tile_variants(dim); }
 
 int sfac_fused_tile_shape(int dim, int variant, int* shape3) {

@@ -100,9 +100,10 @@ CellPoissonFACOps::~CellPoissonFACOps() {
 }
 
 void CellPoissonFACOps::setCoarsestLevelSolverChoice(const std::string& c) {
-  if (c != "redblack" && c != "jacobi" && c != "hypre")
+  if (c != "redblack" && c != "jacobi" && c != "hypre" && c != "mg")
     fail("CellPoissonFACOps: bad coarse_solver_choice '" + c + "'");
   d_coarse_choice = c;
+  if ((c == "mg" || c == "hypre") && d_box.ok && d_mg.empty()) setupMG();
   ++d_generation;
 }
 
@@ -235,12 +236,16 @@ int CellPoissonFACOps::tuneBoxVariant() {
   const std::array<long, 6> key = {dim, G.n[0], G.n[1], G.n[2], d_spec.dIsConstant() ? 0 : 1,
                                    d_spec.cMode()};
   static std::mutex mu;
-  static std::map<std::array<long, 6>, int> cache;
+  static std::map<std::array<long, 6>, std::pair<int, std::vector<float>>> cache;
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
+    if (it != cache.end()) {
+      d_box_timings = it->second.second;
+      return it->second.first;
+    }
   }
+  std::vector<float> times;
   sfac::BoxArrays A{};
   A.dconst = d_spec.getDConstant();
   A.cmode = d_spec.cMode();
@@ -272,6 +277,7 @@ int CellPoissonFACOps::tuneBoxVariant() {
     hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
     float ms = 0.0f;
     hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+    times.push_back(ms / 8.0f);
     if (v == 0 || ms < best_ms) {
       best = v;
       best_ms = ms;
@@ -280,11 +286,13 @@ int CellPoissonFACOps::tuneBoxVariant() {
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   std::lock_guard<std::mutex> lk(mu);
-  cache[key] = best;
+  cache[key] = {best, times};
+  d_box_timings = times;
   return best;
 }
 
 void CellPoissonFACOps::releaseBoxPath() {
+  releaseMG();
   for (double* p : {d_box.e[0], d_box.e[1], d_box.r, d_box.D[0], d_box.D[1], d_box.D[2], d_box.C,
                     d_box.slots})
     if (p) (void)hipFree(p);
@@ -391,19 +399,19 @@ void CellPoissonFACOps::setupBoxPath() {
     }
   }
   B.variant = d_box_variant >= 0 ? d_box_variant : tuneBoxVariant();
+  if (d_coarse_choice == "mg" || d_coarse_choice == "hypre") setupMG();
 }
 
-int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num_sweeps,
-                                 double tol, double* maxres_out) {
+// Gather e (into buffer 0), r and the coefficients of the single-rectangle
+// level.  The coefficients are gathered on every call (the caller owns them
+// and may change their values between solves); nothing here allocates or
+// waits, so the sequence can be captured in a graph.
+sfac::BoxArrays CellPoissonFACOps::gatherBox(PatchDataSet& e, const PatchDataSet& r) {
   BoxPath& B = d_box;
   const int ln = B.ln;
   const int dim = d_h.dim();
   const auto& T = d_h.tables(ln);
   const sfac::BoxLevel& G = B.G;
-  const long vol = (long)G.n[0] * G.n[1] * (dim == 3 ? G.n[2] : 1);
-  // coefficients: gathered on every call (the caller owns them and may
-  // change their values between solves); no allocation or host wait here,
-  // so the sequence can be captured in a graph
   sfac::BoxArrays A{};
   A.dconst = d_spec.getDConstant();
   A.cmode = d_spec.cMode();
@@ -427,6 +435,208 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
   sfac::launch_box_cells(T.cells.launch(), T.geom, const_cast<double* const*>(r.table(ln)),
                          r.ghost(), B.r, B.r, nullptr, 0, G, 1, d_stream);
   A.r = B.r;
+  return A;
+}
+
+// ------------------------------------------------ multigrid coarsest solve
+//
+// coarse_solver_choice "mg" (and "hypre", whose library is not part of this
+// build): V(2,2) cycles over MG levels that halve the single-rectangle level
+// (box_kernels.h), until ||r||_2 <= coarse_solver_tolerance * ||b||_2 or
+// coarse_solver_max_iterations cycles, the stopping test hypre's PFMG applies
+// for CellPoissonFACOps::solveCoarsestLevel_HYPRE (CellPoissonFACOps.C:1740-1780).
+// The sweep counts are even, so every V-cycle ends in buffer 0 of each level
+// and a cycle skipped after convergence leaves the result in place.
+
+namespace {
+constexpr int kMGPre = 2, kMGPost = 2, kMGSmallSweeps = 50, kMGLastSweeps = 64;
+}
+
+void CellPoissonFACOps::releaseMG() {
+  for (size_t m = 1; m < d_mg.size(); ++m) {
+    MGLevel& L = d_mg[m];
+    for (double* p : {L.e[0], L.e[1], L.f, L.D[0], L.D[1], L.D[2], L.C})
+      if (p) (void)hipFree(p);
+  }
+  d_mg.clear();
+  if (d_mg_partial) (void)hipFree(d_mg_partial);
+  if (d_mg_scal) (void)hipFree(d_mg_scal);
+  if (d_mg_flag) (void)hipFree(d_mg_flag);
+  d_mg_partial = d_mg_scal = nullptr;
+  d_mg_flag = nullptr;
+}
+
+void CellPoissonFACOps::setupMG() {
+  releaseMG();
+  if (!d_box.ok) return;
+  const int dim = d_h.dim();
+  const int edge = sfac::mg_small_edge();
+  MGLevel L0;
+  L0.G = d_box.G;
+  for (int d = 0; d < 3; ++d) {
+    L0.G.lo[d] = 0;  // colour by box index, as on the coarser MG levels
+    L0.G.lp_lo[d] = L0.G.lp_hi[d] = 0;
+  }
+  L0.G.shift = 0;
+  L0.e[0] = d_box.e[0];
+  L0.e[1] = d_box.e[1];
+  L0.f = d_box.r;
+  for (int a = 0; a < 3; ++a) L0.D[a] = d_box.D[a];
+  L0.C = d_box.C;
+  d_mg.push_back(L0);
+  for (;;) {
+    const sfac::BoxLevel& P = d_mg.back().G;
+    bool small = true, can = true;
+    for (int d = 0; d < dim; ++d) {
+      if (P.n[d] > edge) small = false;
+      if (P.n[d] % 2 != 0 || P.n[d] < 2) can = false;
+    }
+    if (small || !can || d_mg.size() == 32) break;
+    MGLevel L;
+    L.G = P;
+    for (int d = 0; d < dim; ++d) {
+      L.G.n[d] = P.n[d] / 2;
+      L.G.dx[d] = P.dx[d] * 2;
+    }
+    const long nc = (long)L.G.n[0] * L.G.n[1] * (dim == 3 ? L.G.n[2] : 1);
+    hip_check(hipMalloc(&L.e[0], sizeof(double) * nc), "hipMalloc(mg)");
+    hip_check(hipMalloc(&L.e[1], sizeof(double) * nc), "hipMalloc(mg)");
+    hip_check(hipMalloc(&L.f, sizeof(double) * nc), "hipMalloc(mg)");
+    if (!d_spec.dIsConstant())
+      for (int a = 0; a < dim; ++a)
+        hip_check(hipMalloc(&L.D[a], sizeof(double) * (nc / L.G.n[a] * (L.G.n[a] + 1))),
+                  "hipMalloc(mg)");
+    if (d_spec.cMode() == 2) hip_check(hipMalloc(&L.C, sizeof(double) * nc), "hipMalloc(mg)");
+    d_mg.push_back(L);
+  }
+  d_mg_small_direct = d_mg.size() > 1;
+  for (int d = 0; d < dim; ++d)
+    if (d_mg.back().G.n[d] > edge) d_mg_small_direct = false;
+  hip_check(hipMalloc(&d_mg_partial, sizeof(double) * sfac::mg_partials()), "hipMalloc(mg)");
+  hip_check(hipMalloc(&d_mg_scal, sizeof(double) * 4), "hipMalloc(mg)");
+  hip_check(hipMalloc(&d_mg_flag, sizeof(int) * 2), "hipMalloc(mg)");
+}
+
+sfac::BoxArrays CellPoissonFACOps::mgArrays(int m) const {
+  const MGLevel& L = d_mg[m];
+  sfac::BoxArrays A{};
+  A.dconst = d_spec.getDConstant();
+  A.cmode = d_spec.cMode();
+  A.cconst = d_spec.getCConstant();
+  A.r = L.f;
+  if (!d_spec.dIsConstant())
+    for (int a = 0; a < d_h.dim(); ++a) A.D[a] = L.D[a];
+  if (A.cmode == 2) A.C = L.C;
+  return A;
+}
+
+void CellPoissonFACOps::mgSweeps(int m, int n) {
+  MGLevel& L = d_mg[m];
+  sfac::BoxArrays A = mgArrays(m);
+  sfac::BoxCheck K{};
+  K.stop = d_mg_flag;
+  const sfac::RobinDev bc = robinDev();
+  for (int s = 0; s < n; ++s) {
+    A.ein = L.e[s & 1];
+    A.eout = L.e[(s + 1) & 1];
+    sfac::launch_box_rbgs(L.G, A, bc, 1, K, d_box.variant, d_stream);
+  }
+}
+
+void CellPoissonFACOps::mgVcycle(int m) {
+  MGLevel& L = d_mg[m];
+  const int dim = d_h.dim();
+  const long nc = (long)L.G.n[0] * L.G.n[1] * (dim == 3 ? L.G.n[2] : 1);
+  const sfac::RobinDev bc = robinDev();
+  const bool last = m + 1 == (int)d_mg.size();
+  if (last && d_mg_small_direct) {
+    sfac::launch_mg_small_solve(L.G, mgArrays(m), bc, L.e[0], kMGSmallSweeps, d_mg_flag,
+                                d_stream);
+    return;
+  }
+  if (m > 0) sfac::launch_fill(L.e[0], nc, 0.0, d_stream);
+  if (last) {
+    mgSweeps(m, kMGLastSweeps);
+    return;
+  }
+  mgSweeps(m, kMGPre);
+  MGLevel& C = d_mg[m + 1];
+  sfac::launch_mg_restrict_residual(L.G, mgArrays(m), bc, L.e[0], C.G, C.f, d_mg_flag, d_stream);
+  mgVcycle(m + 1);
+  sfac::launch_mg_prolong_add(C.G, bc, C.e[0], L.G, L.e[0], d_mg_flag, d_stream);
+  mgSweeps(m, kMGPost);
+}
+
+void CellPoissonFACOps::solveBoxMG(PatchDataSet& e, const PatchDataSet& r) {
+  if (!d_box.ok || d_mg.empty())
+    fail("CellPoissonFACOps: coarse_solver_choice '" + d_coarse_choice +
+         "' needs a coarsest level whose patches tile the physical domain");
+  const int ln = d_box.ln;
+  const auto& T = d_h.tables(ln);
+  const int dim = d_h.dim();
+  gatherBox(e, r);
+  for (size_t m = 1; m < d_mg.size(); ++m)
+    if (!d_spec.dIsConstant() || d_spec.cMode() == 2) {
+      sfac::BoxArrays Af = mgArrays((int)m - 1);
+      sfac::launch_mg_coarsen_coef(d_mg[m - 1].G, Af, d_mg[m].G, d_mg[m].D, d_mg[m].C, d_stream);
+    }
+  const sfac::RobinDev bc = robinDev();
+  const int P = sfac::mg_partials();
+  sfac::launch_fill_int(d_mg_flag, 2, 0, d_stream);
+  sfac::launch_mg_sumsq(d_mg[0].G, mgArrays(0), bc, nullptr, d_mg[0].f, d_mg_partial, d_mg_flag,
+                        d_stream);
+  sfac::launch_sum(d_mg_partial, P, d_mg_scal, 0, d_stream);
+  sfac::launch_mg_check(d_mg_scal + 1, d_mg_scal, d_coarse_tol, -1, d_mg_flag, d_mg_flag + 1,
+                        d_mg_scal + 2, d_stream);
+  for (int it = 0; it < d_coarse_max; ++it) {
+    mgVcycle(0);
+    sfac::launch_mg_sumsq(d_mg[0].G, mgArrays(0), bc, d_mg[0].e[0], nullptr, d_mg_partial,
+                          d_mg_flag, d_stream);
+    sfac::launch_sum(d_mg_partial, P, d_mg_scal + 1, 0, d_stream);
+    sfac::launch_mg_check(d_mg_scal + 1, d_mg_scal, d_coarse_tol, it, d_mg_flag, d_mg_flag + 1,
+                          d_mg_scal + 2, d_stream);
+  }
+  sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), d_box.e[0],
+                         d_box.e[0], nullptr, 0, d_box.G, 0, d_stream);
+  (void)dim;
+  if (d_logging) {
+    int it = 0;
+    double rel = 0.0;
+    hip_check(hipMemcpyAsync(&it, d_mg_flag + 1, sizeof(int), hipMemcpyDeviceToHost, d_stream),
+              "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(&rel, d_mg_scal + 2, sizeof(double), hipMemcpyDeviceToHost,
+                             d_stream),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
+    std::printf("CellPoissonFACOps multigrid coarse solve\n\titerations: %d\n\tresidual: %g\n",
+                it, rel);
+  }
+}
+
+void CellPoissonFACOps::mgStats(int* iterations, double* relres) {
+  int it = 0;
+  double rel = 0.0;
+  if (d_mg_flag) {
+    hip_check(hipMemcpyAsync(&it, d_mg_flag + 1, sizeof(int), hipMemcpyDeviceToHost, d_stream),
+              "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(&rel, d_mg_scal + 2, sizeof(double), hipMemcpyDeviceToHost,
+                             d_stream),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
+  }
+  if (iterations) *iterations = it;
+  if (relres) *relres = rel;
+}
+
+int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num_sweeps,
+                                 double tol, double* maxres_out) {
+  BoxPath& B = d_box;
+  const int ln = B.ln;
+  const int dim = d_h.dim();
+  const auto& T = d_h.tables(ln);
+  const sfac::BoxLevel& G = B.G;
+  const long vol = (long)G.n[0] * G.n[1] * (dim == 3 ? G.n[2] : 1);
+  sfac::BoxArrays A = gatherBox(e, r);
   sfac::BoxCheck K{};
   const sfac::BoxTile tile = sfac::box_tile(dim, B.variant);
   const int tsz[3] = {tile.tx, tile.ty, dim == 3 ? tile.tz : 1};
@@ -614,9 +824,13 @@ void CellPoissonFACOps::smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal&
 
 int CellPoissonFACOps::solveCoarsestLevel(SAMRAIVectorReal& e, const SAMRAIVectorReal& r,
                                           int coarsest_ln) {
-  if (d_coarse_choice == "hypre")
-    fail("CellPoissonFACOps: Coarse level solver choice 'hypre' unavailable");
-  smoothErrorByRedBlack(*e.data, *r.data, coarsest_ln, d_coarse_max, d_coarse_tol);
+  if (d_coarse_choice == "mg" || d_coarse_choice == "hypre") {
+    if (coarsest_ln != d_box.ln)
+      fail("CellPoissonFACOps: multigrid coarse solve on a level other than the coarsest");
+    solveBoxMG(*e.data, *r.data);
+  } else {
+    smoothErrorByRedBlack(*e.data, *r.data, coarsest_ln, d_coarse_max, d_coarse_tol);
+  }
   ghostFill(*e.data, coarsest_ln, false, true, false);
   return 0;
 }

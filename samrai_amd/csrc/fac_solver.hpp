@@ -191,6 +191,11 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   void setFusedCoarsest(bool on, int tile_variant);
   bool fusedCoarsestActive() const { return d_box.ok && d_box_enabled; }
   int fusedTileVariant() const { return d_box.ok ? d_box.variant : -1; }
+  // the autotuner's measured time per sweep of every tile shape (ms; empty
+  // when the shape was given or came from the cache)
+  const std::vector<float>& fusedTileTimings() const { return d_box_timings; }
+  // V-cycles and final relative residual of the last multigrid coarse solve
+  void mgStats(int* iterations, double* relres);
 
   // HIP-event timing of the smoother launches (on this stream), with the
   // algorithmic bytes of each launch: a colour pass moves 12 B per cell of
@@ -232,6 +237,27 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   } d_box;
   bool d_box_enabled = true;
   int tuneBoxVariant();
+  std::vector<float> d_box_timings;
+  sfac::BoxArrays gatherBox(pdat::PatchDataSet& e, const pdat::PatchDataSet& r);
+  // geometric multigrid on the single-rectangle coarsest level
+  struct MGLevel {
+    sfac::BoxLevel G{};
+    double* e[2] = {nullptr, nullptr};
+    double* f = nullptr;
+    double* D[3] = {nullptr, nullptr, nullptr};
+    double* C = nullptr;
+  };
+  std::vector<MGLevel> d_mg;  // [0] aliases the box arrays
+  bool d_mg_small_direct = false;
+  double* d_mg_partial = nullptr;
+  double* d_mg_scal = nullptr;  // [0] ||b||^2, [1] ||r||^2, [2] relative residual
+  int* d_mg_flag = nullptr;     // [0] stop, [1] iterations
+  void setupMG();
+  void releaseMG();
+  sfac::BoxArrays mgArrays(int m) const;
+  void mgSweeps(int m, int n);
+  void mgVcycle(int m);
+  void solveBoxMG(pdat::PatchDataSet& e, const pdat::PatchDataSet& r);
   bool d_use_graphs = true;
   unsigned long d_generation = 0;
   int d_box_variant = -1;

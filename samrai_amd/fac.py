@@ -79,6 +79,8 @@ def lib():
     L.sfac_solver_set_fused_coarsest.argtypes = [_vp, C.c_int, C.c_int]
     L.sfac_solver_fused_coarsest_active.argtypes = [_vp, _ip]
     L.sfac_solver_fused_tile_variant.argtypes = [_vp, _ip]
+    L.sfac_solver_fused_tile_timings.argtypes = [_vp, _dp, C.c_int, _ip]
+    L.sfac_solver_coarse_mg_stats.argtypes = [_vp, _ip, _dp]
     L.sfac_solver_set_graphs.argtypes = [_vp, C.c_int]
     L.sfac_fused_tile_variants.argtypes = [C.c_int]
     L.sfac_fused_tile_shape.argtypes = [C.c_int, C.c_int, _ip]
@@ -317,6 +319,20 @@ class CellPoissonFACSolver:
         v = C.c_int()
         _check(lib().sfac_solver_fused_tile_variant(self.s, C.byref(v)))
         return v.value
+
+    def fused_tile_timings(self):
+        """Autotuner milliseconds per sweep for each tile shape."""
+        buf = (C.c_double * 64)()
+        n = C.c_int()
+        _check(lib().sfac_solver_fused_tile_timings(self.s, buf, 64, C.byref(n)))
+        return [buf[i] for i in range(min(n.value, 64))]
+
+    def coarse_mg_stats(self):
+        """(V-cycles, relative residual) of the last multigrid coarse solve."""
+        it = C.c_int()
+        rel = C.c_double()
+        _check(lib().sfac_solver_coarse_mg_stats(self.s, C.byref(it), C.byref(rel)))
+        return it.value, rel.value
 
     def fused_coarsest_active(self):
         a = C.c_int()

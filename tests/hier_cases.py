@@ -152,6 +152,8 @@ class Pair:
                      pre=params.get("num_pre_sweeps", 1), post=params.get("num_post_sweeps", 1),
                      coarse_max_iterations=params.get("coarse_solver_max_iterations", 10),
                      coarse_tol=params.get("coarse_solver_tolerance", 1e-8), ewing=True)
+        if params.get("coarse_solver_choice") in ("mg", "hypre"):
+            o.set_coarse_mg(True)
         for ln, boxes in enumerate(case["levels"]):
             for p in range(len(boxes)):
                 o.put("u", ln, p, data["u"][ln][p])

@@ -86,3 +86,39 @@ def test_fused_coarsest_sweeps_are_bit_identical(gpu_lib, mk, spec):
         np.testing.assert_array_equal(got[1], ref[1])
         for a, b in zip(got[2], ref[2]):
             np.testing.assert_array_equal(a, b)
+
+
+MG = dict(PARAMS, coarse_solver_choice="mg", coarse_solver_tolerance=1e-10,
+          coarse_solver_max_iterations=20)
+
+
+@pytest.mark.parametrize("spec", [dict(dvar=False), dict(dvar=True, cmode=2, robin=True)],
+                         ids=["constD", "varD-varC-robin"])
+@pytest.mark.parametrize("mk", hc.ALL + [hc.case_2d_c0], ids=lambda f: f.__name__)
+def test_multigrid_coarse_solve_matches_oracle(gpu_lib, mk, spec):
+    """coarse_solver_choice "mg": GPU and oracle agree cycle by cycle (the
+    norms differ only in summation order), take the same number of V-cycles
+    in the last coarse solve, and converge to the red-black path's solution
+    within the solver tolerance."""
+    case = mk()
+    spec = dict(spec)
+    if spec.pop("robin", False):
+        spec["bc"] = hc.bc_robin(case["dim"])
+    data = smooth_data(case, 3)
+    pr = hc.Pair(case, MG, data, **spec)
+    print("mg solve", case["name"], spec, flush=True)
+    conv_g = pr.s.solve_system(pr.u, pr.f)
+    hist_g = pr.s.history()
+    conv_o, it_o, res_o, hist_o = pr.o.solve()
+    assert conv_g and conv_o
+    assert pr.s.number_of_iterations() == it_o
+    np.testing.assert_allclose(hist_g, hist_o, rtol=1e-10, atol=0)
+    assert pr.s.coarse_mg_stats()[0] == pr.o.mg_stats()[0]
+    pr.compare(pr.u, "u", interior_only=True, rtol=1e-10)
+    # same fixed point as the red-black coarse solver
+    rb = hc.Pair(case, PARAMS, data, **spec)
+    rb.s.solve_system(rb.u, rb.f)
+    for ln in range(len(case["levels"])):
+        for p in range(len(case["levels"][ln])):
+            a, b = pr.u.get(ln, p), rb.u.get(ln, p)
+            assert np.abs(a - b).max() <= 1e-7 * max(np.abs(b).max(), 1e-300)

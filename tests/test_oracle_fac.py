@@ -122,3 +122,36 @@ def test_stopping_rule_uses_relative_tolerance():
     assert len(hist) == 1 or hist[-2] > target
     # the history decreases monotonically for this smooth problem
     assert np.all(np.diff(hist) < 0)
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_multigrid_coarse_solver_reaches_the_same_solution(levels):
+    """The FAC fixed point does not depend on the coarse solver: with the
+    multigrid coarsest-level solve the converged solution equals the one
+    with red-black smoothing to the solver tolerance, in no more cycles
+    (500 red-black sweeps already solve a 32^2 coarsest level well), and
+    each coarse solve meets its relative-residual test."""
+    n = 32
+    sols, cycles = [], []
+    for mg in (False, True):
+        o, boxes = sine_problem(n, levels)
+        if mg:
+            o.set_coarse_mg(True)
+            o.set_params(max_cycles=400, residual_tol=1e-300, relative_residual_tol=1e-11,
+                         pre=2, post=2, coarse_max_iterations=20, coarse_tol=1e-10,
+                         ewing=True)
+        else:
+            o.set_params(max_cycles=400, residual_tol=1e-300, relative_residual_tol=1e-11,
+                         pre=2, post=2, coarse_max_iterations=500, coarse_tol=1e-13,
+                         ewing=True)
+        conv, it, res, hist = o.solve()
+        assert conv
+        if mg:
+            its, rel = o.mg_stats()
+            assert rel <= 1e-10 and its < 20
+        cycles.append(it)
+        sols.append(np.concatenate([o.get("u", ln, p) for ln in range(len(boxes))
+                                    for p in range(len(boxes[ln]))]))
+    assert cycles[1] <= cycles[0]
+    scale = np.abs(sols[0]).max()
+    assert np.abs(sols[1] - sols[0]).max() <= 1e-8 * scale
