@@ -791,8 +791,8 @@ static double full_composite_residual(orc_fac* s) {
  * MG level m+1 halves every extent; V(2,2) cycles of red-black Gauss-Seidel
  * (red: i+j+k even in box indices), full-weighting restriction of the
  * residual (mean of the children), separable linear prolongation (3/4, 1/4
- * per axis, homogeneous Robin ghosts beyond the domain), and 50 sweeps on a
- * grid of at most 8 cells per axis; stop when ||r||_2 <= ctol ||b||_2 or
+ * per axis, homogeneous Robin ghosts beyond the domain), and 60 sweeps on a
+ * grid of at most 4 cells per axis; stop when ||r||_2 <= ctol ||b||_2 or
  * after cmax cycles (hypre's relative-residual test).  Per-cell arithmetic
  * follows the red-black kernel (rbgswithfluxmax*, expression order kept).
  */
@@ -804,8 +804,8 @@ typedef struct {
 
 #define MG_PRE 2
 #define MG_POST 2
-#define MG_SMALL_EDGE 8
-#define MG_SMALL_SWEEPS 50
+#define MG_SMALL_EDGE 4
+#define MG_SMALL_SWEEPS 60
 
 static double mg_ghost(const orc_fac* s, const MGL* G, int loc, double uin) {
   const int n = loc >> 1;

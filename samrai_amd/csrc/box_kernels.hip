@@ -615,25 +615,44 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArrays Af, BoxLevel Gc, double* Dc0, double* D
   }
 }
 
-// The smallest grid (at most kMGSmall cells and 8 per axis): u = 0, then
-// `sweeps` red-black sweeps with the whole grid and its ghosts in LDS; one
-// work-group.
-constexpr int kMGSmallEdge = 8;
+// The smallest grid (at most kMGSmallEdge cells per axis): u = 0, then
+// `sweeps` red-black sweeps with the whole grid and its ghosts in LDS and
+// each thread's cell coefficients in registers; one work-group.
+constexpr int kMGSmallEdge = 4;
 template <int DIM, bool DVAR, int CMODE>
 __global__ void __launch_bounds__(kThreads)
 k_mg_small_solve(BoxLevel G, BoxArrays A, RobinDev bc, double* u, int sweeps, const int* stop) {
   constexpr int L = kMGSmallEdge + 2;
   constexpr int SY = L, SZ = L * L;
+  constexpr int NMAX = DIM == 3 ? kMGSmallEdge * kMGSmallEdge * kMGSmallEdge
+                                : kMGSmallEdge * kMGSmallEdge;
+  constexpr int NC = (NMAX + kThreads - 1) / kThreads;
   __shared__ double U[DIM == 3 ? L * L * L : L * L];
   if (*stop) return;
   const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
+  const int ncell = n0 * n1 * n2;
   for (int q = threadIdx.x; q < (DIM == 3 ? L * L * L : L * L); q += kThreads) U[q] = 0.0;
+  int qc[NC], col[NC];
+  Coef cf[NC];
+#pragma unroll
+  for (int v = 0; v < NC; ++v) {
+    const int t = threadIdx.x + v * kThreads;
+    qc[v] = -1;
+    if (t < ncell) {
+      const int i = t % n0, j = (t / n0) % n1, k = DIM == 3 ? t / (n0 * n1) : 0;
+      qc[v] = (i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0);
+      col[v] = (i + j + k) & 1;
+      cf[v] = load_coef<DIM, DVAR, CMODE>(G, A, i, j, k);
+    }
+  }
   __syncthreads();
   auto ghosts = [&]() {
-    const long ncell = (long)n0 * n1 * n2;
-    for (long t = threadIdx.x; t < ncell; t += kThreads) {
-      const int i = (int)(t % n0), j = (int)((t / n0) % n1), k = DIM == 3 ? (int)(t / ((long)n0 * n1)) : 0;
-      const int q = (i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0);
+#pragma unroll
+    for (int v = 0; v < NC; ++v) {
+      if (qc[v] < 0) continue;
+      const int t = threadIdx.x + v * kThreads;
+      const int i = t % n0, j = (t / n0) % n1, k = DIM == 3 ? t / (n0 * n1) : 0;
+      const int q = qc[v];
       if (i == 0) U[q - 1] = robin_ghost<DIM>(G, bc, 1, 0, U[q]);
       if (i == n0 - 1) U[q + 1] = robin_ghost<DIM>(G, bc, 1, 1, U[q]);
       if (j == 0) U[q - SY] = robin_ghost<DIM>(G, bc, 1, 2, U[q]);
@@ -642,24 +661,19 @@ k_mg_small_solve(BoxLevel G, BoxArrays A, RobinDev bc, double* u, int sweeps, co
       if (DIM == 3 && k == n2 - 1) U[q + SZ] = robin_ghost<DIM>(G, bc, 1, 5, U[q]);
     }
   };
-  const long ncell = (long)n0 * n1 * n2;
   for (int s = 0; s < sweeps; ++s)
     for (int colour = 0; colour < 2; ++colour) {
       ghosts();
       __syncthreads();
-      for (long t = threadIdx.x; t < ncell; t += kThreads) {
-        const int i = (int)(t % n0), j = (int)((t / n0) % n1), k = DIM == 3 ? (int)(t / ((long)n0 * n1)) : 0;
-        if (((i + j + k) & 1) != colour) continue;
-        const int q = (i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0);
-        cell_update<DIM, DVAR, CMODE>(U, q, SY, SZ, load_coef<DIM, DVAR, CMODE>(G, A, i, j, k), G,
-                                      A);
-      }
+#pragma unroll
+      for (int v = 0; v < NC; ++v)
+        if (qc[v] >= 0 && col[v] == colour)
+          cell_update<DIM, DVAR, CMODE>(U, qc[v], SY, SZ, cf[v], G, A);
       __syncthreads();
     }
-  for (long t = threadIdx.x; t < ncell; t += kThreads) {
-    const int i = (int)(t % n0), j = (int)((t / n0) % n1), k = DIM == 3 ? (int)(t / ((long)n0 * n1)) : 0;
-    u[t] = U[(i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0)];
-  }
+#pragma unroll
+  for (int v = 0; v < NC; ++v)
+    if (qc[v] >= 0) u[threadIdx.x + v * kThreads] = U[qc[v]];
 }
 
 // Partial sums of the squared residual (x == nullptr) or of x^2 over a

@@ -449,7 +449,7 @@ sfac::BoxArrays CellPoissonFACOps::gatherBox(PatchDataSet& e, const PatchDataSet
 // and a cycle skipped after convergence leaves the result in place.
 
 namespace {
-constexpr int kMGPre = 2, kMGPost = 2, kMGSmallSweeps = 50, kMGLastSweeps = 64;
+constexpr int kMGPre = 2, kMGPost = 2, kMGSmallSweeps = 60, kMGLastSweeps = 64;
 }
 
 void CellPoissonFACOps::releaseMG() {
