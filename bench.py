@@ -90,6 +90,53 @@ def cpu_baseline(decks, deck, cycles_gpu, ncycles, threads, mg=False):
                        % (it, threads, secs, per_cycle, cycles_gpu))
 
 
+def setup_c2(fac, decks, torch, params, seed=0):
+    """C2: 4 levels, ~2100 patches with sides from U{8..48}, D = exp(GRF)
+    (correlation length 0.1, sigma 1) on the faces, f = eight Gaussian
+    sources of width 0.02; generated on the GPU and copied device to device."""
+    deck = decks.deck_3d_c2(seed)
+    h = fac.PatchHierarchy(3, deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
+    for boxes in deck["levels"]:
+        h.add_level(deck["ratio"], boxes)
+    h.finalize()
+    u = fac.PatchData(h, 1)
+    f = fac.PatchData(h, 0)
+    D = fac.PatchData(h, 0, "side")
+    modes = decks.grf_modes(seed)
+    kw = dict(dtype=torch.float64, device="cuda")
+    for ln, boxes in enumerate(deck["levels"]):
+        for p, box in enumerate(boxes):
+            fv = decks.gaussian_sources(deck, decks.cell_points(deck, ln, box, xp=torch, **kw),
+                                        xp=torch)
+            f.put_device(ln, p, fv.permute(2, 1, 0).contiguous())
+            for a in range(3):
+                g = decks.grf_eval(decks.face_points(deck, ln, box, a, xp=torch, **kw), modes,
+                                   xp=torch)
+                D.put_device(ln, p, torch.exp(g).permute(2, 1, 0).contiguous(), axis=a)
+    s = fac.CellPoissonFACSolver(h, **params)
+    s.set_d_patch_data(D)
+    s.set_dirichlet(0.0)
+    s.initialize_solver_state(u, f)
+    return deck, h, u, f, D, s
+
+
+def time_c2(fac, decks, torch, params):
+    deck, h, u, f, D, s = setup_c2(fac, decks, torch, params)
+    u.fill(0.0)
+    s.solve_system(u, f)  # warm-up (graph capture, autotuning)
+    u.fill(0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ok = s.solve_system(u, f)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    out = dict(s_per_solve=round(dt, 4), cycles=s.number_of_iterations(), converged=bool(ok),
+               final_residual=float(s.history()[-1]))
+    if params.get("coarse_solver_choice") == "mg":
+        out["last_coarse_vcycles"] = s.coarse_mg_stats()[0]
+    return deck, out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +145,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cycles", type=int, default=2)
     ap.add_argument("--smoother-sweeps", type=int, default=200)
+    ap.add_argument("--no-c2", action="store_true",
+                    help="skip the variable-coefficient 4-level deck (C2) section")
     ap.add_argument("--mode", choices=["solve", "smoother"], default="solve",
                     help="smoother: only a few smoother launches and a 256 MiB copy of known "
                          "size (for hardware-counter passes); prints no benchmark line")
@@ -224,6 +273,18 @@ def main():
     print("bench: mg coarse solver: %.4f s/solve, %d cycles" % (mg["s_per_solve"], mg["cycles"]),
           file=sys.stderr, flush=True)
 
+    c2 = None
+    if not args.no_c2:
+        print("bench: C2 deck (variable D), red-black and multigrid coarse solvers",
+              file=sys.stderr, flush=True)
+        deck2, rb2 = time_c2(fac, decks, torch, PARAMS)
+        _, mg2 = time_c2(fac, decks, torch, MG_PARAMS)
+        c2 = dict(workload="c2-3d: 4 levels ratio 2, base 128^3, patch sides U{8..48} (seed 0), "
+                           "D = exp(GRF, corr 0.1, sigma 1), f = 8 Gaussian sources (width 0.02)",
+                  patches=sum(len(l) for l in deck2["levels"]), cells=decks.count_cells(deck2),
+                  redblack_coarse=rb2, mg_coarse=mg2)
+        print("bench: C2 %s" % json.dumps(c2), file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -251,7 +312,7 @@ def main():
                smoother_gcells_per_s=round(smoother_gcells, 2),
                finest_smoother_gcells_per_s=round(finest_gcells, 2),
                fused_coarsest=fused, fused_tile=tile, fused_tile_autotune_us_per_sweep=tuning,
-               roofline=roofline, cpu_baseline=cpu, with_mg_coarse_solver=mg)
+               roofline=roofline, cpu_baseline=cpu, with_mg_coarse_solver=mg, c2=c2)
     print(json.dumps(out), flush=True)
 
 

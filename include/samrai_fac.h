@@ -183,6 +183,9 @@ int sfac_patch_data_pointer(sfac_patch_data* d, int ln, int p, int axis, double*
 /* Synchronous host copies of one patch array (whole ghost box). */
 int sfac_patch_data_put(sfac_patch_data* d, int ln, int p, int axis, const double* host);
 int sfac_patch_data_get(sfac_patch_data* d, int ln, int p, int axis, double* host);
+/* The same from device memory the caller owns (synchronous device-to-device copy). */
+int sfac_patch_data_put_device(sfac_patch_data* d, int ln, int p, int axis,
+                               const double* device_src);
 int sfac_patch_data_fill(sfac_patch_data* d, double value);
 
 /* Input keys of fac_precond (FACPreconditioner.C:79-114) and fac_ops

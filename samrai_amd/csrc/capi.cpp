@@ -156,6 +156,17 @@ int sfac_patch_data_put(sfac_patch_data* d, int ln, int p, int axis, const doubl
   });
 }
 
+int sfac_patch_data_put_device(sfac_patch_data* d, int ln, int p, int axis,
+                               const double* device_src) {
+  return guarded([&] {
+    check_index(d, ln, p, axis);
+    need(device_src, "device array");
+    hip_check(hipMemcpy(d->d->pointer(ln, p, axis), device_src,
+                        sizeof(double) * d->d->size(ln, p, axis), hipMemcpyDeviceToDevice),
+              "hipMemcpy(put_device)");
+  });
+}
+
 int sfac_patch_data_get(sfac_patch_data* d, int ln, int p, int axis, double* host) {
   return guarded([&] {
     check_index(d, ln, p, axis);

@@ -12,7 +12,9 @@ describe, deterministically by seed:
                   percentile of the level, clustered into boxes of 16^3..64^3;
   deck_3d_c2(s)   3-D, four levels at ratio 2, base 128^3, patches whose
                   per-dimension sizes are drawn from U{8..48}, refined around
-                  eight Gaussian sources.
+                  eight Gaussian sources; D = exp(g) with g a Gaussian random
+                  field (correlation length 0.1, sigma 1; grf_modes / grf_eval)
+                  and f the sum of the eight Gaussians of width 0.02.
 
 Boxes are ((lo...), (hi...)) in each level's index space.
 """
@@ -202,4 +204,57 @@ def gaussian_rhs(deck, ln, box, width=0.02):
     for c, a in zip(deck["centres"], deck["amps"]):
         r2 = sum((X[d] - c[d]) ** 2 for d in range(3))
         f += a * np.exp(-r2 / (2 * width ** 2))
+    return f
+
+
+def grf_modes(seed, corr=0.1, nmodes=256):
+    """Random Fourier features of a stationary Gaussian random field with
+    covariance exp(-|r|^2 / (2 corr^2)) and unit variance:
+    g(x) = sqrt(2 / M) sum_m cos(k_m . x + phi_m), k_m ~ N(0, I / corr^2),
+    phi_m ~ U[0, 2 pi)."""
+    rng = np.random.default_rng(1000 + seed)
+    k = rng.standard_normal((nmodes, 3)) / corr
+    phi = rng.uniform(0.0, 2 * PI, nmodes)
+    return k, phi
+
+
+def grf_eval(points, modes, xp=np):
+    """g at points (..., 3) with the array module xp (numpy or torch)."""
+    k, phi = modes
+    if xp is not np:
+        k = xp.as_tensor(k, dtype=points.dtype, device=points.device)
+        phi = xp.as_tensor(phi, dtype=points.dtype, device=points.device)
+    arg = points @ k.T + phi
+    return (2.0 / k.shape[0]) ** 0.5 * xp.cos(arg).sum(-1)
+
+
+def face_points(deck, ln, box, axis, xp=np, **kw):
+    """Face-centre coordinates (n0', n1', n2', 3) of the side data of `axis`
+    on `box` (i fastest when flattened in Fortran order)."""
+    lo, hi = box
+    dx = level_dx(deck, ln)
+    axes = []
+    for d in range(3):
+        if d == axis:
+            axes.append(xp.arange(lo[d], hi[d] + 2, **kw) * dx[d])
+        else:
+            axes.append((xp.arange(lo[d], hi[d] + 1, **kw) + 0.5) * dx[d])
+    g = xp.meshgrid(*axes, indexing="ij")
+    return xp.stack(g, -1)
+
+
+def cell_points(deck, ln, box, xp=np, **kw):
+    lo, hi = box
+    dx = level_dx(deck, ln)
+    axes = [(xp.arange(lo[d], hi[d] + 1, **kw) + 0.5) * dx[d] for d in range(3)]
+    return xp.stack(xp.meshgrid(*axes, indexing="ij"), -1)
+
+
+def gaussian_sources(deck, points, width=0.02, xp=np):
+    """f = sum_s a_s exp(-|x - c_s|^2 / (2 width^2)) at points (..., 3)."""
+    f = 0.0 * points[..., 0]
+    for c, a in zip(deck["centres"], deck["amps"]):
+        r2 = ((points[..., 0] - float(c[0])) ** 2 + (points[..., 1] - float(c[1])) ** 2 +
+              (points[..., 2] - float(c[2])) ** 2)
+        f = f + float(a) * xp.exp(-r2 / (2 * width ** 2))
     return f

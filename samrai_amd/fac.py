@@ -55,6 +55,7 @@ def lib():
                                           C.POINTER(C.c_long)]
     L.sfac_patch_data_put.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp]
     L.sfac_patch_data_get.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp]
+    L.sfac_patch_data_put_device.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp]
     L.sfac_patch_data_fill.argtypes = [_vp, C.c_double]
     L.sfac_solver_params_default.argtypes = [C.POINTER(_Params)]
     L.sfac_solver_params_default.restype = None
@@ -186,6 +187,17 @@ class PatchData:
         a = np.empty(self.size(ln, p, axis), dtype=np.float64)
         _check(lib().sfac_patch_data_get(self.d, ln, p, axis, a.ctypes.data_as(_dp)))
         return a
+
+    def put_device(self, ln, p, tensor, axis=0):
+        """Copy a device tensor holding the whole ghost box of patch p
+        (i fastest in memory, float64) into the patch array."""
+        t = tensor
+        if t.element_size() != 8 or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError("put_device needs a contiguous float64 device tensor")
+        if t.numel() != self.size(ln, p, axis):
+            raise ValueError("tensor size %d != patch array size %d"
+                             % (t.numel(), self.size(ln, p, axis)))
+        _check(lib().sfac_patch_data_put_device(self.d, ln, p, axis, C.c_void_p(t.data_ptr())))
 
     def fill(self, value):
         _check(lib().sfac_patch_data_fill(self.d, float(value)))
