@@ -570,7 +570,8 @@ void CellPoissonFACOps::mgVcycle(int m) {
 void CellPoissonFACOps::solveBoxMG(PatchDataSet& e, const PatchDataSet& r) {
   if (!d_box.ok || d_mg.empty())
     fail("CellPoissonFACOps: coarse_solver_choice '" + d_coarse_choice +
-         "' needs a coarsest level whose patches tile the physical domain");
+         "' needs a coarsest level whose patches tile the physical domain (and, with "
+         "variable D, hold equal values on the faces they share)");
   const int ln = d_box.ln;
   const auto& T = d_h.tables(ln);
   const int dim = d_h.dim();
@@ -825,7 +826,7 @@ void CellPoissonFACOps::smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal&
 int CellPoissonFACOps::solveCoarsestLevel(SAMRAIVectorReal& e, const SAMRAIVectorReal& r,
                                           int coarsest_ln) {
   if (d_coarse_choice == "mg" || d_coarse_choice == "hypre") {
-    if (coarsest_ln != d_box.ln)
+    if (d_box.ok && coarsest_ln != d_box.ln)
       fail("CellPoissonFACOps: multigrid coarse solve on a level other than the coarsest");
     solveBoxMG(*e.data, *r.data);
   } else {

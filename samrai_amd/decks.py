@@ -207,7 +207,7 @@ def gaussian_rhs(deck, ln, box, width=0.02):
     return f
 
 
-def grf_modes(seed, corr=0.1, nmodes=256):
+def grf_modes(seed, corr=0.1, nmodes=256):  # nmodes: a power of two
     """Random Fourier features of a stationary Gaussian random field with
     covariance exp(-|r|^2 / (2 corr^2)) and unit variance:
     g(x) = sqrt(2 / M) sum_m cos(k_m . x + phi_m), k_m ~ N(0, I / corr^2),
@@ -224,8 +224,16 @@ def grf_eval(points, modes, xp=np):
     if xp is not np:
         k = xp.as_tensor(k, dtype=points.dtype, device=points.device)
         phi = xp.as_tensor(phi, dtype=points.dtype, device=points.device)
-    arg = points @ k.T + phi
-    return (2.0 / k.shape[0]) ** 0.5 * xp.cos(arg).sum(-1)
+    # elementwise only (no matrix product, no library reduction, a fixed
+    # pairwise sum over the modes): each point's value is the same bits
+    # whatever array it is evaluated in, so patches sharing a face agree
+    arg = (points[..., 0:1] * k[:, 0] + points[..., 1:2] * k[:, 1] +
+           points[..., 2:3] * k[:, 2] + phi)
+    c = xp.cos(arg)
+    while c.shape[-1] > 1:
+        h = c.shape[-1] // 2
+        c = c[..., :h] + c[..., h:2 * h]
+    return (2.0 / k.shape[0]) ** 0.5 * c[..., 0]
 
 
 def face_points(deck, ln, box, axis, xp=np, **kw):
