@@ -33,6 +33,16 @@ def case_2d_c0(n=32):
     return dict(name="2dc0", dim=2, levels=[L0, L1], ratio=2, dlo=(0, 0), dhi=(n - 1, n - 1))
 
 
+def case_3d_wide0():
+    """Level 0 of 40 x 36 x 44 cells in four patches (several tiles and
+    z-segments of every fused tile shape, ragged edges), one finer level."""
+    L0 = [((0, 0, 0), (19, 35, 21)), ((20, 0, 0), (39, 35, 21)),
+          ((0, 0, 22), (23, 35, 43)), ((24, 0, 22), (39, 35, 43))]
+    L1 = [((20, 20, 20), (43, 39, 47)), ((44, 20, 20), (59, 39, 35))]
+    return dict(name="3dwide0", dim=3, levels=[L0, L1], ratio=2, dlo=(0, 0, 0),
+                dhi=(39, 35, 43))
+
+
 def case_2d_odd_split():
     """Level 0 split at an odd x: the 2-D patch-relative colourings of the
     two patches differ, so the coarsest level cannot be swept as one box."""

@@ -52,7 +52,7 @@ FIXED = dict(PARAMS, max_cycles=4, relative_residual_tol=1e-300, coarse_solver_m
 
 @pytest.mark.parametrize("spec", [dict(dvar=False), dict(dvar=True, cmode=2, robin=True)],
                          ids=["constD", "varD-varC-robin"])
-@pytest.mark.parametrize("mk", hc.ALL + [hc.case_2d_c0, hc.case_2d_odd_split],
+@pytest.mark.parametrize("mk", hc.ALL + [hc.case_2d_c0, hc.case_2d_odd_split, hc.case_3d_wide0],
                          ids=lambda f: f.__name__)
 def test_fused_coarsest_sweeps_are_bit_identical(gpu_lib, mk, spec):
     """The colour-fused coarsest-level sweeps (every tile shape), with and
