@@ -192,7 +192,6 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
   constexpr int NRW = HX * RY * RZ, NR = (NRW + kThreads - 1) / kThreads;
   constexpr int HB = TX / 2, BZ = DIM == 3 ? TZ : 1;
   constexpr int NBW = HB * TY * BZ, NB = (NBW + kThreads - 1) / kThreads;
-  constexpr int NTOT = LX * LY * LZ, NL = (NTOT + kThreads - 1) / kThreads;
   auto red_cell = [&](int w, int& lx, int& ly, int& lz) {
     const int m = w % HX;
     ly = 1 + (w / HX) % RY;
@@ -211,22 +210,45 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
 
   // 2. issue every global load of the sweep up front: the grown tile of u
   // and the coefficients of each cell this thread updates
-  double v[NL];
+  // u: each thread loads whole z-columns of the grown tile (one index
+  // decomposition per column, then a fixed stride per plane)
+  constexpr int NCOL = (LX * LY + kThreads - 1) / kThreads;
+  const long plane = (long)n0 * n1;
+  double v[NCOL][LZ];
 #pragma unroll
-  for (int u = 0; u < NL; ++u) {
-    const int q = threadIdx.x + u * kThreads;
-    const int lx = q % LX, ly = (q / LX) % LY, lz = q / (LX * LY);
-    const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
-    v[u] = (q < NTOT && inside(i, j, k)) ? A.ein[(long)i + (long)n0 * (j + (long)n1 * k)] : 0.0;
+  for (int c = 0; c < NCOL; ++c) {
+    const int col = threadIdx.x + c * kThreads;
+    const int lx = col % LX, ly = col / LX;
+    const int i = x0 - 2 + lx, j = y0 - 2 + ly;
+    const bool xy = col < LX * LY && i >= 0 && i < n0 && j >= 0 && j < n1;
+    const double* src = A.ein + ((long)i + (long)n0 * j);
+#pragma unroll
+    for (int lz = 0; lz < LZ; ++lz) {
+      const int k = z0 - ZH + lz;
+      v[c][lz] = (xy && (DIM == 2 || (k >= 0 && k < n2))) ? src[plane * k] : 0.0;
+    }
   }
+  // the cells this thread updates: LDS index, flags (1 update, 2 counts
+  // for the early-exit maximum) and coefficients
+  const bool lp_tile = K.cur != nullptr;
+  auto in_lp = [&](int i, int j, int k) {
+    return i >= G.lp_lo[0] && i <= G.lp_hi[0] && j >= G.lp_lo[1] && j <= G.lp_hi[1] &&
+           (DIM == 2 || (k >= G.lp_lo[2] && k <= G.lp_hi[2]));
+  };
   Coef cr[NR], cb[NB];
+  int rq[NR], rf[NR], bq[NB], bf[NB];
 #pragma unroll
   for (int u = 0; u < NR; ++u) {
     const int w = threadIdx.x + u * kThreads;
     int lx, ly, lz;
     red_cell(w, lx, ly, lz);
     const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
-    cr[u] = (w < NRW && inside(i, j, k)) ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef{};
+    const bool act = w < NRW && inside(i, j, k);
+    const bool own = lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2 &&
+                     (DIM == 2 || (lz >= 2 && lz < TZ + 2));
+    rq[u] = lx + SY * ly + SZ * lz;
+    rf[u] = (act ? 1 : 0) | ((act && lp_tile && own && in_lp(i, j, k)) ? 2 : 0);
+    cr[u] = act ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef{};
   }
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
@@ -234,12 +256,17 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
     int lx, ly, lz;
     black_cell(w, lx, ly, lz);
     const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
-    cb[u] = (w < NBW && inside(i, j, k)) ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef{};
+    const bool act = w < NBW && inside(i, j, k);
+    bq[u] = lx + SY * ly + SZ * lz;
+    bf[u] = (act ? 1 : 0) | ((act && lp_tile && in_lp(i, j, k)) ? 2 : 0);
+    cb[u] = act ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef{};
   }
 #pragma unroll
-  for (int u = 0; u < NL; ++u) {
-    const int q = threadIdx.x + u * kThreads;
-    if (q < NTOT) U[q] = v[u];
+  for (int c = 0; c < NCOL; ++c) {
+    const int col = threadIdx.x + c * kThreads;
+    if (col < LX * LY)
+#pragma unroll
+      for (int lz = 0; lz < LZ; ++lz) U[col + SZ * lz] = v[c][lz];
   }
   __syncthreads();
 
@@ -272,26 +299,14 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
   ghosts();
   __syncthreads();
 
-  const bool lp_tile = K.cur != nullptr;
   double mr = 0.0;
-  auto in_lp = [&](int i, int j, int k) {
-    return i >= G.lp_lo[0] && i <= G.lp_hi[0] && j >= G.lp_lo[1] && j <= G.lp_hi[1] &&
-           (DIM == 2 || (k >= G.lp_lo[2] && k <= G.lp_hi[2]));
-  };
 
   // 3. red cells of the tile grown by one
 #pragma unroll
   for (int u = 0; u < NR; ++u) {
-    const int w = threadIdx.x + u * kThreads;
-    int lx, ly, lz;
-    red_cell(w, lx, ly, lz);
-    const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
-    if (w < NRW && inside(i, j, k)) {
-      const double ra =
-          cell_update<DIM, DVAR, CMODE>(U, lx + SY * ly + SZ * lz, SY, SZ, SZ, cr[u], G, A);
-      const bool owned = lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2 &&
-                         (DIM == 2 || (lz >= 2 && lz < TZ + 2));
-      if (lp_tile && owned && in_lp(i, j, k)) mr = fmax(mr, ra);
+    if (rf[u] & 1) {
+      const double ra = cell_update<DIM, DVAR, CMODE>(U, rq[u], SY, SZ, SZ, cr[u], G, A);
+      if (rf[u] & 2) mr = fmax(mr, ra);
     }
   }
   __syncthreads();
@@ -301,23 +316,24 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
   // 5. black cells of the owned tile
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
-    const int w = threadIdx.x + u * kThreads;
-    int lx, ly, lz;
-    black_cell(w, lx, ly, lz);
-    const int i = x0 - 2 + lx, j = y0 - 2 + ly, k = z0 - ZH + lz;
-    if (w < NBW && inside(i, j, k)) {
-      const double ra =
-          cell_update<DIM, DVAR, CMODE>(U, lx + SY * ly + SZ * lz, SY, SZ, SZ, cb[u], G, A);
-      if (lp_tile && in_lp(i, j, k)) mr = fmax(mr, ra);
+    if (bf[u] & 1) {
+      const double ra = cell_update<DIM, DVAR, CMODE>(U, bq[u], SY, SZ, SZ, cb[u], G, A);
+      if (bf[u] & 2) mr = fmax(mr, ra);
     }
   }
   __syncthreads();
-  for (int w = threadIdx.x; w < TX * TY * (DIM == 3 ? TZ : 1); w += kThreads) {
-    const int ox = w % TX, oy = (w / TX) % TY, oz = DIM == 3 ? w / (TX * TY) : 0;
-    const int i = x0 + ox, j = y0 + oy, k = z0 + oz;
-    if (i >= n0 || j >= n1 || (DIM == 3 && k >= n2)) continue;
-    A.eout[(long)i + (long)n0 * (j + (long)n1 * k)] =
-        U[(ox + 2) + SY * (oy + 2) + SZ * (oz + ZH)];
+  constexpr int NOC = (TX * TY + kThreads - 1) / kThreads;
+#pragma unroll
+  for (int c = 0; c < NOC; ++c) {
+    const int col = threadIdx.x + c * kThreads;
+    const int ox = col % TX, oy = col / TX;
+    const int i = x0 + ox, j = y0 + oy;
+    if (col >= TX * TY || i >= n0 || j >= n1) continue;
+    double* dst = A.eout + ((long)i + (long)n0 * j);
+    const double* src = U + (ox + 2) + SY * (oy + 2) + SZ * ZH;
+#pragma unroll
+    for (int oz = 0; oz < (DIM == 3 ? TZ : 1); ++oz)
+      if (DIM == 2 || z0 + oz < n2) dst[plane * (z0 + oz)] = src[SZ * oz];
   }
   if (lp_tile) {
     const int ti[3] = {tx - K.tlo[0], ty - K.tlo[1], DIM == 3 ? tz - K.tlo[2] : 0};
