@@ -406,227 +406,6 @@ k_box_side(LevelGeom L, const double* const* x, int g, int a, double* box, BoxLe
   }
 }
 
-// 2.5-D variant (3-D): a work-group owns TX x TY columns over ZS planes and
-// marches up z with a ring of four LDS planes of the columns grown by two.
-// Step k: red cells of plane k (columns grown by one; the halo reds equal
-// their owners' bits as above), face ghosts refreshed, black cells of plane
-// k-1 (owned columns), plane k-1 written out, plane k+2 stored into the ring
-// from registers (loaded one step ahead, with the coefficients of the next
-// step's cells).  No z halo is recomputed except one plane per segment end,
-// and every thread keeps fixed columns, so the index arithmetic is per
-// work-group, not per cell.
-template <bool DVAR, int CMODE, int TX, int TY, int ZS>
-__global__ void __launch_bounds__(kThreads)
-k_box_rbgs_z(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, int ntx, int nty,
-             int ntiles, int per_xcd) {
-  constexpr int LX = TX + 4, LY = TY + 4, NPL = LX * LY;
-  constexpr int NPOS = (NPL + kThreads - 1) / kThreads;
-  constexpr int SY = LX;
-  constexpr int HX = (TX + 2) / 2, NR = HX * (TY + 2);  // red cells per plane
-  constexpr int HB = TX / 2, NB = HB * TY;             // black cells per plane
-  static_assert(NR <= kThreads && NB <= kThreads, "one red and one black cell per thread");
-  __shared__ double U[4 * NPL];
-  __shared__ double wm[kThreads / 64];
-  __shared__ int s_stop;
-  const int b = blockIdx.x;
-  const int t = (b & 7) * per_xcd + (b >> 3);
-  if (t >= ntiles) return;
-  const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
-  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * ZS;
-  const int n0 = G.n[0], n1 = G.n[1], n2 = G.n[2];
-  const int z1 = min(z0 + ZS, n2) - 1;
-  const long plane = (long)n0 * n1;
-  if (stop_sweep(K, b, wm, s_stop)) return;
-
-  // fixed per-thread plane positions
-  int gpos[NPOS];    // global column offset i + n0 j, or -1 outside the level
-  int kind[NPOS];    // 0 none, 1 interior, 2..5 face ghost x-,x+,y-,y+ (in-plane)
-  int owned[NPOS];
-#pragma unroll
-  for (int v = 0; v < NPOS; ++v) {
-    const int q = threadIdx.x + v * kThreads;
-    const int lx = q % LX, ly = q / LX;
-    const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-    const bool xin = i >= 0 && i < n0, yin = j >= 0 && j < n1;
-    gpos[v] = (q < NPL && xin && yin) ? i + n0 * j : -1;
-    kind[v] = 0;
-    if (q < NPL) {
-      if (xin && yin) kind[v] = 1;
-      else if (yin && i == -1) kind[v] = 2;
-      else if (yin && i == n0) kind[v] = 3;
-      else if (xin && j == -1) kind[v] = 4;
-      else if (xin && j == n1) kind[v] = 5;
-    }
-    owned[v] = q < NPL && lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2 && xin && yin;
-  }
-  const int base = G.lo[0] + G.lo[1] + G.lo[2] + x0 + y0 - 4 + G.shift;
-  // this thread's red cell (columns grown by one) and black cell (owned)
-  const int rm = threadIdx.x % HX, rly = 1 + threadIdx.x / HX;
-  const int bm = threadIdx.x % HB, bly = 2 + threadIdx.x / HB;
-  auto red_lx = [&](int k) { return 1 + ((base + k + 1 + rly) & 1) + 2 * rm; };
-  auto black_lx = [&](int k) { return 2 + ((base + k + bly + 1) & 1) + 2 * bm; };
-  auto red_ok = [&](int k, int lx) {
-    const int i = x0 - 2 + lx, j = y0 - 2 + rly;
-    return threadIdx.x < NR && k >= 0 && k < n2 && i >= 0 && i < n0 && j >= 0 && j < n1;
-  };
-  auto black_ok = [&](int k, int lx) {
-    const int i = x0 - 2 + lx, j = y0 - 2 + bly;
-    return threadIdx.x < NB && k >= z0 && k <= z1 && i < n0 && j < n1;
-  };
-  auto slot = [&](int k) { return ((k - z0 + 4) & 3) * NPL; };
-  auto in_lp = [&](int i, int j, int k) {
-    return i >= G.lp_lo[0] && i <= G.lp_hi[0] && j >= G.lp_lo[1] && j <= G.lp_hi[1] &&
-           k >= G.lp_lo[2] && k <= G.lp_hi[2];
-  };
-  // u of plane p at this thread's positions (zero outside the level)
-  auto load_plane = [&](int p, double* r) {
-#pragma unroll
-    for (int v = 0; v < NPOS; ++v)
-      r[v] = (p >= 0 && p < n2 && gpos[v] >= 0) ? A.ein[gpos[v] + plane * p] : 0.0;
-  };
-  auto store_plane = [&](int p, const double* r) {
-    double* P = U + slot(p);
-#pragma unroll
-    for (int v = 0; v < NPOS; ++v) {
-      const int q = threadIdx.x + v * kThreads;
-      if (q < NPL) P[q] = r[v];
-    }
-  };
-  // in-plane face ghosts of plane p from its interior
-  auto plane_ghosts = [&](int p) {
-    if (p < 0 || p >= n2) return;
-    double* P = U + slot(p);
-#pragma unroll
-    for (int v = 0; v < NPOS; ++v) {
-      const int q = threadIdx.x + v * kThreads;
-      switch (kind[v]) {
-        case 2: P[q] = robin_ghost<3>(G, bc, homogeneous, 0, P[q + 1]); break;
-        case 3: P[q] = robin_ghost<3>(G, bc, homogeneous, 1, P[q - 1]); break;
-        case 4: P[q] = robin_ghost<3>(G, bc, homogeneous, 2, P[q + SY]); break;
-        case 5: P[q] = robin_ghost<3>(G, bc, homogeneous, 3, P[q - SY]); break;
-        default: break;
-      }
-    }
-  };
-  // z ghost plane g (-1 or n2) from its interior neighbour plane
-  auto zghost_plane = [&](int g) {
-    const int in = g < 0 ? 0 : n2 - 1;
-    const int loc = g < 0 ? 4 : 5;
-    double* P = U + slot(g);
-    const double* Q = U + slot(in);
-#pragma unroll
-    for (int v = 0; v < NPOS; ++v) {
-      const int q = threadIdx.x + v * kThreads;
-      if (q < NPL) P[q] = kind[v] == 1 ? robin_ghost<3>(G, bc, homogeneous, loc, Q[q]) : 0.0;
-    }
-  };
-  auto red_coef = [&](int k) {
-    const int lx = red_lx(k);
-    return red_ok(k, lx) ? load_coef<3, DVAR, CMODE>(G, A, x0 - 2 + lx, y0 - 2 + rly, k)
-                         : Coef{};
-  };
-  auto black_coef = [&](int k) {
-    const int lx = black_lx(k);
-    return black_ok(k, lx) ? load_coef<3, DVAR, CMODE>(G, A, x0 - 2 + lx, y0 - 2 + bly, k)
-                           : Coef{};
-  };
-
-  // prologue: planes z0-2, z0-1, z0 in the ring; plane z0+1 and the red
-  // coefficients of plane z0-1 in registers
-  {
-    double r0[NPOS], r1[NPOS], r2[NPOS];
-    load_plane(z0 - 2, r0);
-    load_plane(z0 - 1, r1);
-    load_plane(z0, r2);
-    store_plane(z0 - 2, r0);
-    store_plane(z0 - 1, r1);
-    store_plane(z0, r2);
-  }
-  double up[NPOS];
-  load_plane(z0 + 1, up);
-  Coef cr = red_coef(z0 - 1), cb = Coef{};
-  __syncthreads();
-  for (int p = z0 - 2; p <= z0; ++p) plane_ghosts(p);
-  if (z0 == 0) zghost_plane(-1);
-  if (z0 == n2 - 1) zghost_plane(n2);  // only for a one-plane top segment
-  __syncthreads();
-
-  double mr = 0.0;
-  for (int k = z0 - 1; k <= z1 + 1; ++k) {
-    // issue next step's loads
-    double upn[NPOS];
-    load_plane(k + 3, upn);
-    const Coef crn = red_coef(k + 1);
-    const Coef cbn = black_coef(k);
-    // red cells of plane k
-    {
-      const int lx = red_lx(k);
-      if (red_ok(k, lx)) {
-        const double ra = cell_update<3, DVAR, CMODE>(U + slot(k), lx + SY * rly, SY,
-                                                      slot(k) - slot(k - 1),
-                                                      slot(k + 1) - slot(k), cr, G, A);
-        const bool own = lx >= 2 && lx < TX + 2 && rly >= 2 && rly < TY + 2 && k >= z0 &&
-                         k <= z1;
-        if (K.cur && own && in_lp(x0 - 2 + lx, y0 - 2 + rly, k)) mr = fmax(mr, ra);
-      }
-    }
-    __syncthreads();
-    plane_ghosts(k);
-    if (k == 0) zghost_plane(-1);
-    if (k == n2 - 1) zghost_plane(n2);
-    __syncthreads();
-    // black cells of plane k-1
-    {
-      const int kb = k - 1;
-      const int lx = black_lx(kb);
-      if (black_ok(kb, lx)) {
-        const double ra = cell_update<3, DVAR, CMODE>(U + slot(kb), lx + SY * bly, SY,
-                                                      slot(kb) - slot(kb - 1),
-                                                      slot(kb + 1) - slot(kb), cb, G, A);
-        if (K.cur && in_lp(x0 - 2 + lx, y0 - 2 + bly, kb)) mr = fmax(mr, ra);
-      }
-    }
-    __syncthreads();
-    // plane k-1 is final: write its owned cells; plane k+2 enters the ring
-    if (k - 1 >= z0 && k - 1 <= z1) {
-      const double* P = U + slot(k - 1);
-#pragma unroll
-      for (int v = 0; v < NPOS; ++v)
-        if (owned[v]) A.eout[gpos[v] + plane * (k - 1)] = P[threadIdx.x + v * kThreads];
-    }
-    store_plane(k + 2, up);
-    __syncthreads();
-    if (k + 2 == n2) zghost_plane(n2);
-    else plane_ghosts(k + 2);
-    __syncthreads();
-#pragma unroll
-    for (int v = 0; v < NPOS; ++v) up[v] = upn[v];
-    cr = crn;
-    cb = cbn;
-  }
-  if (K.cur) {
-    const int ti[3] = {tx - K.tlo[0], ty - K.tlo[1], tz - K.tlo[2]};
-    const bool meets = ti[0] >= 0 && ti[0] < K.tn[0] && ti[1] >= 0 && ti[1] < K.tn[1] &&
-                       ti[2] >= 0 && ti[2] < K.tn[2];
-    if (meets) {
-      mr = block_max(mr, wm);
-      if (threadIdx.x == 0) K.cur[ti[0] + K.tn[0] * (ti[1] + K.tn[1] * ti[2])] = mr;
-    }
-  }
-}
-
-template <bool DVAR, int CMODE, int TX, int TY, int ZS>
-void launch_tile_z(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int homogeneous,
-                   const BoxCheck& K, hipStream_t s) {
-  const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
-  const int ntz = (G.n[2] + ZS - 1) / ZS;
-  const int ntiles = ntx * nty * ntz;
-  const int per_xcd = (ntiles + 7) / 8;
-  hipLaunchKernelGGL((k_box_rbgs_z<DVAR, CMODE, TX, TY, ZS>), dim3(8 * per_xcd), dim3(kThreads),
-                     0, s, G, A, bc, homogeneous, K, ntx, nty, ntiles, per_xcd);
-  check_launch();
-}
-
 template <int DIM, bool DVAR, int CMODE, int TX, int TY, int TZ>
 void launch_tile(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int homogeneous,
                  const BoxCheck& K, hipStream_t s) {
@@ -640,10 +419,9 @@ void launch_tile(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int 
 }
 
 constexpr int kVariants = 9;
-// 3-D variants 6..8 are the z-marching kernel (tz = planes per segment)
-constexpr BoxTile kTiles3[kVariants] = {{32, 8, 8},  {16, 8, 8},   {32, 16, 4},
-                                        {64, 4, 4},  {32, 8, 4},   {16, 16, 4},
-                                        {16, 16, 16}, {16, 16, 32}, {32, 8, 16}};
+constexpr BoxTile kTiles3[kVariants] = {{32, 8, 8}, {16, 8, 8},  {32, 16, 4},
+                                        {64, 4, 4}, {32, 8, 4},  {16, 16, 4},
+                                        {16, 16, 8}, {32, 16, 8}, {24, 8, 8}};
 constexpr BoxTile kTiles2[kVariants] = {{64, 16, 1}, {32, 16, 1}, {32, 32, 1},
                                         {128, 8, 1}, {64, 8, 1},  {16, 16, 1},
                                         {64, 16, 1}, {64, 16, 1}, {64, 16, 1}};
@@ -658,9 +436,9 @@ void launch_variant(int variant, const BoxLevel& G, const BoxArrays& A, const Ro
       case 3: launch_tile<3, DVAR, CMODE, 64, 4, 4>(G, A, bc, homogeneous, K, s); break;
       case 4: launch_tile<3, DVAR, CMODE, 32, 8, 4>(G, A, bc, homogeneous, K, s); break;
       case 5: launch_tile<3, DVAR, CMODE, 16, 16, 4>(G, A, bc, homogeneous, K, s); break;
-      case 6: launch_tile_z<DVAR, CMODE, 16, 16, 16>(G, A, bc, homogeneous, K, s); break;
-      case 7: launch_tile_z<DVAR, CMODE, 16, 16, 32>(G, A, bc, homogeneous, K, s); break;
-      case 8: launch_tile_z<DVAR, CMODE, 32, 8, 16>(G, A, bc, homogeneous, K, s); break;
+      case 6: launch_tile<3, DVAR, CMODE, 16, 16, 8>(G, A, bc, homogeneous, K, s); break;
+      case 7: launch_tile<3, DVAR, CMODE, 32, 16, 8>(G, A, bc, homogeneous, K, s); break;
+      case 8: launch_tile<3, DVAR, CMODE, 24, 8, 8>(G, A, bc, homogeneous, K, s); break;
       default: launch_tile<3, DVAR, CMODE, 32, 8, 8>(G, A, bc, homogeneous, K, s); break;
     }
   } else {
