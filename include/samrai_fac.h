@@ -145,6 +145,13 @@ int sfac_cartclinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
                              const int* cihi, const int* filo, const int* fihi,
                              const int* ratio, const double* dxc, const double* dxf,
                              const double* arrayc, double* arrayf, hipStream_t stream);
+/* cartlinrefcelldoub{2d,3d}_: geom_m4cartrefineops3d.i:64-116, called from
+ * CartesianCellDoubleLinearRefine::refine (geom/CartesianCellDoubleLinearRefine.C:114-300) */
+int sfac_cartlinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
+                            const int* ifirstf, const int* ilastf, const int* cilo,
+                            const int* cihi, const int* filo, const int* fihi,
+                            const int* ratio, const double* dxc, const double* dxf,
+                            const double* arrayc, double* arrayf, hipStream_t stream);
 /* conrefcelldoub{2d,3d}_: pdat/fortran/pdat_m4conrefineops3d.i:33-..., called from
  * pdat::CellConstantRefine<double>::refine (pdat/CellConstantRefine.C:32-148) */
 int sfac_conrefcelldoub(int dim, const int* ifirstf, const int* ilastf, const int* cilo,
@@ -200,7 +207,8 @@ typedef struct {
   double coarse_solver_tolerance;
   int coarse_solver_max_iterations;
   const char* cf_discretization;     /* "Ewing" */
-  const char* prolongation_method;   /* "CONSERVATIVE_LINEAR_REFINE" | "CONSTANT_REFINE" */
+  const char* prolongation_method;   /* "CONSERVATIVE_LINEAR_REFINE" | "LINEAR_REFINE" |
+                                        "CONSTANT_REFINE" */
   int enable_logging;
 } sfac_solver_params;
 

@@ -75,6 +75,7 @@ struct orc_fac {
   int cmax; double ctol; int ewing; int nthreads;
   double last_seconds;
   int coarse_mg;                 /* coarsest level by multigrid (mg_solve) */
+  int prolong_method;            /* 0 conservative linear, 1 constant, 2 linear */
   int mg_iterations;             /* V-cycles of the last coarse solve */
   double mg_relres;              /* its final relative residual */
 };
@@ -718,8 +719,15 @@ static void prolong(orc_fac* s, int ef, int ln) {
       for (int d = 0; d < dim; ++d) { rq.lo[d] *= r; rq.hi[d] = (rq.hi[d] + 1) * r - 1; }
       if (!intersect(&FP->b, &rq, &o, dim)) continue;
       Box cb = coarsen_box(o, r, dim);
-      orc_cartclinrefcelldoub(dim, cb.lo, cb.hi, o.lo, o.hi, Q->gb.lo, Q->gb.hi, FP->gb.lo,
-                              FP->gb.hi, rr, C->dx, Fl->dx, vec(Q, ef), FP->scr);
+      if (s->prolong_method == 1)
+        orc_conrefcelldoub(dim, o.lo, o.hi, Q->gb.lo, Q->gb.hi, FP->gb.lo, FP->gb.hi, rr,
+                           vec(Q, ef), FP->scr);
+      else if (s->prolong_method == 2)
+        orc_cartlinrefcelldoub(dim, cb.lo, cb.hi, o.lo, o.hi, Q->gb.lo, Q->gb.hi, FP->gb.lo,
+                               FP->gb.hi, rr, C->dx, Fl->dx, vec(Q, ef), FP->scr);
+      else
+        orc_cartclinrefcelldoub(dim, cb.lo, cb.hi, o.lo, o.hi, Q->gb.lo, Q->gb.hi, FP->gb.lo,
+                                FP->gb.hi, rr, C->dx, Fl->dx, vec(Q, ef), FP->scr);
     }
     double* e = vec(FP, ef);
     for (int k = (dim == 3 ? FP->b.lo[2] : 0); k <= (dim == 3 ? FP->b.hi[2] : 0); ++k)
@@ -1112,6 +1120,7 @@ static void mg_solve(orc_fac* s) {
 }
 
 void orc_fac_set_coarse_mg(orc_fac* s, int on) { s->coarse_mg = on; }
+void orc_fac_set_prolongation(orc_fac* s, int method) { s->prolong_method = method; }
 int orc_fac_mg_iterations(orc_fac* s) { return s->mg_iterations; }
 double orc_fac_mg_relres(orc_fac* s) { return s->mg_relres; }
 

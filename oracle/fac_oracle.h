@@ -174,6 +174,15 @@ void orc_cartclinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
                              const int* ratio, const double* dxc,
                              const double* dxf, const double* arrayc,
                              double* arrayf);
+/* cartlinrefcelldoub{2,3}d: geom_m4cartrefineops3d.i:64-116 (and the 2-D
+ * twin) with the cell deltas of geom_m4cartopstuff.i:39-43 */
+void orc_cartlinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
+                            const int* ifirstf, const int* ilastf,
+                            const int* cilo, const int* cihi,
+                            const int* filo, const int* fihi,
+                            const int* ratio, const double* dxc,
+                            const double* dxf, const double* arrayc,
+                            double* arrayf);
 /* conrefcelldoub{2,3}d: pdat_m4conrefineops3d.i:33-... */
 void orc_conrefcelldoub(int dim, const int* ifirstf, const int* ilastf,
                         const int* cilo, const int* cihi, const int* filo,
@@ -247,6 +256,9 @@ void orc_fac_op_vcycle(orc_fac* s);
  * its parity is pinned by the converged FAC solution, not by a reference
  * routine).  mg_iterations / mg_relres report the last coarse solve. */
 void orc_fac_set_coarse_mg(orc_fac* s, int on);
+/* prolongation_method: 0 CONSERVATIVE_LINEAR_REFINE (default), 1
+ * CONSTANT_REFINE, 2 LINEAR_REFINE */
+void orc_fac_set_prolongation(orc_fac* s, int method);
 int orc_fac_mg_iterations(orc_fac* s);
 double orc_fac_mg_relres(orc_fac* s);
 void orc_fac_op_get_flux(orc_fac* s, int ln, int p, int axis, double* dst);

@@ -732,6 +732,63 @@ void orc_cartclinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
   free(slope);
 }
 
+void orc_cartlinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
+                            const int* ifirstf, const int* ilastf,
+                            const int* cilo, const int* cihi,
+                            const int* filo, const int* fihi,
+                            const int* ratio, const double* dxc,
+                            const double* dxf, const double* arrayc,
+                            double* arrayf) {
+  const double half = 0.5, one = 1.0;
+  (void)ifirstc; (void)ilastc;
+  int lf[3] = {0, 0, 0}, hf[3] = {0, 0, 0}, rr[3] = {1, 1, 1};
+  int gcl[3] = {0, 0, 0}, gch[3] = {0, 0, 0}, gfl[3] = {0, 0, 0}, gfh[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) {
+    lf[d] = ifirstf[d]; hf[d] = ilastf[d]; rr[d] = ratio[d];
+    gcl[d] = cilo[d]; gch[d] = cihi[d]; gfl[d] = filo[d]; gfh[d] = fihi[d];
+  }
+  V3 C = v3(arrayc, gcl[0], gch[0], gcl[1], gch[1], gcl[2], gch[2]);
+  V3 F = v3(arrayf, gfl[0], gfh[0], gfl[1], gfh[1], gfl[2], gfh[2]);
+  double deltax[16][3];
+  for (int d = 0; d < dim; ++d)
+    for (int r = 0; r < rr[d]; ++r) deltax[r][d] = ((double)r + half) * dxf[d] - dxc[d] * half;
+  for (int f2 = lf[2]; f2 <= hf[2]; ++f2) {
+    int c2 = 0;
+    double z = 0.0;
+    if (dim == 3) {
+      c2 = coarsen_index(f2, rr[2]);
+      z = deltax[f2 - c2 * rr[2]][2] / dxc[2];
+      if (z < 0.0) { c2 = c2 - 1; z = z + one; }
+    }
+    for (int f1 = lf[1]; f1 <= hf[1]; ++f1) {
+      int c1 = coarsen_index(f1, rr[1]);
+      double y = deltax[f1 - c1 * rr[1]][1] / dxc[1];
+      if (y < 0.0) { c1 = c1 - 1; y = y + one; }
+      for (int f0 = lf[0]; f0 <= hf[0]; ++f0) {
+        int c0 = coarsen_index(f0, rr[0]);
+        double x = deltax[f0 - c0 * rr[0]][0] / dxc[0];
+        if (x < 0.0) { c0 = c0 - 1; x = x + one; }
+        if (dim == 3)
+          AT(F, f0, f1, f2) =
+              ((AT(C, c0, c1, c2) + (AT(C, c0 + 1, c1, c2) - AT(C, c0, c1, c2)) * x) * (one - y) +
+               (AT(C, c0, c1 + 1, c2) + (AT(C, c0 + 1, c1 + 1, c2) - AT(C, c0, c1 + 1, c2)) * x) *
+                   y) *
+                  (one - z) +
+              ((AT(C, c0, c1, c2 + 1) + (AT(C, c0 + 1, c1, c2 + 1) - AT(C, c0, c1, c2 + 1)) * x) *
+                   (one - y) +
+               (AT(C, c0, c1 + 1, c2 + 1) +
+                (AT(C, c0 + 1, c1 + 1, c2 + 1) - AT(C, c0, c1 + 1, c2 + 1)) * x) *
+                   y) *
+                  z;
+        else
+          AT(F, f0, f1, 0) =
+              (AT(C, c0, c1, 0) + (AT(C, c0 + 1, c1, 0) - AT(C, c0, c1, 0)) * x) * (one - y) +
+              (AT(C, c0, c1 + 1, 0) + (AT(C, c0 + 1, c1 + 1, 0) - AT(C, c0, c1 + 1, 0)) * x) * y;
+      }
+    }
+  }
+}
+
 void orc_conrefcelldoub(int dim, const int* ifirstf, const int* ilastf,
                         const int* cilo, const int* cihi, const int* filo,
                         const int* fihi, const int* ratio, const double* arrayc,

@@ -80,6 +80,7 @@ def oracle_lib():
         lib.orc_fac_op_sum_cv.argtypes = [C.c_void_p]
         lib.orc_fac_op_get_flux.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
         lib.orc_fac_set_coarse_mg.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_fac_set_prolongation.argtypes = [C.c_void_p, C.c_int]
         lib.orc_fac_mg_iterations.argtypes = [C.c_void_p]
         lib.orc_fac_mg_iterations.restype = C.c_int
         lib.orc_fac_mg_relres.argtypes = [C.c_void_p]
@@ -158,6 +159,11 @@ class OracleFAC:
     def set_coarse_mg(self, on=True):
         """Coarsest level by geometric multigrid (cmax V-cycles, relative tol ctol)."""
         self.lib.orc_fac_set_coarse_mg(self.h, int(on))
+
+    def set_prolongation(self, method):
+        """prolongation_method name of CellPoissonFACOps."""
+        m = {"CONSERVATIVE_LINEAR_REFINE": 0, "CONSTANT_REFINE": 1, "LINEAR_REFINE": 2}[method]
+        self.lib.orc_fac_set_prolongation(self.h, m)
 
     def mg_stats(self):
         return self.lib.orc_fac_mg_iterations(self.h), self.lib.orc_fac_mg_relres(self.h)

@@ -54,4 +54,41 @@ __host__ __device__ inline double ewing_tranwt(int r) {
   return (double)(1.0f / (float)(1 + r));
 }
 
+// cartlinrefcelldoub{2,3}d (geom/fortran/geom_m4cartrefineops3d.i:64-116,
+// called from geom/CartesianCellDoubleLinearRefine.C:114-300): value of fine
+// cell f by multilinear interpolation of the coarse array c (view v), in the
+// Fortran expression order.
+template <int DIM>
+__host__ __device__ inline double linref_value(const double* c, const View& v, const int* f,
+                                               const int* r, const double* dxc,
+                                               const double* dxf) {
+  int ic[3] = {0, 0, 0};
+  double w[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < DIM; ++d) {
+    ic[d] = cdiv(f[d], r[d]);
+    const int ir = f[d] - ic[d] * r[d];
+    const double deltax = ((double)ir + 0.5) * dxf[d] - dxc[d] * 0.5;
+    w[d] = deltax / dxc[d];
+    if (w[d] < 0.0) {
+      ic[d] = ic[d] - 1;
+      w[d] = w[d] + 1.0;
+    }
+  }
+  const double x = w[0], y = w[1];
+  auto A = [&](int a, int b, int e) {
+    return c[v.at(ic[0] + a, ic[1] + b, DIM == 3 ? ic[2] + e : 0)];
+  };
+  if (DIM == 3) {
+    const double z = w[2];
+    return ((A(0, 0, 0) + (A(1, 0, 0) - A(0, 0, 0)) * x) * (1.0 - y) +
+            (A(0, 1, 0) + (A(1, 1, 0) - A(0, 1, 0)) * x) * y) *
+               (1.0 - z) +
+           ((A(0, 0, 1) + (A(1, 0, 1) - A(0, 0, 1)) * x) * (1.0 - y) +
+            (A(0, 1, 1) + (A(1, 1, 1) - A(0, 1, 1)) * x) * y) *
+               z;
+  }
+  return (A(0, 0, 0) + (A(1, 0, 0) - A(0, 0, 0)) * x) * (1.0 - y) +
+         (A(0, 1, 0) + (A(1, 1, 0) - A(0, 1, 0)) * x) * y;
+}
+
 }  // namespace sfac

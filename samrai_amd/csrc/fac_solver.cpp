@@ -114,7 +114,7 @@ void CellPoissonFACOps::setCoarseFineDiscretization(const std::string& c) {
 }
 
 void CellPoissonFACOps::setProlongationMethod(const std::string& m) {
-  if (m != "CONSERVATIVE_LINEAR_REFINE" && m != "CONSTANT_REFINE")
+  if (m != "CONSERVATIVE_LINEAR_REFINE" && m != "CONSTANT_REFINE" && m != "LINEAR_REFINE")
     fail("CellPoissonFACOps: prolongation_method '" + m + "' not available");
   d_prolongation = m;
   ++d_generation;
@@ -884,7 +884,9 @@ void CellPoissonFACOps::prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAI
   ghostFill(*s.data, dest_ln - 1, false, true, false);
   sfac::launch_prolong_add(Tf.prolong_from_coarser.launch(), Tc.geom, Tf.geom,
                            s.data->table(dest_ln - 1), s.data->ghost(), d.data->table(dest_ln),
-                           d.data->ghost(), d_prolongation == "CONSTANT_REFINE" ? 1 : 0,
+                           d.data->ghost(), d_prolongation == "CONSTANT_REFINE" ? sfac::kProlongConstant
+                           : d_prolongation == "LINEAR_REFINE" ? sfac::kProlongLinear
+                                                               : sfac::kProlongConservative,
                            d_stream);
 }
 

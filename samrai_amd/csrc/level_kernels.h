@@ -74,11 +74,12 @@ void launch_restrict(const GhostLaunch& w, const LevelGeom& coarse, const LevelG
                      double* const* dst, int dg, const double* const* src, int sg,
                      hipStream_t s);
 
-// Conservative linear refine of the coarse error and e_f += it
-// (items: p fine, q coarse, fine box).
+// Refine the coarse error and e_f += it (items: p fine, q coarse, fine box),
+// by the prolongation method below.
+enum ProlongMethod { kProlongConservative = 0, kProlongConstant = 1, kProlongLinear = 2 };
 void launch_prolong_add(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
                         const double* const* ce, int cg, double* const* fe, int fg,
-                        int constant, hipStream_t s);
+                        int method, hipStream_t s);
 
 // Partial sums of w*x*y over interior chunks (one double per chunk);
 // x == nullptr sums w alone.

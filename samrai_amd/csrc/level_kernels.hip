@@ -554,7 +554,7 @@ __device__ inline double clin_slope(double dm, double dp, double dxc) {
 template <int DIM>
 __global__ void __launch_bounds__(kThreads)
 k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
-              const double* const* ce, int cg, double* const* fe, int fg, int constant) {
+              const double* const* ce, int cg, double* const* fe, int fg, int method) {
   const Chunk ch = chunks[blockIdx.x];
   const BoxItem it = items[ch.item];
   const View vc = vview(C.box[it.q], cg, DIM), vf = vview(Fl.box[it.p], fg, DIM);
@@ -567,9 +567,16 @@ k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom 
     int c0 = cdiv(f[0], r), c1 = cdiv(f[1], r), c2 = DIM == 3 ? cdiv(f[2], r) : 0;
     int i0 = f[0] - c0 * r, i1 = f[1] - c1 * r, i2 = DIM == 3 ? f[2] - c2 * r : 0;
     double uc = uc_[vc.at(c0, c1, c2)];
-    if (constant) {  // CONSTANT_REFINE (pdat_m4conrefineops3d.i)
+    if (method == kProlongConstant) {  // CONSTANT_REFINE (pdat_m4conrefineops3d.i)
       long o = vf.at(f[0], f[1], f[2]);
       uf[o] = uf[o] + uc;
+      continue;
+    }
+    if (method == kProlongLinear) {  // LINEAR_REFINE (geom_m4cartrefineops3d.i:64-116)
+      const int rr[3] = {r, r, r};
+      const double v = linref_value<DIM>(uc_, vc, f, rr, C.dx, Fl.dx);
+      long o = vf.at(f[0], f[1], f[2]);
+      uf[o] = uf[o] + v;
       continue;
     }
     double s0 = clin_slope(uc - uc_[vc.at(c0 - 1, c1, c2)], uc_[vc.at(c0 + 1, c1, c2)] - uc,
@@ -839,14 +846,14 @@ void launch_restrict(const GhostLaunch& w, const LevelGeom& coarse, const LevelG
 
 void launch_prolong_add(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
                         const double* const* ce, int cg, double* const* fe, int fg,
-                        int constant, hipStream_t s) {
+                        int method, hipStream_t s) {
   if (w.nchunks == 0) return;
   if (coarse.dim == 3)
     hipLaunchKernelGGL(k_prolong_add<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
-                       w.chunks, coarse, fine, ce, cg, fe, fg, constant);
+                       w.chunks, coarse, fine, ce, cg, fe, fg, method);
   else
     hipLaunchKernelGGL(k_prolong_add<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
-                       w.chunks, coarse, fine, ce, cg, fe, fg, constant);
+                       w.chunks, coarse, fine, ce, cg, fe, fg, method);
   check_launch();
 }
 

@@ -410,6 +410,18 @@ __global__ void k_conref(const double* Cc, View vc, double* F, View vf, Rng3 rf,
   F[vf.at(f[0], f[1], f[2])] = Cc[vc.at(c0, c1, c2)];
 }
 
+template <int DIM>
+__global__ void k_linref(const double* Cc, View vc, double* F, View vf, Rng3 rf, int r0, int r1,
+                         int r2, double dc0, double dc1, double dc2, double df0, double df1,
+                         double df2) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int f[3];
+  if (!rf.decode(t, f)) return;
+  const int r[3] = {r0, r1, r2};
+  const double dxc[3] = {dc0, dc1, dc2}, dxf[3] = {df0, df1, df2};
+  F[vf.at(f[0], f[1], f[2])] = linref_value<DIM>(Cc, vc, f, r, dxc, dxf);
+}
+
 // Outerside average: arrays are slabs over the two tangential axes
 // (in = fastest).  3-D: (in,out) = (1,2), (0,2), (0,1) for axis 0,1,2.
 __global__ void k_outside(const double* F, int fin_lo, int fout_lo, int fin_n,
@@ -665,6 +677,27 @@ int sfac_cartclinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
     hipLaunchKernelGGL(k_clinref<2>, dim3(nblocks(rf.size())), dim3(kBlock), 0, stream, arrayc,
                        vc, arrayf, vf, rf, ratio[0], ratio[1], 1, dxc[0], dxc[1], 0.0, dxf[0],
                        dxf[1], 0.0);
+  return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int sfac_cartlinrefcelldoub(int dim, const int* ifirstc, const int* ilastc,
+                            const int* ifirstf, const int* ilastf, const int* cilo,
+                            const int* cihi, const int* filo, const int* fihi,
+                            const int* ratio, const double* dxc, const double* dxf,
+                            const double* arrayc, double* arrayf, hipStream_t stream) {
+  (void)ifirstc; (void)ilastc;
+  View vc = box_view(cilo, cihi, dim), vf = box_view(filo, fihi, dim);
+  Rng3 rf = rng3(ifirstf[0], ilastf[0], ifirstf[1], ilastf[1], dim == 3 ? ifirstf[2] : 0,
+                 dim == 3 ? ilastf[2] : 0);
+  if (rf.size() == 0) return 0;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_linref<3>, dim3(nblocks(rf.size())), dim3(kBlock), 0, stream, arrayc,
+                       vc, arrayf, vf, rf, ratio[0], ratio[1], ratio[2], dxc[0], dxc[1], dxc[2],
+                       dxf[0], dxf[1], dxf[2]);
+  else
+    hipLaunchKernelGGL(k_linref<2>, dim3(nblocks(rf.size())), dim3(kBlock), 0, stream, arrayc,
+                       vc, arrayf, vf, rf, ratio[0], ratio[1], 1, dxc[0], dxc[1], 1.0, dxf[0],
+                       dxf[1], 1.0);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

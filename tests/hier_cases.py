@@ -164,6 +164,8 @@ class Pair:
                      coarse_tol=params.get("coarse_solver_tolerance", 1e-8), ewing=True)
         if params.get("coarse_solver_choice") in ("mg", "hypre"):
             o.set_coarse_mg(True)
+        # the GPU solver's default prolongation is the reference's (CONSTANT_REFINE)
+        o.set_prolongation(params.get("prolongation_method", "CONSTANT_REFINE"))
         for ln, boxes in enumerate(case["levels"]):
             for p in range(len(boxes)):
                 o.put("u", ln, p, data["u"][ln][p])

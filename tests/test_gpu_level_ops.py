@@ -21,11 +21,11 @@ PARAMS = dict(max_cycles=4, num_pre_sweeps=2, num_post_sweeps=2,
               coarse_solver_max_iterations=20, coarse_solver_tolerance=1e-12)
 
 
-def _pair(mk, spec, seed=1, bc=None):
+def _pair(mk, spec, seed=1, bc=None, params=None):
     case = mk()
     data = hc.make_data(case, seed=seed)
     kw = dict(spec)
-    return hc.Pair(case, PARAMS, data, bc=bc, **kw)
+    return hc.Pair(case, params or PARAMS, data, bc=bc, **kw)
 
 
 @pytest.mark.parametrize("mk", hc.ALL, ids=lambda f: f.__name__)
@@ -92,10 +92,13 @@ def test_composite_residual_and_restriction(gpu_lib, mk, spec):
     assert abs(g - o) <= 1e-13 * abs(o)
 
 
+@pytest.mark.parametrize("method", ["CONSERVATIVE_LINEAR_REFINE", "LINEAR_REFINE",
+                                    "CONSTANT_REFINE"])
 @pytest.mark.parametrize("mk", hc.ALL, ids=lambda f: f.__name__)
-def test_restrict_solution_and_prolong(gpu_lib, mk):
+def test_restrict_solution_and_prolong(gpu_lib, mk, method):
     case = mk()
-    pr = _pair(mk, SPECS[0], bc=hc.bc_robin(case["dim"]))
+    pr = _pair(mk, SPECS[0], bc=hc.bc_robin(case["dim"]),
+               params=dict(PARAMS, prolongation_method=method))
     lmax = len(case["levels"]) - 1
     for ln in range(lmax - 1, -1, -1):
         print("restrict solution", case["name"], ln, flush=True)
@@ -103,8 +106,7 @@ def test_restrict_solution_and_prolong(gpu_lib, mk):
         pr.o.restrict_solution("u", ln)
         pr.compare(pr.u, "u", levels=[ln])
     for ln in range(1, lmax + 1):
-        for method in ("CONSERVATIVE_LINEAR_REFINE",):
-            print("prolong", case["name"], ln, method, flush=True)
-            pr.s.prolong_and_correct(pr.u, ln)
-            pr.o.prolong("u", ln)
-            pr.compare(pr.u, "u", levels=[ln - 1, ln])
+        print("prolong", case["name"], ln, method, flush=True)
+        pr.s.prolong_and_correct(pr.u, ln)
+        pr.o.prolong("u", ln)
+        pr.compare(pr.u, "u", levels=[ln - 1, ln])

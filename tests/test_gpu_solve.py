@@ -158,3 +158,20 @@ def test_c2_like_deck_matches_oracle(gpu_lib, choice):
     conv_o, it_o, res_o, hist_o = pr.o.solve()
     np.testing.assert_allclose(pr.s.history(), hist_o, rtol=1e-10, atol=0)
     pr.compare(pr.u, "u", interior_only=True, rtol=1e-10)
+
+
+@pytest.mark.parametrize("method", ["LINEAR_REFINE", "CONSTANT_REFINE"])
+@pytest.mark.parametrize("mk", [hc.case_2d, hc.case_3d], ids=lambda f: f.__name__)
+def test_solve_with_other_prolongations_matches_oracle(gpu_lib, mk, method):
+    """prolongation_method LINEAR_REFINE and CONSTANT_REFINE (the reference's
+    default) through whole solves, cycle by cycle against the oracle."""
+    case = mk()
+    data = smooth_data(case, 3)
+    pr = hc.Pair(case, dict(PARAMS, prolongation_method=method), data)
+    print("solve", case["name"], method, flush=True)
+    conv_g = pr.s.solve_system(pr.u, pr.f)
+    conv_o, it_o, res_o, hist_o = pr.o.solve()
+    assert conv_g == conv_o
+    assert pr.s.number_of_iterations() == it_o
+    np.testing.assert_allclose(pr.s.history(), hist_o, rtol=1e-12, atol=0)
+    pr.compare(pr.u, "u", interior_only=True, rtol=1e-12)

@@ -82,6 +82,15 @@ def clinref(run, dim, cbox, fbox, cg, fg, ratio, dxc, dxf, uc):
                [np.ascontiguousarray(uc, np.float64), uf], 1)
 
 
+def linref(run, dim, cbox, fbox, cg, fg, ratio, dxc, dxf, uc):
+    (clo, chi), (flo, fhi) = cbox, fbox
+    uf = np.full(shape(*fg), np.nan)
+    return run("cartlinrefcelldoub",
+               [C.c_int(dim), iv(clo), iv(chi), iv(flo), iv(fhi), iv(cg[0]), iv(cg[1]),
+                iv(fg[0]), iv(fg[1]), iv(ratio), dv(dxc), dv(dxf)],
+               [np.ascontiguousarray(uc, np.float64), uf], 1)
+
+
 def wgtavg(run, dim, cbox, fg, cg, ratio, dxf, dxc, uf):
     clo, chi = cbox
     uc = np.full(shape(*cg), np.nan)
@@ -181,6 +190,40 @@ def test_clinref_limiter_hand_values():
     for k in range(2):
         for j in range(2):
             np.testing.assert_array_equal(uf[k, j], want)
+
+
+@pytest.mark.parametrize("dim,ratio", [(3, (2, 2, 2)), (3, (2, 3, 4)), (2, (2, 2)), (2, (4, 3))])
+def test_linear_refine_reproduces_linear_data(dim, ratio):
+    """CartesianCellDoubleLinearRefine: multilinear interpolation of the
+    coarse cell centres is exact for linear data (with one ghost cell of
+    coarse data for the fine cells nearest the coarse box edges)."""
+    run = Runner()
+    clo, chi = (-3, 1, 2)[:dim], (2, 4, 5)[:dim]
+    dxc = (0.2, 0.125, 0.25)[:dim]
+    dxf = tuple(dxc[d] / ratio[d] for d in range(dim))
+    cg = (tuple(l - 1 for l in clo), tuple(h + 1 for h in chi))
+    flo = tuple(clo[d] * ratio[d] for d in range(dim))
+    fhi = tuple((chi[d] + 1) * ratio[d] - 1 for d in range(dim))
+    uc = linear(cg[0], cg[1], dxc)
+    uf = linref(run, dim, (clo, chi), (flo, fhi), cg, (flo, fhi), ratio, dxc, dxf, uc)
+    np.testing.assert_allclose(uf, linear(flo, fhi, dxf), rtol=1e-13, atol=0)
+
+
+def test_linear_refine_hand_values():
+    """1-D profile 0, 1, 3, 10 along x (dx = 1, ratio 2): the children of
+    coarse cell 1 sit at x = -1/4, +1/4 from its centre, so they interpolate
+    between cells (0, 1) at 3/4 and (1, 2) at 1/4:
+      cell 1: 0 + 1 * 0.75 = 0.75,  1 + 2 * 0.25 = 1.5
+      cell 2: 1 + 2 * 0.75 = 2.5,   3 + 7 * 0.25 = 4.75"""
+    run = Runner()
+    prof = np.array([0.0, 1.0, 3.0, 10.0])
+    cg = ((0, 0, 0), (3, 2, 2))
+    uc = np.broadcast_to(prof, shape(*cg)).copy()
+    uf = linref(run, 3, ((1, 1, 1), (2, 1, 1)), ((2, 2, 2), (5, 3, 3)), cg,
+                ((2, 2, 2), (5, 3, 3)), (2, 2, 2), (1.0,) * 3, (0.5,) * 3, uc)
+    for k in range(2):
+        for j in range(2):
+            np.testing.assert_array_equal(uf[k, j], [0.75, 1.5, 2.5, 4.75])
 
 
 @pytest.mark.parametrize("dim,ratio", [(3, (2, 2, 2)), (3, (3, 2, 4)), (2, (2, 2)), (2, (2, 3))])
@@ -289,6 +332,7 @@ def test_device_transfers_match_oracle(gpu_lib, torch_gpu, dim, ratio, clo, chi)
     uf = rng.standard_normal(shape(*fg))
     for run_args in [
         lambda r: clinref(r, dim, (clo, chi), (flo, fhi), cg, fg, ratio, dxc, dxf, uc),
+        lambda r: linref(r, dim, (clo, chi), (flo, fhi), cg, fg, ratio, dxc, dxf, uc),
         lambda r: wgtavg(r, dim, (clo, chi), fg, cg, ratio, dxf, dxc, uf),
         lambda r: conref(r, dim, fg, cg, fg, ratio, uc),
     ]:
