@@ -62,15 +62,20 @@ struct RobinDev {
   double a[6], b[6], g[6];
 };
 
+// Cell t (i fastest) of the item's box.  A patch box holds far fewer than
+// 2^32 cells, so the decomposition uses 32-bit unsigned division (the 64-bit
+// form costs several times the instructions on the GPU).
 __host__ __device__ inline bool decode_box(const BoxItem& it, long t, int* c) {
-  int n0 = it.hi[0] - it.lo[0] + 1, n1 = it.hi[1] - it.lo[1] + 1;
-  int n2 = it.hi[2] - it.lo[2] + 1;
-  long n01 = (long)n0 * n1;
-  if (t >= n01 * n2) return false;
-  c[2] = it.lo[2] + (int)(t / n01);
-  long r = t % n01;
-  c[1] = it.lo[1] + (int)(r / n0);
-  c[0] = it.lo[0] + (int)(r % n0);
+  const unsigned n0 = it.hi[0] - it.lo[0] + 1, n1 = it.hi[1] - it.lo[1] + 1;
+  const unsigned n2 = it.hi[2] - it.lo[2] + 1;
+  const unsigned n01 = n0 * n1;
+  if (t >= (long)n01 * n2) return false;
+  const unsigned u = (unsigned)t;
+  const unsigned k = u / n01, r = u - k * n01;
+  const unsigned j = r / n0;
+  c[2] = it.lo[2] + (int)k;
+  c[1] = it.lo[1] + (int)j;
+  c[0] = it.lo[0] + (int)(r - j * n0);
   return true;
 }
 
