@@ -24,16 +24,27 @@ __device__ inline void tangential(int a, int dim, int& t1, int& t2) {
   if (a == 0) { t1 = 1; t2 = 2; } else if (a == 1) { t1 = 0; t2 = 2; } else { t1 = 0; t2 = 1; }
 }
 
-// Is the ghost cell across face `face` (2*axis+side) of patch p, next to
-// interior cell c, a coarse-fine ghost?
+// Element t of a 3-vector by selects (a run-time index into a local array
+// would put the array in scratch memory).
+__device__ inline int sel3(const int* v, int t) { return t == 0 ? v[0] : (t == 1 ? v[1] : v[2]); }
+
+// Position of interior cell c in the per-face tables of face `face`
+// (2*axis+side) of patch p (t1 fastest, then t2).
+__device__ inline long cf_pos(const LevelGeom& L, int face, const int* c, const Box& b) {
+  int t1, t2;
+  tangential(face >> 1, L.dim, t1, t2);
+  long idx = sel3(c, t1) - sel3(b.lo, t1);
+  if (L.dim == 3)
+    idx += (long)(sel3(b.hi, t1) - sel3(b.lo, t1) + 1) * (sel3(c, t2) - sel3(b.lo, t2));
+  return idx;
+}
+
+// Is the ghost cell across face `face` of patch p, next to interior cell c,
+// a coarse-fine ghost?
 __device__ inline bool cf_flag(const LevelGeom& L, int p, int face, const int* c, const Box& b) {
   long off = L.cf_off[p * 6 + face];
   if (off < 0) return false;
-  int t1, t2;
-  tangential(face >> 1, L.dim, t1, t2);
-  long idx = c[t1] - b.lo[t1];
-  if (L.dim == 3) idx += (long)(b.hi[t1] - b.lo[t1] + 1) * (c[t2] - b.lo[t2]);
-  return L.cf[off + idx] != 0;
+  return L.cf[off + cf_pos(L, face, c, b)] != 0;
 }
 
 // Index of interior cell c of face `face` in the per-face tables, or -1 when
@@ -41,11 +52,8 @@ __device__ inline bool cf_flag(const LevelGeom& L, int p, int face, const int* c
 __device__ inline long cf_index(const LevelGeom& L, int p, int face, const int* c, const Box& b) {
   long off = L.cf_off[p * 6 + face];
   if (off < 0) return -1;
-  int t1, t2;
-  tangential(face >> 1, L.dim, t1, t2);
-  long idx = c[t1] - b.lo[t1];
-  if (L.dim == 3) idx += (long)(b.hi[t1] - b.lo[t1] + 1) * (c[t2] - b.lo[t2]);
-  return L.cf[off + idx] ? off + idx : -1;
+  const long idx = off + cf_pos(L, face, c, b);
+  return L.cf[idx] ? idx : -1;
 }
 
 // Flux D grad u on the face of axis a with face index c (between c-e_a and
@@ -63,7 +71,10 @@ __device__ inline double face_flux(const LevelGeom& L, int p, const Box& b, cons
   if (ewing == 2) {
     int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
     if (side >= 0) {
-      long x = cf_index(L, p, 2 * a + side, side == 0 ? c : m, b);
+      int in[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) in[d] = side == 0 ? c[d] : m[d];
+      long x = cf_index(L, p, 2 * a + side, in, b);
       if (x >= 0) return L.cf_flux[x];
     }
   }
@@ -72,7 +83,11 @@ __device__ inline double face_flux(const LevelGeom& L, int p, const Box& b, cons
   if (ewing == 1) {
     int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
     if (side >= 0) {
-      const int* in = side == 0 ? c : m;
+      // the interior cell next to the face, by value (a pointer chosen at run
+      // time between two local arrays would force them into scratch memory)
+      int in[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) in[d] = side == 0 ? c[d] : m[d];
       if (cf_flag(L, p, 2 * a + side, in, b)) {
         int r = L.ratio;
         int q[3] = {in[0], in[1], in[2]};
@@ -355,8 +370,11 @@ k_flux(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* fl
   const int p = it.p;
   const Box b = L.box[p];
   const View vu = vview(b, ug, DIM);
+  // every per-axis array below is indexed by unrolled loop counters only, so
+  // it lives in registers (a runtime index would put it in scratch memory)
   View vd[3], vf[3];
   const double* Dp[3] = {nullptr, nullptr, nullptr};
+#pragma unroll
   for (int a = 0; a < DIM; ++a) {
     vf[a] = vview(b, 0, DIM, a);
     if (DVAR) {
@@ -371,6 +389,7 @@ k_flux(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* fl
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
     int c[3];
     if (!decode_box(it, (long)ch.start + t, c)) break;
+#pragma unroll
     for (int a = 0; a < DIM; ++a) {
       F[a][vf[a].at(c[0], c[1], c[2])] = face_flux<DIM, DVAR>(
           L, p, b, u, vu, Dp, vd, ps.dconst, a, c[0], c[1], c[2], di[a], ewing);
@@ -403,21 +422,31 @@ k_ewing_faces(const BoxItem* items, const Chunk* chunks, LevelGeom L, const doub
   View vd[3];
   const double* Dp[3] = {nullptr, nullptr, nullptr};
   if (DVAR) {
+#pragma unroll
     for (int d = 0; d < DIM; ++d) {
       vd[d] = vview(b, ps.dghost, DIM, d);
       Dp[d] = ps.D[d][p];
     }
   }
-  const double dai = 1.0 / L.dx[a];
+  const double dai = 1.0 / (a == 0 ? L.dx[0] : (a == 1 ? L.dx[1] : L.dx[2]));
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
     int c[3];
     if (!decode_box(it, (long)ch.start + t, c)) break;
     long x = cf_index(L, p, face, c, b);
     if (x < 0) continue;
-    int f[3] = {c[0], c[1], c[2]};
-    f[a] += side;
-    L.cf_flux[x] = face_flux<DIM, DVAR>(L, p, b, uu[p], vu, Dp, vd, ps.dconst, a, f[0], f[1],
-                                        f[2], dai, 1);
+    // one call per axis, each with a constant axis (a run-time axis index
+    // into the local arrays would put them in scratch memory)
+    double F;
+    if (a == 0)
+      F = face_flux<DIM, DVAR>(L, p, b, uu[p], vu, Dp, vd, ps.dconst, 0, c[0] + side, c[1],
+                               c[2], dai, 1);
+    else if (a == 1)
+      F = face_flux<DIM, DVAR>(L, p, b, uu[p], vu, Dp, vd, ps.dconst, 1, c[0], c[1] + side,
+                               c[2], dai, 1);
+    else
+      F = face_flux<DIM, DVAR>(L, p, b, uu[p], vu, Dp, vd, ps.dconst, 2, c[0], c[1],
+                               c[2] + side, dai, 1);
+    L.cf_flux[x] = F;
   }
 }
 
