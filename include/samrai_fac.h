@@ -206,7 +206,8 @@ typedef struct {
   const char* coarse_solver_choice;  /* "redblack" | "jacobi" | "mg" | "hypre" (= "mg") */
   double coarse_solver_tolerance;
   int coarse_solver_max_iterations;
-  const char* cf_discretization;     /* "Ewing" */
+  const char* cf_discretization;     /* "Ewing" | "CONSTANT_REFINE" | "LINEAR_REFINE" |
+                                        "CONSERVATIVE_LINEAR_REFINE" */
   const char* prolongation_method;   /* "CONSERVATIVE_LINEAR_REFINE" | "LINEAR_REFINE" |
                                         "CONSTANT_REFINE" */
   int enable_logging;
@@ -253,6 +254,12 @@ int sfac_solver_enable_kernel_timing(sfac_solver* s, int on);
 int sfac_solver_kernel_timing(sfac_solver* s, int level, long* launches, double* ms,
                               double* bytes);
 
+/* FACPreconditioner::setAlgorithmChoice (FACPreconditioner.h:316,
+ * FACPreconditioner.C:843-866): "default" (facCycle_Recursive),
+ * "mccormick-s4.3" (facCycle_McCormick, .C:530-668) or "pernice" (facCycle,
+ * .C:678-763), case-insensitive; any other name is an error. */
+int sfac_solver_set_algorithm_choice(sfac_solver* s, const char* choice);
+
 /* Execution choice with no reference counterpart (results are unchanged):
  * when the coarsest level of the solve tiles the physical domain exactly,
  * its red-black sweeps run colour-fused over one contiguous copy of the
@@ -279,6 +286,12 @@ int sfac_fused_tile_shape(int dim, int variant, int* shape3);
  * hypre path (CellPoissonFACOps::solveCoarsestLevel_HYPRE, .C:1740-1780).
  * coarse_mg_stats: V-cycles and relative residual of the last coarse solve. */
 int sfac_solver_coarse_mg_stats(sfac_solver* s, int* iterations, double* relres);
+
+/* FACPreconditioner::setAlgorithmChoice (FACPreconditioner.h:316,
+ * FACPreconditioner.C:843-866): "default" (facCycle_Recursive),
+ * "mccormick-s4.3" (facCycle_McCormick, .C:530-668) or "pernice" (facCycle,
+ * .C:678-763), case-insensitive; any other name is an error. */
+int sfac_solver_set_algorithm_choice(sfac_solver* s, const char* choice);
 
 /* Execution choice with no reference counterpart (results are unchanged):
  * on = 1 (default) captures one FACPreconditioner cycle (e = 0, V-cycle,

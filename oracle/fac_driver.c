@@ -29,7 +29,7 @@
 #include <string.h>
 
 #define MAXL 8
-enum { F_U = 0, F_F = 1, F_C = 2, F_D = 3, F_E = 4, F_R = 5, F_TR = 6, NFIELD = 7 };
+enum { F_U = 0, F_F = 1, F_C = 2, F_D = 3, F_E = 4, F_R = 5, F_TR = 6, F_TE = 7, NFIELD = 8 };
 
 typedef struct { int lo[3], hi[3]; } Box;
 
@@ -42,7 +42,7 @@ typedef struct {
   Box gb;       /* ghost-1 box */
   long n1;      /* cells in ghost-1 box */
   long n0;      /* interior cells */
-  double *u, *e, *scr;           /* ghost 1 */
+  double *u, *e, *te, *scr;      /* ghost 1 (te: McCormick's temporary error) */
   double *f, *r, *tr, *w, *C;    /* ghost 0 */
   double *D[3], *flux[3];        /* side, ghost 0 */
   double *oflux[3][2];           /* outerside slabs */
@@ -76,6 +76,9 @@ struct orc_fac {
   double last_seconds;
   int coarse_mg;                 /* coarsest level by multigrid (mg_solve) */
   int prolong_method;            /* 0 conservative linear, 1 constant, 2 linear */
+  int cf_method;                 /* 0 Ewing, 1 CONSTANT_REFINE, 2 LINEAR_REFINE,
+                                    3 CONSERVATIVE_LINEAR_REFINE */
+  int algorithm;                 /* 0 default, 1 mccormick-s4.3, 2 pernice */
   int mg_iterations;             /* V-cycles of the last coarse solve */
   double mg_relres;              /* its final relative residual */
 };
@@ -126,10 +129,11 @@ static double** field_ptr(Patch* P, int field, int axis) {
     case F_E: return &P->e;
     case F_R: return &P->r;
     case F_TR: return &P->tr;
+    case F_TE: return &P->te;
   }
   return NULL;
 }
-static int field_ghost(int field) { return (field == F_U || field == F_E) ? 1 : 0; }
+static int field_ghost(int field) { return (field == F_U || field == F_E || field == F_TE) ? 1 : 0; }
 
 /* Side-array index for axis a over the interior box (ghost 0). */
 static long sidx(const Patch* P, int a, int i, int j, int k) {
@@ -330,7 +334,8 @@ orc_fac* orc_fac_create(int dim, int nlevels, const int* npatches,
       P->gb = grow(P->b, 1, dim);
       P->n1 = boxsize(&P->gb, dim);
       P->n0 = boxsize(&P->b, dim);
-      P->u = dalloc(P->n1); P->e = dalloc(P->n1); P->scr = dalloc(P->n1);
+      P->u = dalloc(P->n1); P->e = dalloc(P->n1); P->te = dalloc(P->n1);
+      P->scr = dalloc(P->n1);
       P->f = dalloc(P->n0); P->r = dalloc(P->n0); P->tr = dalloc(P->n0);
       P->w = dalloc(P->n0); P->C = dalloc(P->n0);
       for (int a = 0; a < dim; ++a) {
@@ -382,7 +387,7 @@ void orc_fac_destroy(orc_fac* s) {
     Level* L = &s->L[ln];
     for (int p = 0; p < L->np; ++p) {
       Patch* P = &L->p[p];
-      free(P->u); free(P->e); free(P->scr); free(P->f); free(P->r); free(P->tr);
+      free(P->u); free(P->e); free(P->te); free(P->scr); free(P->f); free(P->r); free(P->tr);
       free(P->w); free(P->C);
       for (int a = 0; a < s->dim; ++a) {
         free(P->D[a]); free(P->flux[a]); free(P->oflux[a][0]); free(P->oflux[a][1]);
@@ -431,11 +436,29 @@ void orc_fac_get(orc_fac* s, int field, int ln, int p, int axis, double* dst) {
 
 static double* vec(Patch* P, int field) { return *field_ptr(P, field, 0); }
 
-/* xeqScheduleGhostFill(NoCoarse): copies, optional C/F constant refine,
- * then the physical Robin ghosts (homogeneous => zero g). */
+/* Cell index of offset `off` in ghost box g. */
+static void unidx(const Box* g, long off, int dim, int* c) {
+  long nx = g->hi[0] - g->lo[0] + 1, ny = g->hi[1] - g->lo[1] + 1;
+  c[0] = g->lo[0] + (int)(off % nx);
+  c[1] = g->lo[1] + (int)((off / nx) % ny);
+  c[2] = dim == 3 ? g->lo[2] + (int)(off / (nx * ny)) : 0;
+}
+
+/* xeqScheduleGhostFill(NoCoarse): copies, optional C/F ghosts from the
+ * coarser level, then the physical Robin ghosts (homogeneous => zero g).
+ * C/F ghosts take the ghost-fill refine operator of cf_discretization
+ * (CellPoissonFACOps.C:938-943): constant refinement for "Ewing" and
+ * CONSTANT_REFINE; for LINEAR_REFINE / CONSERVATIVE_LINEAR_REFINE the
+ * refine operator reads a one-cell coarse stencil, which RefineSchedule
+ * fills from the coarser level's interiors, its own C/F interpolation from
+ * the level below, and the physical boundary conditions: here the coarser
+ * level's ghosts are filled that way first (recursively), then each fine
+ * C/F ghost is refined from the coarse patch holding its parent. */
 static void ghost_fill(orc_fac* s, int field, int ln, int with_coarse, int homogeneous) {
   Level* L = &s->L[ln];
   const int dim = s->dim;
+  const int interp = with_coarse && ln > 0 && s->cf_method >= 2;
+  if (interp) ghost_fill(s, field, ln - 1, ln - 1 > 0, homogeneous);
 #pragma omp parallel for schedule(dynamic) num_threads(s->nthreads)
   for (int p = 0; p < L->np; ++p) {
     Patch* P = &L->p[p];
@@ -443,8 +466,25 @@ static void ghost_fill(orc_fac* s, int field, int ln, int with_coarse, int homog
     for (int t = 0; t < P->ncp; ++t) x[P->cp[t].dst] = vec(&L->p[P->cp[t].sp], field)[P->cp[t].so];
     if (with_coarse && ln > 0) {
       Level* C = &s->L[ln - 1];
-      for (int t = 0; t < P->ncf; ++t)
-        x[P->cf[t].dst] = vec(&C->p[P->cf[t].sp], field)[P->cf[t].so];
+      if (!interp) {
+        for (int t = 0; t < P->ncf; ++t)
+          x[P->cf[t].dst] = vec(&C->p[P->cf[t].sp], field)[P->cf[t].so];
+      } else {
+        int r = L->ratio;
+        int rr[3] = {r, r, r};
+        for (int t = 0; t < P->ncf; ++t) {
+          Patch* Q = &C->p[P->cf[t].sp];
+          int fc[3], cc[3] = {0, 0, 0};
+          unidx(&P->gb, P->cf[t].dst, dim, fc);
+          for (int d = 0; d < dim; ++d) cc[d] = cdiv(fc[d], r);
+          if (s->cf_method == 2)
+            orc_cartlinrefcelldoub(dim, cc, cc, fc, fc, Q->gb.lo, Q->gb.hi, P->gb.lo, P->gb.hi,
+                                   rr, C->dx, L->dx, vec(Q, field), x);
+          else
+            orc_cartclinrefcelldoub(dim, cc, cc, fc, fc, Q->gb.lo, Q->gb.hi, P->gb.lo,
+                                    P->gb.hi, rr, C->dx, L->dx, vec(Q, field), x);
+        }
+      }
     }
   }
 #pragma omp parallel for schedule(dynamic) num_threads(s->nthreads)
@@ -498,7 +538,7 @@ static void flux_on_patch(orc_fac* s, int ln, Patch* P, const double* w) {
       orc_compfluxcondc2d(P->flux[0], P->flux[1], 0, 0, s->dconst, w, 1, 1, b->lo[0],
                           b->hi[0], b->lo[1], b->hi[1], L->dx);
   }
-  if (s->ewing && ln > 0) {
+  if (s->ewing && s->cf_method == 0 && ln > 0) {
     int rr[3] = {L->ratio, L->ratio, L->ratio};
     for (int t = 0; t < P->ncfb; ++t) {
       const Box* B = &P->cfb[t];
@@ -1124,11 +1164,45 @@ void orc_fac_set_prolongation(orc_fac* s, int method) { s->prolong_method = meth
 int orc_fac_mg_iterations(orc_fac* s) { return s->mg_iterations; }
 double orc_fac_mg_relres(orc_fac* s) { return s->mg_relres; }
 
+void orc_fac_set_cf_discretization(orc_fac* s, int method) { s->cf_method = method; }
+void orc_fac_set_algorithm(orc_fac* s, int algorithm) { s->algorithm = algorithm; }
+
+/* solveCoarsestLevel (CellPoissonFACOps.C:1680-1780) on level 0. */
+static void coarse_solve(orc_fac* s) {
+  if (s->coarse_mg) mg_solve(s);
+  else smooth(s, F_E, F_R, 0, s->cmax, s->ctol, NULL);
+  ghost_fill(s, F_E, 0, 0, 1);
+}
+
+/* setToScalar(0) / add on one level, interior only
+ * (HierarchyCellDataOpsReal, interior_only = true). */
+static void level_zero(orc_fac* s, int field, int ln) {
+  for (int p = 0; p < s->L[ln].np; ++p) {
+    Patch* P = &s->L[ln].p[p];
+    double* x = vec(P, field);
+    for (int k = (s->dim == 3 ? P->b.lo[2] : 0); k <= (s->dim == 3 ? P->b.hi[2] : 0); ++k)
+      for (int j = P->b.lo[1]; j <= P->b.hi[1]; ++j)
+        for (int i = P->b.lo[0]; i <= P->b.hi[0]; ++i) x[gidx(&P->gb, i, j, k)] = 0.0;
+  }
+}
+static void level_add(orc_fac* s, int dst, int src, int ln) {
+  for (int p = 0; p < s->L[ln].np; ++p) {
+    Patch* P = &s->L[ln].p[p];
+    double* y = vec(P, dst);
+    const double* x = vec(P, src);
+    for (int k = (s->dim == 3 ? P->b.lo[2] : 0); k <= (s->dim == 3 ? P->b.hi[2] : 0); ++k)
+      for (int j = P->b.lo[1]; j <= P->b.hi[1]; ++j)
+        for (int i = P->b.lo[0]; i <= P->b.hi[0]; ++i) {
+          long t = gidx(&P->gb, i, j, k);
+          y[t] = y[t] + x[t];
+        }
+  }
+}
+
+/* facCycle_Recursive, FACPreconditioner.C:461-518 */
 static void fac_cycle(orc_fac* s, int ln) {
   if (ln == 0) {
-    if (s->coarse_mg) mg_solve(s);
-    else smooth(s, F_E, F_R, ln, s->cmax, s->ctol, NULL);
-    ghost_fill(s, F_E, ln, 0, 1);
+    coarse_solve(s);
     return;
   }
   smooth(s, F_E, F_R, ln, s->pre, -1.0, NULL);
@@ -1139,11 +1213,62 @@ static void fac_cycle(orc_fac* s, int ln) {
   smooth(s, F_E, F_R, ln, s->post, -1.0, NULL);
 }
 
+/* facCycle_McCormick, FACPreconditioner.C:530-668: smoothing acts on a
+ * temporary error against the current composite residual and is added to
+ * e; the residual restricted in step 3a goes to the temporary residual
+ * (later overwritten on the coarser level), so the coarser levels keep the
+ * residual of the cycle's start. */
+static void fac_cycle_mccormick(orc_fac* s, int ln) {
+  if (ln == 0) {
+    coarse_solve(s);
+    return;
+  }
+  composite_residual(s, F_TR, F_E, F_R, ln, 1);      /* 2a */
+  level_zero(s, F_TE, ln);
+  smooth(s, F_TE, F_TR, ln, s->pre, -1.0, NULL);     /* 2b */
+  level_add(s, F_E, F_TE, ln);                       /* 2c */
+  composite_residual(s, F_TR, F_E, F_R, ln, 1);      /* 3a */
+  restrict_cells(s, F_TR, F_TR, ln - 1);
+  level_zero(s, F_E, ln - 1);                        /* 3b */
+  fac_cycle_mccormick(s, ln - 1);                    /* 3c */
+  prolong(s, F_E, ln);                               /* 3d */
+  composite_residual(s, F_TR, F_E, F_R, ln, 1);      /* 4a */
+  level_zero(s, F_TE, ln);
+  smooth(s, F_TE, F_TR, ln, s->post, -1.0, NULL);    /* 4b */
+  level_add(s, F_E, F_TE, ln);                       /* 4c */
+}
+
+/* facCycle ("pernice"), FACPreconditioner.C:678-763: the V-cycle as a
+ * descending then an ascending loop. */
+static void fac_cycle_pernice(orc_fac* s) {
+  int lmax = s->nl - 1;
+  for (int ln = lmax; ln > 0; --ln) {
+    smooth(s, F_E, F_R, ln, s->pre, -1.0, NULL);
+    composite_residual(s, F_TR, F_E, F_R, ln, 1);
+    restrict_cells(s, F_R, F_TR, ln - 1);
+  }
+  for (int ln = 0; ln <= lmax; ++ln) {
+    if (ln == 0) {
+      coarse_solve(s);
+    } else {
+      prolong(s, F_E, ln);
+      smooth(s, F_E, F_R, ln, s->post, -1.0, NULL);
+    }
+  }
+}
+
+static void run_cycle(orc_fac* s) {
+  if (s->algorithm == 1) fac_cycle_mccormick(s, s->nl - 1);
+  else if (s->algorithm == 2) fac_cycle_pernice(s);
+  else fac_cycle(s, s->nl - 1);
+}
+
 int orc_fac_solve(orc_fac* s, int* iterations, double* residual_norm, double* hist,
                   int max_hist) {
   double t0 = omp_get_wtime();
   int lmax = s->nl - 1;
   set_zero(s, F_E, 1);
+  set_zero(s, F_TE, 1);
   set_zero(s, F_R, 0);
   double init = full_composite_residual(s);
   double res = init;
@@ -1155,7 +1280,7 @@ int orc_fac_solve(orc_fac* s, int* iterations, double* residual_norm, double* hi
   int it = 0;
   do {
     set_zero(s, F_E, 1);
-    fac_cycle(s, lmax);
+    run_cycle(s);
     for (int ln = 0; ln <= lmax; ++ln)
       for (int p = 0; p < s->L[ln].np; ++p) {
         Patch* P = &s->L[ln].p[p];
@@ -1243,7 +1368,7 @@ double orc_fac_op_sum_cv(orc_fac* s) {
 }
 void orc_fac_op_vcycle(orc_fac* s) {
   set_zero(s, F_E, 1);
-  fac_cycle(s, s->nl - 1);
+  run_cycle(s);
 }
 void orc_fac_op_get_flux(orc_fac* s, int ln, int p, int axis, double* dst) {
   Patch* P = &s->L[ln].p[p];

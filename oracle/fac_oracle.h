@@ -236,7 +236,8 @@ double orc_fac_rms_norm_f(orc_fac* s);
 /* seconds spent in the last solve, cycles */
 double orc_fac_last_solve_seconds(orc_fac* s);
 
-/* Operator-level entry points (fields: 0 u, 1 f, 2 C, 3 D, 4 e, 5 r, 6 tmp_r):
+/* Operator-level entry points (fields: 0 u, 1 f, 2 C, 3 D, 4 e, 5 r, 6 tmp_r,
+ * 7 tmp_e):
  * the FACOperatorStrategy operations on one level, for level parity tests. */
 void orc_fac_op_ghost_fill(orc_fac* s, int field, int ln, int with_coarse,
                            int homogeneous);
@@ -259,6 +260,13 @@ void orc_fac_set_coarse_mg(orc_fac* s, int on);
 /* prolongation_method: 0 CONSERVATIVE_LINEAR_REFINE (default), 1
  * CONSTANT_REFINE, 2 LINEAR_REFINE */
 void orc_fac_set_prolongation(orc_fac* s, int method);
+/* cf_discretization: 0 "Ewing" (constant-refined C/F ghosts and the Ewing
+ * flux fix), 1 CONSTANT_REFINE, 2 LINEAR_REFINE, 3
+ * CONSERVATIVE_LINEAR_REFINE (C/F ghosts by that refine operator, no fix). */
+void orc_fac_set_cf_discretization(orc_fac* s, int method);
+/* FACPreconditioner algorithm_choice: 0 "default" (facCycle_Recursive), 1
+ * "mccormick-s4.3", 2 "pernice". */
+void orc_fac_set_algorithm(orc_fac* s, int algorithm);
 int orc_fac_mg_iterations(orc_fac* s);
 double orc_fac_mg_relres(orc_fac* s);
 void orc_fac_op_get_flux(orc_fac* s, int ln, int p, int axis, double* dst);

@@ -81,6 +81,8 @@ def oracle_lib():
         lib.orc_fac_op_get_flux.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp]
         lib.orc_fac_set_coarse_mg.argtypes = [C.c_void_p, C.c_int]
         lib.orc_fac_set_prolongation.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_fac_set_cf_discretization.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_fac_set_algorithm.argtypes = [C.c_void_p, C.c_int]
         lib.orc_fac_mg_iterations.argtypes = [C.c_void_p]
         lib.orc_fac_mg_iterations.restype = C.c_int
         lib.orc_fac_mg_relres.argtypes = [C.c_void_p]
@@ -100,7 +102,7 @@ def ref_lib():
     return _ref
 
 
-FIELDS = {"u": 0, "f": 1, "C": 2, "D": 3, "e": 4, "r": 5, "tr": 6}
+FIELDS = {"u": 0, "f": 1, "C": 2, "D": 3, "e": 4, "r": 5, "tr": 6, "te": 7}
 
 
 class OracleFAC:
@@ -164,6 +166,17 @@ class OracleFAC:
         """prolongation_method name of CellPoissonFACOps."""
         m = {"CONSERVATIVE_LINEAR_REFINE": 0, "CONSTANT_REFINE": 1, "LINEAR_REFINE": 2}[method]
         self.lib.orc_fac_set_prolongation(self.h, m)
+
+    def set_cf_discretization(self, name):
+        """cf_discretization of CellPoissonFACOps."""
+        m = {"Ewing": 0, "CONSTANT_REFINE": 1, "LINEAR_REFINE": 2,
+             "CONSERVATIVE_LINEAR_REFINE": 3}[name]
+        self.lib.orc_fac_set_cf_discretization(self.h, m)
+
+    def set_algorithm(self, name):
+        """algorithm_choice of FACPreconditioner."""
+        a = {"default": 0, "mccormick-s4.3": 1, "pernice": 2}[name]
+        self.lib.orc_fac_set_algorithm(self.h, a)
 
     def mg_stats(self):
         return self.lib.orc_fac_mg_iterations(self.h), self.lib.orc_fac_mg_relres(self.h)

@@ -392,6 +392,14 @@ int sfac_solver_fused_tile_timings(sfac_solver* s, double* ms_per_sweep, int max
   });
 }
 
+int sfac_solver_set_algorithm_choice(sfac_solver* s, const char* choice) {
+  return guarded([&] {
+    need(s, "solver");
+    need(choice, "choice");
+    s->s->preconditioner().setAlgorithmChoice(choice);
+  });
+}
+
 int sfac_solver_coarse_mg_stats(sfac_solver* s, int* iterations, double* relres) {
   return guarded([&] {
     need(s, "solver");

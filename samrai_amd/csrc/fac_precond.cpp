@@ -1,9 +1,10 @@
 // FACPreconditioner and CellPoissonFACSolver (host control flow).
 //
-// FACPreconditioner::solveSystem / facCycle_Recursive /
-// computeFullCompositeResidual restate solv/FACPreconditioner.C:276-518 and
-// 774-814; CellPoissonFACSolver restates the set-up and solve wrappers of
-// solv/CellPoissonFACSolver.C:175-400.
+// FACPreconditioner::solveSystem / facCycle_Recursive / facCycle_McCormick /
+// facCycle / computeFullCompositeResidual restate
+// solv/FACPreconditioner.C:276-814; CellPoissonFACSolver restates the set-up
+// and solve wrappers of solv/CellPoissonFACSolver.C:175-400.
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -37,8 +38,24 @@ void FACPreconditioner::setRelativeResidualTolerance(double t) {
   if (!(t > 0.0 || t == -1)) fail("FACPreconditioner: relative_residual_tol out of range");
   d_rel_tol = t;
 }
+// FACPreconditioner.C:843-866: case-insensitive "default" (recursive),
+// "mccormick-s4.3" or "pernice".
 void FACPreconditioner::setAlgorithmChoice(const std::string& c) {
-  if (c != "default") fail("FACPreconditioner: algorithm '" + c + "' not available");
+  std::string lower = c;
+  for (char& ch : lower) ch = (char)std::tolower((unsigned char)ch);
+  if (lower != "default" && lower != "mccormick-s4.3" && lower != "pernice")
+    fail("FACPreconditioner: algorithm should be one of 'default', 'mccormick-s4.3' or "
+         "'pernice'");
+  d_algorithm = lower;
+  releaseCycleGraph();
+  if (d_initialized && d_algorithm == "mccormick-s4.3" && !d_te_data) allocateTmpError();
+}
+
+void FACPreconditioner::allocateTmpError() {
+  const hier::PatchHierarchy& h = d_e_data->hierarchy();
+  d_te_data.reset(new PatchDataSet(h, PatchDataSet::kCell, d_e_data->ghost()));
+  d_te = {d_te_data.get(), d_e.weights, d_lmin, d_lmax};
+  d_gpu->setToScalar(*d_te.data, 0.0, d_lmin, d_lmax);
 }
 
 void FACPreconditioner::getConvergenceFactors(double& avg, double& fin) const {
@@ -61,6 +78,7 @@ void FACPreconditioner::initializeSolverState(const SAMRAIVectorReal& u,
   d_e = {d_e_data.get(), u.weights, d_lmin, d_lmax};
   d_r = {d_r_data.get(), f.weights, d_lmin, d_lmax};
   d_tr = {d_tr_data.get(), f.weights, d_lmin, d_lmax};
+  if (d_algorithm == "mccormick-s4.3") allocateTmpError();
   d_ops.initializeOperatorState(u, f);
   d_initialized = true;
 }
@@ -71,6 +89,7 @@ void FACPreconditioner::deallocateSolverState() {
   d_e_data.reset();
   d_r_data.reset();
   d_tr_data.reset();
+  d_te_data.reset();
   d_initialized = false;
 }
 
@@ -99,6 +118,60 @@ void FACPreconditioner::facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal
   d_ops.smoothError(e, r, ln, d_post);
 }
 
+// FACPreconditioner.C:530-668.  Smoothing acts on a temporary error
+// against the current composite residual and is added to e; the residual
+// restricted in step 3a lands in the temporary residual (overwritten on the
+// coarser level before use), so coarser levels keep the cycle's initial
+// residual, as in the reference.
+void FACPreconditioner::facCycle_McCormick(SAMRAIVectorReal& e, SAMRAIVectorReal& r,
+                                           SAMRAIVectorReal& u, int lmax, int lmin, int ln) {
+  if (ln == lmin) {
+    d_ops.solveCoarsestLevel(e, r, ln);
+    return;
+  }
+  d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);  // 2a
+  d_gpu->setToScalarInterior(*d_te.data, 0.0, ln);
+  d_ops.smoothError(d_te, d_tr, ln, d_pre);                     // 2b
+  d_gpu->add(*e.data, *d_te.data, ln, ln);                      // 2c
+  d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);  // 3a
+  d_ops.restrictResidual(d_tr, d_tr, ln - 1);
+  d_gpu->setToScalarInterior(*e.data, 0.0, ln - 1);             // 3b
+  facCycle_McCormick(e, r, u, lmax, lmin, ln - 1);              // 3c
+  d_ops.prolongErrorAndCorrect(e, e, ln);                       // 3d
+  d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);  // 4a
+  d_gpu->setToScalarInterior(*d_te.data, 0.0, ln);
+  d_ops.smoothError(d_te, d_tr, ln, d_post);                    // 4b
+  d_gpu->add(*e.data, *d_te.data, ln, ln);                      // 4c
+}
+
+// FACPreconditioner.C:678-763 ("pernice"): the V-cycle as a descending and
+// an ascending loop.
+void FACPreconditioner::facCycle(SAMRAIVectorReal& e, SAMRAIVectorReal& r, SAMRAIVectorReal&,
+                                 int lmax, int lmin) {
+  for (int ln = lmax; ln > lmin; --ln) {
+    d_ops.smoothError(e, r, ln, d_pre);
+    d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);
+    d_ops.restrictResidual(d_tr, r, ln - 1);
+  }
+  for (int ln = lmin; ln <= lmax; ++ln) {
+    if (ln == lmin) {
+      d_ops.solveCoarsestLevel(e, r, ln);
+    } else {
+      d_ops.prolongErrorAndCorrect(e, e, ln);
+      d_ops.smoothError(e, r, ln, d_post);
+    }
+  }
+}
+
+void FACPreconditioner::runCycle(SAMRAIVectorReal& u) {
+  if (d_algorithm == "mccormick-s4.3")
+    facCycle_McCormick(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
+  else if (d_algorithm == "pernice")
+    facCycle(d_e, d_r, u, d_lmax, d_lmin);
+  else
+    facCycle_Recursive(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
+}
+
 // One pass of the solveSystem loop body up to the residual norm's device
 // reductions (the host read-back is rmsNormFinish): e = 0, V-cycle, u += e,
 // restrict u, full composite residual (FACPreconditioner.C:378-420).  It
@@ -106,7 +179,7 @@ void FACPreconditioner::facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal
 // it can be captured as one HIP graph and replayed every cycle.
 void FACPreconditioner::enqueueCycle(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
   d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
-  facCycle_Recursive(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
+  runCycle(u);
   d_gpu->add(*u.data, *d_e.data, d_lmin, d_lmax);
   for (int ln = d_lmax - 1; ln >= d_lmin; --ln) d_ops.restrictSolution(u, u, ln);
   d_ops.computeCompositeResidualOnLevel(d_r, u, f, d_lmax, false);
@@ -120,7 +193,7 @@ void FACPreconditioner::enqueueCycle(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
 
 void FACPreconditioner::runOneCycle(SAMRAIVectorReal& u) {
   d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
-  facCycle_Recursive(d_e, d_r, u, d_lmax, d_lmin, d_lmax);
+  runCycle(u);
 }
 
 bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
@@ -131,6 +204,7 @@ bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
   }
   auto t0 = std::chrono::steady_clock::now();
   d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
+  if (d_te_data) d_gpu->setToScalar(*d_te.data, 0.0, d_lmin, d_lmax);
   d_gpu->setToScalar(*d_r.data, 0.0, d_lmin, d_lmax);
   const double initial = d_residual_norm = computeFullCompositeResidual(d_r, u, f);
   double eff = d_residual_tol;

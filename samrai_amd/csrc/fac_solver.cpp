@@ -107,9 +107,16 @@ void CellPoissonFACOps::setCoarsestLevelSolverChoice(const std::string& c) {
   ++d_generation;
 }
 
+// CellPoissonFACOps.C:719-726 (accepted names) and 938-943: "Ewing" fills
+// coarse-fine ghosts by constant refinement and corrects the C/F fluxes;
+// the refine-operator names fill them with that operator and leave the
+// fluxes uncorrected (computeFluxOnPatch, CellPoissonFACOps.C:2260).
 void CellPoissonFACOps::setCoarseFineDiscretization(const std::string& c) {
-  if (c != "Ewing") fail("CellPoissonFACOps: cf_discretization '" + c + "' not available");
-  d_ewing = true;
+  if (c == "Ewing" || c == "CONSTANT_REFINE") d_cf_method = sfac::kProlongConstant;
+  else if (c == "LINEAR_REFINE") d_cf_method = sfac::kProlongLinear;
+  else if (c == "CONSERVATIVE_LINEAR_REFINE") d_cf_method = sfac::kProlongConservative;
+  else fail("CellPoissonFACOps: bad cf_discretization '" + c + "'");
+  d_ewing = c == "Ewing";
   ++d_generation;
 }
 
@@ -721,16 +728,27 @@ void CellPoissonFACOps::ghostFill(PatchDataSet& x, int ln, bool with_coarse, boo
   const double* const* coarse = wc ? x.table(ln - 1) : nullptr;
   const sfac::Box* cbox = wc ? d_h.tables(ln - 1).d_box : nullptr;
   sfac::RobinDev bc = robinDev();
+  sfac::CfRefine cfr{};
+  cfr.method = wc ? d_cf_method : sfac::kProlongConstant;
+  if (wc) {
+    for (int d = 0; d < 3; ++d) cfr.cdx[d] = d_h.tables(ln - 1).geom.dx[d];
+    // the linear operators read a one-cell coarse stencil around each
+    // parent: RefineSchedule fills it from the coarser level's interiors,
+    // its own coarse-fine interpolation and the physical boundary
+    // conditions, i.e. a full ghost fill of the coarser level first
+    if (cfr.method != sfac::kProlongConstant)
+      ghostFill(x, ln - 1, ln - 1 > d_ln_min, homogeneous, false);
+  }
   if (faces_only) {
     sfac::launch_ghost_items((wc ? T.face_wc : T.face_nc).launch(), T.geom, x.table(ln),
                              x.ghost(), coarse, x.ghost(), cbox, homogeneous, bc, nullptr,
-                             d_stream);
+                             d_stream, nullptr, &cfr);
     return;
   }
   for (int ph = 0; ph < 4; ++ph)
     sfac::launch_ghost_items((wc ? T.full_wc[ph] : T.full_nc[ph]).launch(), T.geom, x.table(ln),
                              x.ghost(), coarse, x.ghost(), cbox, homogeneous, bc, nullptr,
-                             d_stream);
+                             d_stream, nullptr, &cfr);
 }
 
 int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet& r, int ln,
@@ -938,6 +956,11 @@ void CellPoissonFACOps::computeVectorWeights(PatchDataSet& w, int coarsest_ln, i
 void CellPoissonFACOps::setToScalar(PatchDataSet& x, double v, int coarse_ln, int fine_ln) {
   for (int ln = coarse_ln; ln <= fine_ln; ++ln)
     sfac::launch_fill(x.slab(ln), x.slabSize(ln), v, d_stream);
+}
+
+void CellPoissonFACOps::setToScalarInterior(PatchDataSet& x, double v, int ln) {
+  const auto& T = d_h.tables(ln);
+  sfac::launch_set_items(T.cells.launch(), T.geom, x.table(ln), x.ghost(), v, d_stream);
 }
 
 void CellPoissonFACOps::add(PatchDataSet& y, const PatchDataSet& x, int coarse_ln, int fine_ln) {

@@ -181,6 +181,7 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   // HierarchyCellDataOpsReal helpers (interior unless noted)
   void setToScalar(pdat::PatchDataSet& x, double v, int coarse_ln, int fine_ln);  // incl. ghosts
   void add(pdat::PatchDataSet& y, const pdat::PatchDataSet& x, int coarse_ln, int fine_ln);
+  void setToScalarInterior(pdat::PatchDataSet& x, double v, int ln);
   hipStream_t stream() const { return d_stream; }
   pdat::PatchDataSet& flux() { return *d_flux; }
 
@@ -271,6 +272,7 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   double d_coarse_tol = 1e-8;
   int d_coarse_max = 10;
   bool d_ewing = true;
+  int d_cf_method = sfac::kProlongConstant;  // refine operator of C/F ghosts
   std::string d_prolongation = "CONSTANT_REFINE";
   PoissonSpecifications d_spec;
   LocationIndexRobinBcCoefs d_bc;
@@ -298,6 +300,7 @@ class FACPreconditioner {
   void setPresmoothingSweeps(int n) { d_pre = n; releaseCycleGraph(); }
   void setPostsmoothingSweeps(int n) { d_post = n; releaseCycleGraph(); }
   void setAlgorithmChoice(const std::string& c);
+  const std::string& algorithmChoice() const { return d_algorithm; }
 
   void initializeSolverState(const SAMRAIVectorReal& u, const SAMRAIVectorReal& f);
   void deallocateSolverState();
@@ -321,6 +324,13 @@ class FACPreconditioner {
  private:
   void facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal& r, SAMRAIVectorReal& u,
                           int lmax, int lmin, int ln);
+  void facCycle_McCormick(SAMRAIVectorReal& e, SAMRAIVectorReal& r, SAMRAIVectorReal& u,
+                          int lmax, int lmin, int ln);
+  void facCycle(SAMRAIVectorReal& e, SAMRAIVectorReal& r, SAMRAIVectorReal& u, int lmax,
+                int lmin);
+  // the cycle of the algorithm choice
+  void runCycle(SAMRAIVectorReal& u);
+  void allocateTmpError();
   // one cycle of solveSystem up to the residual norm's device reductions
   void enqueueCycle(SAMRAIVectorReal& u, SAMRAIVectorReal& f);
   void releaseCycleGraph();
@@ -336,8 +346,9 @@ class FACPreconditioner {
   int d_pre = 1, d_post = 1;
   bool d_initialized = false;
   int d_lmin = 0, d_lmax = 0;
-  std::unique_ptr<pdat::PatchDataSet> d_e_data, d_r_data, d_tr_data;
-  SAMRAIVectorReal d_e, d_r, d_tr;
+  std::string d_algorithm = "default";
+  std::unique_ptr<pdat::PatchDataSet> d_e_data, d_r_data, d_tr_data, d_te_data;
+  SAMRAIVectorReal d_e, d_r, d_tr, d_te;  // d_te: McCormick's temporary error
   int d_iterations = 0;
   double d_residual_norm = 0.0;
   double d_avg_factor = 0.0, d_net_factor = 0.0;

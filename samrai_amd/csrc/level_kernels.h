@@ -30,10 +30,16 @@ struct SweepCheck {
 
 // Fill ghost cells of a level vector from a ghost work list (one phase);
 // with `check`, work-group 0 also runs the convergence test first.
+// Refine operator of the coarse-fine ghosts (cf_discretization): method is
+// a ProlongMethod; cdx is the coarser level's cell size.
+struct CfRefine {
+  int method;
+  double cdx[3];
+};
 void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int dg,
                         const double* const* coarse, int cg, const Box* coarse_box,
                         int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s,
-                        const SweepCheck* check = nullptr);
+                        const SweepCheck* check = nullptr, const CfRefine* cf = nullptr);
 
 // One red (colour 0) or black (colour 1) Gauss-Seidel pass with the flux
 // (and Ewing coarse-fine correction) computed on the fly.

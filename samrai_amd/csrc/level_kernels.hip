@@ -28,6 +28,42 @@ __device__ inline void tangential(int a, int dim, int& t1, int& t2) {
 // would put the array in scratch memory).
 __device__ inline int sel3(const int* v, int t) { return t == 0 ? v[0] : (t == 1 ? v[1] : v[2]); }
 
+// Limited slope of the conservative linear refine (geom_m4cartopstuff.i:45-63).
+__device__ inline double clin_slope(double dm, double dp, double dxc) {
+  double coef2 = 0.5 * (dp + dm);
+  double bound = 2.0 * fmin(fabs(dp), fabs(dm));
+  if (dm * dp > 0.0) return copysign(fmin(fabs(coef2), bound), coef2) / dxc;
+  return 0.0;
+}
+
+// Value at fine cell f of coarse data uc_ (view vc) refined by ratio r:
+// CONSTANT_REFINE (pdat_m4conrefineops3d.i), LINEAR_REFINE
+// (geom_m4cartrefineops3d.i:64-116) or CONSERVATIVE_LINEAR_REFINE
+// (geom_m4cartrefineops3d.i:118-167), in the Fortran expression order.
+template <int DIM>
+__device__ inline double refine_value(int method, const double* uc_, const View& vc, const int* f,
+                                      int r, const double* cdx, const double* fdx) {
+  const int c0 = cdiv(f[0], r), c1 = cdiv(f[1], r), c2 = DIM == 3 ? cdiv(f[2], r) : 0;
+  const double uc = uc_[vc.at(c0, c1, c2)];
+  if (method == kProlongConstant) return uc;
+  if (method == kProlongLinear) {
+    const int rr[3] = {r, r, r};
+    return linref_value<DIM>(uc_, vc, f, rr, cdx, fdx);
+  }
+  const int i0 = f[0] - c0 * r, i1 = f[1] - c1 * r, i2 = DIM == 3 ? f[2] - c2 * r : 0;
+  double s0 = clin_slope(uc - uc_[vc.at(c0 - 1, c1, c2)], uc_[vc.at(c0 + 1, c1, c2)] - uc, cdx[0]);
+  double s1 = clin_slope(uc - uc_[vc.at(c0, c1 - 1, c2)], uc_[vc.at(c0, c1 + 1, c2)] - uc, cdx[1]);
+  double d0 = ((double)i0 + 0.5) * fdx[0] - cdx[0] * 0.5;
+  double d1 = ((double)i1 + 0.5) * fdx[1] - cdx[1] * 0.5;
+  if (DIM == 3) {
+    double s2 =
+        clin_slope(uc - uc_[vc.at(c0, c1, c2 - 1)], uc_[vc.at(c0, c1, c2 + 1)] - uc, cdx[2]);
+    double d2 = ((double)i2 + 0.5) * fdx[2] - cdx[2] * 0.5;
+    return uc + s0 * d0 + s1 * d1 + s2 * d2;
+  }
+  return uc + s0 * d0 + s1 * d1;
+}
+
 // Position of interior cell c in the per-face tables of face `face`
 // (2*axis+side) of patch p (t1 fastest, then t2).
 __device__ inline long cf_pos(const LevelGeom& L, int face, const int* c, const Box& b) {
@@ -155,7 +191,7 @@ __device__ inline void sweep_check(const SweepCheck& k) {
 __global__ void __launch_bounds__(kThreads)
 k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* dst, int dg,
               const double* const* coarse, int cg, const Box* cbox, int homogeneous,
-              RobinDev bc, const int* stop, SweepCheck chk) {
+              RobinDev bc, const int* stop, SweepCheck chk, CfRefine cfr) {
   if (stop && *stop) return;
   if (chk.stop && blockIdx.x == 0) sweep_check(chk);
   const Chunk ch = chunks[blockIdx.x];
@@ -171,10 +207,12 @@ k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* co
       const Box sb = L.box[it.q];
       x[vd.at(c[0], c[1], c[2])] = dst[it.q][vview(sb, dg, L.dim).at(c[0], c[1], c[2])];
     } else if (kind == kCoarseFine) {
-      const Box sb = cbox[it.q];
-      int r = L.ratio;
-      int cc0 = cdiv(c[0], r), cc1 = cdiv(c[1], r), cc2 = L.dim == 3 ? cdiv(c[2], r) : 0;
-      x[vd.at(c[0], c[1], c[2])] = coarse[it.q][vview(sb, cg, L.dim).at(cc0, cc1, cc2)];
+      // the refine operator of cf_discretization over the coarse patch
+      // holding the parent (its ghosts filled beforehand for stencils)
+      const View vc = vview(cbox[it.q], cg, L.dim);
+      x[vd.at(c[0], c[1], c[2])] =
+          L.dim == 3 ? refine_value<3>(cfr.method, coarse[it.q], vc, c, L.ratio, cfr.cdx, L.dx)
+                     : refine_value<2>(cfr.method, coarse[it.q], vc, c, L.ratio, cfr.cdx, L.dx);
     } else if (kind == kPhys1) {
       // settype1cells: u_g = (h g + u_in (b - h a / 2)) / (b + h a / 2)
       int n = loc >> 1, up = loc & 1;
@@ -573,13 +611,6 @@ k_restrict(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
   }
 }
 
-__device__ inline double clin_slope(double dm, double dp, double dxc) {
-  double coef2 = 0.5 * (dp + dm);
-  double bound = 2.0 * fmin(fabs(dp), fabs(dm));
-  if (dm * dp > 0.0) return copysign(fmin(fabs(coef2), bound), coef2) / dxc;
-  return 0.0;
-}
-
 template <int DIM>
 __global__ void __launch_bounds__(kThreads)
 k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
@@ -589,40 +620,10 @@ k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom 
   const View vc = vview(C.box[it.q], cg, DIM), vf = vview(Fl.box[it.p], fg, DIM);
   const double* uc_ = ce[it.q];
   double* uf = fe[it.p];
-  const int r = Fl.ratio;
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
     int f[3];
     if (!decode_box(it, (long)ch.start + t, f)) break;
-    int c0 = cdiv(f[0], r), c1 = cdiv(f[1], r), c2 = DIM == 3 ? cdiv(f[2], r) : 0;
-    int i0 = f[0] - c0 * r, i1 = f[1] - c1 * r, i2 = DIM == 3 ? f[2] - c2 * r : 0;
-    double uc = uc_[vc.at(c0, c1, c2)];
-    if (method == kProlongConstant) {  // CONSTANT_REFINE (pdat_m4conrefineops3d.i)
-      long o = vf.at(f[0], f[1], f[2]);
-      uf[o] = uf[o] + uc;
-      continue;
-    }
-    if (method == kProlongLinear) {  // LINEAR_REFINE (geom_m4cartrefineops3d.i:64-116)
-      const int rr[3] = {r, r, r};
-      const double v = linref_value<DIM>(uc_, vc, f, rr, C.dx, Fl.dx);
-      long o = vf.at(f[0], f[1], f[2]);
-      uf[o] = uf[o] + v;
-      continue;
-    }
-    double s0 = clin_slope(uc - uc_[vc.at(c0 - 1, c1, c2)], uc_[vc.at(c0 + 1, c1, c2)] - uc,
-                           C.dx[0]);
-    double s1 = clin_slope(uc - uc_[vc.at(c0, c1 - 1, c2)], uc_[vc.at(c0, c1 + 1, c2)] - uc,
-                           C.dx[1]);
-    double d0 = ((double)i0 + 0.5) * Fl.dx[0] - C.dx[0] * 0.5;
-    double d1 = ((double)i1 + 0.5) * Fl.dx[1] - C.dx[1] * 0.5;
-    double v;
-    if (DIM == 3) {
-      double s2 = clin_slope(uc - uc_[vc.at(c0, c1, c2 - 1)], uc_[vc.at(c0, c1, c2 + 1)] - uc,
-                             C.dx[2]);
-      double d2 = ((double)i2 + 0.5) * Fl.dx[2] - C.dx[2] * 0.5;
-      v = uc + s0 * d0 + s1 * d1 + s2 * d2;
-    } else {
-      v = uc + s0 * d0 + s1 * d1;
-    }
+    const double v = refine_value<DIM>(method, uc_, vc, f, Fl.ratio, C.dx, Fl.dx);
     long o = vf.at(f[0], f[1], f[2]);
     uf[o] = uf[o] + v;
   }
@@ -726,15 +727,18 @@ inline void check_launch() {
 void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int dg,
                         const double* const* coarse, int cg, const Box* coarse_box,
                         int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s,
-                        const SweepCheck* check) {
+                        const SweepCheck* check, const CfRefine* cf) {
   SweepCheck chk{};
   if (check) chk = *check;
+  CfRefine cfr{};
+  cfr.method = kProlongConstant;
+  if (cf) cfr = *cf;
   if (w.nchunks == 0) {
     if (check) launch_sweep_check(*check, s);
     return;
   }
   hipLaunchKernelGGL(k_ghost_items, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L,
-                     dst, dg, coarse, cg, coarse_box, homogeneous, bc, stop, chk);
+                     dst, dg, coarse, cg, coarse_box, homogeneous, bc, stop, chk, cfr);
   check_launch();
 }
 

@@ -83,6 +83,7 @@ def lib():
     L.sfac_solver_fused_tile_timings.argtypes = [_vp, _dp, C.c_int, _ip]
     L.sfac_solver_coarse_mg_stats.argtypes = [_vp, _ip, _dp]
     L.sfac_solver_set_graphs.argtypes = [_vp, C.c_int]
+    L.sfac_solver_set_algorithm_choice.argtypes = [_vp, C.c_char_p]
     L.sfac_fused_tile_variants.argtypes = [C.c_int]
     L.sfac_fused_tile_shape.argtypes = [C.c_int, C.c_int, _ip]
     L.sfac_ops_internal_vector.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
@@ -322,6 +323,11 @@ class CellPoissonFACSolver:
         (one launch per sweep, same results); off: patch-wise colour passes.
         tile_variant -1: autotuned tile shape."""
         _check(lib().sfac_solver_set_fused_coarsest(self.s, int(on), int(tile_variant)))
+
+    def set_algorithm_choice(self, choice):
+        """FACPreconditioner::setAlgorithmChoice: "default", "mccormick-s4.3"
+        or "pernice"."""
+        _check(lib().sfac_solver_set_algorithm_choice(self.s, choice.encode()))
 
     def set_graphs(self, on=True):
         """Capture each FAC cycle as a HIP graph and replay it (same results)."""

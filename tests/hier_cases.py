@@ -112,7 +112,7 @@ class Pair:
     """The GPU solver and the oracle on one hierarchy with identical data."""
 
     def __init__(self, case, params, data, dvar=False, dconst=1.0, cmode=0, cconst=0.0,
-                 bc=None):
+                 bc=None, algorithm=None):
         from samrai_amd import fac
         dim = case["dim"]
         self.case = case
@@ -150,6 +150,8 @@ class Pair:
             for p in range(len(boxes)):
                 self.u.put(ln, p, data["u"][ln][p])
                 self.f.put(ln, p, data["f"][ln][p])
+        if algorithm:
+            self.s.set_algorithm_choice(algorithm)
         self.s.initialize_solver_state(self.u, self.f)
         # oracle
         o = OracleFAC(dim, case["levels"], case["ratio"], case["dlo"], case["dhi"], [0.0] * dim,
@@ -166,6 +168,9 @@ class Pair:
             o.set_coarse_mg(True)
         # the GPU solver's default prolongation is the reference's (CONSTANT_REFINE)
         o.set_prolongation(params.get("prolongation_method", "CONSTANT_REFINE"))
+        o.set_cf_discretization(params.get("cf_discretization", "Ewing"))
+        if algorithm:
+            o.set_algorithm(algorithm)
         for ln, boxes in enumerate(case["levels"]):
             for p in range(len(boxes)):
                 o.put("u", ln, p, data["u"][ln][p])

@@ -41,6 +41,45 @@ def test_ghost_fill(gpu_lib, mk):
                 pr.compare(pr.u, "u", levels=[ln])
 
 
+CF_METHODS = ["CONSTANT_REFINE", "LINEAR_REFINE", "CONSERVATIVE_LINEAR_REFINE"]
+
+
+@pytest.mark.parametrize("cf", CF_METHODS)
+@pytest.mark.parametrize("mk", hc.ALL, ids=lambda f: f.__name__)
+def test_ghost_fill_cf_discretization(gpu_lib, mk, cf):
+    """Coarse-fine ghosts by the cf_discretization refine operator; the
+    linear operators first fill the coarser levels' ghosts (recursively),
+    so every level is compared."""
+    case = mk()
+    pr = _pair(mk, SPECS[0], bc=hc.bc_robin(case["dim"]),
+               params=dict(PARAMS, cf_discretization=cf))
+    for ln in range(1, len(case["levels"])):
+        for hom in (0, 1):
+            print("cf ghost fill", case["name"], cf, ln, hom, flush=True)
+            pr.s.ghost_fill(pr.u, ln, 1, hom)
+            pr.o.ghost_fill("u", ln, 1, hom)
+            pr.compare(pr.u, "u", levels=list(range(ln + 1)))
+
+
+@pytest.mark.parametrize("cf", CF_METHODS)
+@pytest.mark.parametrize("mk", hc.ALL, ids=lambda f: f.__name__)
+def test_smooth_and_residual_cf_discretization(gpu_lib, mk, cf):
+    """Smoothing and composite residual without the Ewing flux fix, with the
+    C/F ghosts of each refine operator (variable D and C)."""
+    case = mk()
+    pr = _pair(mk, SPECS[1], params=dict(PARAMS, cf_discretization=cf))
+    lmax = len(case["levels"]) - 1
+    for ln in range(lmax, 0, -1):
+        print("cf smooth/residual", case["name"], cf, ln, flush=True)
+        pr.s.smooth(pr.u, pr.f, ln, 2)
+        pr.o.smooth("u", "f", ln, 2)
+        pr.compare(pr.u, "u", levels=[ln])
+        for hom in (1, 0):
+            pr.s.residual(pr.r, pr.u, pr.f, ln, hom)
+            pr.o.residual("r", "u", "f", ln, hom)
+            pr.compare(pr.r, "r", levels=[ln])
+
+
 @pytest.mark.parametrize("spec", SPECS, ids=["constD", "varD-varC", "constC"])
 @pytest.mark.parametrize("mk", hc.ALL, ids=lambda f: f.__name__)
 def test_smooth(gpu_lib, mk, spec):

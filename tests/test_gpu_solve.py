@@ -175,3 +175,43 @@ def test_solve_with_other_prolongations_matches_oracle(gpu_lib, mk, method):
     assert pr.s.number_of_iterations() == it_o
     np.testing.assert_allclose(pr.s.history(), hist_o, rtol=1e-12, atol=0)
     pr.compare(pr.u, "u", interior_only=True, rtol=1e-12)
+
+
+@pytest.mark.parametrize("cf", ["CONSTANT_REFINE", "LINEAR_REFINE", "CONSERVATIVE_LINEAR_REFINE"])
+@pytest.mark.parametrize("mk", hc.ALL, ids=lambda f: f.__name__)
+def test_solve_with_other_cf_discretizations_matches_oracle(gpu_lib, mk, cf):
+    """cf_discretization by a refine operator (C/F ghosts refined from the
+    coarser level, no Ewing flux fix) through whole solves, cycle by cycle
+    against the oracle, with variable D."""
+    case = mk()
+    data = smooth_data(case, 5)
+    pr = hc.Pair(case, dict(PARAMS, cf_discretization=cf), data, dvar=True)
+    print("solve", case["name"], cf, flush=True)
+    conv_g = pr.s.solve_system(pr.u, pr.f)
+    conv_o, it_o, res_o, hist_o = pr.o.solve()
+    assert conv_g == conv_o
+    assert pr.s.number_of_iterations() == it_o
+    np.testing.assert_allclose(pr.s.history(), hist_o, rtol=1e-12, atol=0)
+    pr.compare(pr.u, "u", interior_only=True, rtol=1e-12)
+
+
+@pytest.mark.parametrize("graphs", [False, True], ids=["eager", "graphs"])
+@pytest.mark.parametrize("algorithm", ["mccormick-s4.3", "pernice", "MCCORMICK-S4.3"])
+@pytest.mark.parametrize("mk", hc.ALL + [hc.case_2d_c0], ids=lambda f: f.__name__)
+def test_solve_with_other_cycles_matches_oracle(gpu_lib, mk, algorithm, graphs):
+    """FACPreconditioner algorithm_choice "mccormick-s4.3" and "pernice"
+    (case-insensitive) through whole solves, cycle by cycle against the
+    oracle, eagerly and as replayed whole-cycle graphs."""
+    case = mk()
+    data = smooth_data(case, 3)
+    pr = hc.Pair(case, PARAMS, data, algorithm=algorithm.lower())
+    if algorithm != algorithm.lower():
+        pr.s.set_algorithm_choice(algorithm)
+    pr.s.set_graphs(graphs)
+    print("solve", case["name"], algorithm, graphs, flush=True)
+    conv_g = pr.s.solve_system(pr.u, pr.f)
+    conv_o, it_o, res_o, hist_o = pr.o.solve()
+    assert conv_g == conv_o
+    assert pr.s.number_of_iterations() == it_o
+    np.testing.assert_allclose(pr.s.history(), hist_o, rtol=1e-12, atol=0)
+    pr.compare(pr.u, "u", interior_only=True, rtol=1e-12)

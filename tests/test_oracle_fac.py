@@ -67,12 +67,14 @@ def test_hiertest_control_volumes_and_norms():
 
 
 def sine_problem(n, levels):
-    """2-D unit square, n^2 base; finer levels double the resolution over
-    the middle half.  u = sin(pi x) sin(pi y), f = -2 pi^2 u, Dirichlet 0."""
+    """2-D unit square, n^2 base; level ln doubles the resolution over the
+    centred square of side 2^-ln (properly nested).  u = sin(pi x) sin(pi y),
+    f = -2 pi^2 u, Dirichlet 0."""
     boxes = [[((0, 0), (n - 1, n - 1))]]
     for ln in range(1, levels):
         m = n * 2 ** ln
-        boxes.append([((m // 4, m // 4), (3 * m // 4 - 1, 3 * m // 4 - 1))])
+        lo, hi = m // 2 - m // 2 ** (ln + 1), m // 2 + m // 2 ** (ln + 1) - 1
+        boxes.append([((lo, lo), (hi, hi))])
     o = OracleFAC(2, boxes, 2, (0, 0), (n - 1, n - 1), (0.0, 0.0), (1.0, 1.0))
     o.set_poisson(False, 1.0, 0, 0.0)
     o.set_bc([1.0] * 6, [0.0] * 6, [0.0] * 6)
@@ -155,3 +157,120 @@ def test_multigrid_coarse_solver_reaches_the_same_solution(levels):
     assert cycles[1] <= cycles[0]
     scale = np.abs(sols[0]).max()
     assert np.abs(sols[1] - sols[0]).max() <= 1e-8 * scale
+
+
+def linear_problem(dim, n, axis):
+    """u = x_axis exactly: f = 0, Dirichlet 0 / 1 on the two sides normal to
+    `axis`, homogeneous Neumann elsewhere; base n^dim, one finer level
+    (ratio 2) over the middle half."""
+    lo0, hi0 = (0,) * dim, (n - 1,) * dim
+    m = 2 * n
+    boxes = [[(lo0, hi0)], [((m // 4,) * dim, (3 * m // 4 - 1,) * dim)]]
+    o = OracleFAC(dim, boxes, 2, lo0, hi0, (0.0,) * dim, (1.0,) * dim)
+    o.set_poisson(False, 1.0, 0, 0.0)
+    a, b, g = [0.0] * 6, [1.0] * 6, [0.0] * 6
+    a[2 * axis] = a[2 * axis + 1] = 1.0
+    b[2 * axis] = b[2 * axis + 1] = 0.0
+    g[2 * axis + 1] = 1.0
+    o.set_bc(a, b, g)
+    for ln, bl in enumerate(boxes):
+        for p, (lo, hi) in enumerate(bl):
+            shp = tuple(h - l + 1 for l, h in zip(lo, hi))
+            o.put("f", ln, p, np.zeros(shp))
+            o.put("u", ln, p, np.zeros(tuple(x + 2 for x in shp)))
+    return o, boxes
+
+
+def linear_error(o, boxes, n, axis, dim):
+    err = 0.0
+    for ln, bl in enumerate(boxes):
+        dx = (1.0 / (n * 2 ** ln),) * dim
+        for p, (lo, hi) in enumerate(bl):
+            shp = tuple(h - l + 3 for l, h in zip(lo, hi))
+            u = o.get("u", ln, p).reshape(shp, order="F")[(slice(1, -1),) * dim]
+            X = cell_centres(lo, hi, dx)
+            err = max(err, float(np.abs(u - X[axis]).max()))
+    return err
+
+
+@pytest.mark.parametrize("cf", ["LINEAR_REFINE", "CONSERVATIVE_LINEAR_REFINE", "CONSTANT_REFINE"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cf_refine_operators_and_linear_solutions(dim, cf):
+    """With C/F ghosts refined linearly (LINEAR_REFINE: trilinear
+    interpolation; CONSERVATIVE_LINEAR_REFINE: limited slopes, which equal
+    the exact ones on linear data) the composite discretisation is exact for
+    a linear solution, along every axis: the converged FAC solution equals
+    x_axis at every cell centre of both levels to rounding.  A wrong
+    fine-cell offset, axis or coarse neighbour in the ghost refinement breaks
+    this.  Constant refinement without the Ewing fix is not consistent at the
+    C/F boundary: the same test then misses by O(h)."""
+    n = 16 if dim == 2 else 8
+    for axis in range(dim):
+        o, boxes = linear_problem(dim, n, axis)
+        o.set_cf_discretization(cf)
+        o.set_params(max_cycles=300, residual_tol=1e-13, relative_residual_tol=-1.0, pre=2,
+                     post=2, coarse_max_iterations=500, coarse_tol=1e-14, ewing=True)
+        conv, it, res, hist = o.solve()
+        assert conv, (axis, it, res)
+        err = linear_error(o, boxes, n, axis, dim)
+        if cf == "CONSTANT_REFINE":
+            assert err > 1e-3, (axis, err)
+        else:
+            assert err < 1e-12, (axis, err)
+
+
+@pytest.mark.parametrize("cf,lo,hi", [("LINEAR_REFINE", 3.4, 4.6),
+                                      ("CONSERVATIVE_LINEAR_REFINE", 3.4, 4.6),
+                                      ("CONSTANT_REFINE", 1.7, 2.4)])
+def test_cf_discretization_convergence_order(cf, lo, hi):
+    """Against the analytic sine solution: second order with linear C/F
+    ghosts, first order with constant ones and no flux fix."""
+    errs = []
+    for n in (16, 32):
+        o, boxes = sine_problem(n, 2)
+        o.set_cf_discretization(cf)
+        o.set_params(max_cycles=400, residual_tol=1e-300, relative_residual_tol=1e-11,
+                     pre=2, post=2, coarse_max_iterations=500, coarse_tol=1e-13, ewing=True)
+        conv, it, res, hist = o.solve()
+        assert conv
+        errs.append(max_error(o, boxes, n))
+    ratio = errs[0] / errs[1]
+    assert lo < ratio < hi, (errs, ratio)
+
+
+def _solve_with(algorithm, cf="Ewing", levels=3, n=16):
+    o, boxes = sine_problem(n, levels)
+    o.set_cf_discretization(cf)
+    o.set_algorithm(algorithm)
+    o.set_params(max_cycles=400, residual_tol=1e-300, relative_residual_tol=1e-11, pre=2,
+                 post=2, coarse_max_iterations=500, coarse_tol=1e-13, ewing=True)
+    conv, it, res, hist = o.solve()
+    u = np.concatenate([o.get("u", ln, p) for ln in range(len(boxes))
+                        for p in range(len(boxes[ln]))])
+    return conv, it, hist, u
+
+
+@pytest.mark.parametrize("cf", ["Ewing", "CONSERVATIVE_LINEAR_REFINE"])
+def test_pernice_cycle_is_the_recursive_cycle(cf):
+    """facCycle ("pernice", FACPreconditioner.C:678-763) runs the operations
+    of facCycle_Recursive (:461-518) in the same order: identical residual
+    histories and solutions, bit for bit."""
+    c0, i0, h0, u0 = _solve_with("default", cf)
+    c1, i1, h1, u1 = _solve_with("pernice", cf)
+    assert c0 and c1 and i0 == i1
+    assert np.array_equal(h0, h1) and np.array_equal(u0, u1)
+
+
+@pytest.mark.parametrize("cf", ["Ewing", "LINEAR_REFINE"])
+def test_mccormick_cycle_reaches_the_same_solution(cf):
+    """facCycle_McCormick (FACPreconditioner.C:530-668) is another iteration
+    for the same composite system: on two levels it converges, and its fixed
+    point equals the recursive cycle's to the solver tolerance.  (Step 3a
+    restricts the residual into the temporary residual, which step 2a of the
+    next coarser level overwrites, so levels below the finest keep the
+    cycle's initial residual; from three levels on the restated cycle then
+    does not converge on this problem, as the reference's code reads.)"""
+    c0, i0, h0, u0 = _solve_with("default", cf, levels=2)
+    c1, i1, h1, u1 = _solve_with("mccormick-s4.3", cf, levels=2)
+    assert c0 and c1
+    assert np.abs(u1 - u0).max() <= 1e-8 * np.abs(u0).max()
