@@ -120,8 +120,26 @@ def setup_c2(fac, decks, torch, params, seed=0):
     return deck, h, u, f, D, s
 
 
-def time_c2(fac, decks, torch, params):
+def level_errors(deck, ua, ub):
+    """Per-level RMS and max |ua - ub| over interior cells (ub: FP64 reference)."""
+    out = []
+    for ln, boxes in enumerate(deck["levels"]):
+        sq, n, mx = 0.0, 0, 0.0
+        for p, box in enumerate(boxes):
+            shp = tuple(h - l + 3 for l, h in zip(*box))
+            a = ua.get(ln, p).reshape(shp, order="F")[1:-1, 1:-1, 1:-1]
+            b = ub.get(ln, p).reshape(shp, order="F")[1:-1, 1:-1, 1:-1]
+            d = a - b
+            sq += float(np.sum(d * d))
+            n += d.size
+            mx = max(mx, float(np.max(np.abs(d))))
+        out.append(dict(level=ln, l2=float(np.sqrt(sq / max(n, 1))), max=mx))
+    return out
+
+
+def time_c2(fac, decks, torch, params, bits=64):
     deck, h, u, f, D, s = setup_c2(fac, decks, torch, params)
+    s.set_coarse_precision(bits)
     u.fill(0.0)
     s.solve_system(u, f)  # warm-up (graph capture, autotuning)
     u.fill(0.0)
@@ -134,7 +152,7 @@ def time_c2(fac, decks, torch, params):
                final_residual=float(s.history()[-1]))
     if params.get("coarse_solver_choice") == "mg":
         out["last_coarse_vcycles"] = s.coarse_mg_stats()[0]
-    return deck, out
+    return deck, out, (h, u, f, D, s)
 
 
 def main():
@@ -273,16 +291,47 @@ def main():
     print("bench: mg coarse solver: %.4f s/solve, %d cycles" % (mg["s_per_solve"], mg["cycles"]),
           file=sys.stderr, flush=True)
 
+    # mixed precision: the coarsest-level solve of the error equation in FP32
+    # (storage and arithmetic) inside the FP64 FAC iteration; errors per
+    # level against the FP64 solution of the same solver (which equals the
+    # oracle's to 1e-12 in the parity tests)
+    def mixed(params, ref_u):
+        hm, um, fm, sm = build_problem(fac, decks, deck, params)
+        sm.set_coarse_precision(32)
+        um.fill(0.0)
+        sm.solve_system(um, fm)
+        times = []
+        for k in range(args.steps):
+            um.fill(0.0)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            okm = sm.solve_system(um, fm)
+            sm.synchronize()
+            times.append(time.perf_counter() - t3)
+        out = dict(s_per_solve=round(sum(times) / len(times), 4), cycles=sm.number_of_iterations(),
+                   converged=bool(okm), final_residual=float(sm.history()[-1]),
+                   error_vs_fp64_per_level=level_errors(deck, um, ref_u))
+        return out
+    mixed_c1 = dict(coarse="coarsest-level solve in FP32 storage and arithmetic; FAC "
+                           "iteration, residuals and finer levels FP64",
+                    redblack_coarse=mixed(PARAMS, u), mg_coarse=mixed(MG_PARAMS, u2))
+    mixed_c1["redblack_coarse"]["fp64_s_per_solve"] = round(ms_per_step / 1e3, 4)
+    mixed_c1["mg_coarse"]["fp64_s_per_solve"] = mg["s_per_solve"]
+    print("bench: mixed precision C1 %s" % json.dumps(mixed_c1), file=sys.stderr, flush=True)
+
     c2 = None
     if not args.no_c2:
         print("bench: C2 deck (variable D), red-black and multigrid coarse solvers",
               file=sys.stderr, flush=True)
-        deck2, rb2 = time_c2(fac, decks, torch, PARAMS)
-        _, mg2 = time_c2(fac, decks, torch, MG_PARAMS)
+        deck2, rb2, _ = time_c2(fac, decks, torch, PARAMS)
+        _, mg2, ref2 = time_c2(fac, decks, torch, MG_PARAMS)
+        _, mg2m, mix2 = time_c2(fac, decks, torch, MG_PARAMS, bits=32)
+        mg2m["error_vs_fp64_per_level"] = level_errors(deck2, mix2[1], ref2[1])
+        mg2m["fp64_s_per_solve"] = mg2["s_per_solve"]
         c2 = dict(workload="c2-3d: 4 levels ratio 2, base 128^3, patch sides U{8..48} (seed 0), "
                            "D = exp(GRF, corr 0.1, sigma 1), f = 8 Gaussian sources (width 0.02)",
                   patches=sum(len(l) for l in deck2["levels"]), cells=decks.count_cells(deck2),
-                  redblack_coarse=rb2, mg_coarse=mg2)
+                  redblack_coarse=rb2, mg_coarse=mg2, mg_coarse_mixed_fp32=mg2m)
         print("bench: C2 %s" % json.dumps(c2), file=sys.stderr, flush=True)
 
     cpu = None
@@ -312,7 +361,8 @@ def main():
                smoother_gcells_per_s=round(smoother_gcells, 2),
                finest_smoother_gcells_per_s=round(finest_gcells, 2),
                fused_coarsest=fused, fused_tile=tile, fused_tile_autotune_us_per_sweep=tuning,
-               roofline=roofline, cpu_baseline=cpu, with_mg_coarse_solver=mg, c2=c2)
+               roofline=roofline, cpu_baseline=cpu, with_mg_coarse_solver=mg,
+               mixed_precision=mixed_c1, c2=c2)
     print(json.dumps(out), flush=True)
 
 

@@ -293,6 +293,15 @@ int sfac_solver_coarse_mg_stats(sfac_solver* s, int* iterations, double* relres)
  * .C:678-763), case-insensitive; any other name is an error. */
 int sfac_solver_set_algorithm_choice(sfac_solver* s, const char* choice);
 
+/* Mixed precision, no reference counterpart: bits = 32 runs the coarsest-level
+ * solve of the error equation (red-black sweeps or multigrid on a coarsest
+ * level that tiles the domain) in FP32 storage and arithmetic; the FAC
+ * iteration, its residuals, the finer levels and u stay FP64, so the
+ * solution still meets the FP64 relative residual test.  The multigrid
+ * tolerance is floored at 1e-6 in FP32.  bits = 64 (default): the reference
+ * arithmetic.  Takes effect at once (re-creates the coarse buffers). */
+int sfac_solver_set_coarse_precision(sfac_solver* s, int bits);
+
 /* Execution choice with no reference counterpart (results are unchanged):
  * on = 1 (default) captures one FACPreconditioner cycle (e = 0, V-cycle,
  * u += e, restriction, composite residual and its norm reductions) as a HIP

@@ -29,6 +29,11 @@ struct BoxLevel {
   int lp_hi[3];
 };
 
+// Arrays of the level.  fp32 = 0: FP64 storage and arithmetic (the
+// reference's).  fp32 = 1: the arrays hold float values (the pointers are
+// reinterpreted; buffers are allocated at the FP64 size either way) and the
+// sweeps, transfers and multigrid compute in FP32; the early-exit maxima and
+// norm partial sums stay FP64.
 struct BoxArrays {
   const double* ein;    // u before the sweep: n0*n1*n2, no ghosts
   double* eout;         // u after the sweep (a second buffer: work-groups read
@@ -36,6 +41,20 @@ struct BoxArrays {
   const double* r;      // same shape
   const double* D[3];   // side data per axis, +1 along the axis (variable D)
   const double* C;      // cell data (C mode 2)
+  double dconst;
+  int cmode;
+  double cconst;
+  int fp32;
+};
+
+// The same arrays typed by their storage precision (device side).
+template <typename T>
+struct BoxArraysT {
+  const T* ein;
+  T* eout;
+  const T* r;
+  const T* D[3];
+  const T* C;
   double dconst;
   int cmode;
   double cconst;
@@ -74,27 +93,31 @@ void launch_box_final_check(const BoxCheck& K, int nsweeps, hipStream_t s);
 // contiguous box array: to_box 1 gathers, 0 scatters.  The box array is
 // box[0] or box[1] by the parity of *count (count == nullptr: of count_host),
 // which lets the scatter follow a sweep count known only on the device.
+// fp32: the box arrays hold float (converted from / to the FP64 patch data).
 void launch_box_cells(const GhostLaunch& cells, const LevelGeom& L, double* const* x, int g,
                       double* box0, double* box1, const int* count, int count_host,
-                      const BoxLevel& G, int to_box, hipStream_t s);
+                      const BoxLevel& G, int to_box, hipStream_t s, int fp32 = 0);
 // Gather side data of axis a into the box face array; with verify, faces
 // shared by two patches are then compared and *mismatch set to 1 if the
 // patches hold different values there.
 void launch_box_side(const LevelGeom& L, const double* const* x, int g, int a, double* box,
-                     const BoxLevel& G, int* mismatch, hipStream_t s, int verify);
+                     const BoxLevel& G, int* mismatch, hipStream_t s, int verify,
+                     int fp32 = 0);
 
 // ---- geometric multigrid on a single-rectangle level ----
 // Largest edge of the grid the one-work-group LDS solve accepts, and the
 // number of partial sums launch_mg_sumsq writes.
 int mg_small_edge();
 int mg_partials();
+// Arrays u, x, fc, ec, ef, Dc, Cc have the precision of A.fp32 (fp32).
 // fc = mean over the 2^dim children of f - (div D grad u + C u) on Gf.
 void launch_mg_restrict_residual(const BoxLevel& Gf, const BoxArrays& A, const RobinDev& bc,
                                  const double* u, const BoxLevel& Gc, double* fc,
                                  const int* stop, hipStream_t s);
 // ef += separable linear interpolation of ec (homogeneous Robin beyond the domain).
 void launch_mg_prolong_add(const BoxLevel& Gc, const RobinDev& bc, const double* ec,
-                           const BoxLevel& Gf, double* ef, const int* stop, hipStream_t s);
+                           const BoxLevel& Gf, double* ef, const int* stop, hipStream_t s,
+                           int fp32 = 0);
 // Coarse D faces (mean of the fine faces they cover) and C (mean of children).
 void launch_mg_coarsen_coef(const BoxLevel& Gf, const BoxArrays& Af, const BoxLevel& Gc,
                             double* const* Dc, double* Cc, hipStream_t s);

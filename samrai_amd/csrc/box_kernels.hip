@@ -16,6 +16,10 @@
 // as in the level-batched kernels.  Edge and corner ghosts are not read by
 // the 7-point (5-point) stencil, so they are not formed here; the caller's
 // final ghost fill sets them in the patches.
+//
+// Every kernel is templated on the storage type T: double is the reference
+// arithmetic; float is the mixed-precision coarse solve (FP32 storage and
+// arithmetic inside the FP64 FAC iteration), with the same expression order.
 #include "box_kernels.h"
 
 namespace sfac {
@@ -30,27 +34,44 @@ __device__ inline int pos2(int x) { return x & 1; }  // parity of any int (two's
 
 // Robin face ghost from its interior neighbour (settype1cells, homogeneous
 // or not), location loc = 2*axis + side.
-template <int DIM>
-__device__ inline double robin_ghost(const BoxLevel& G, const RobinDev& bc, int homogeneous,
-                                     int loc, double uin) {
+template <int DIM, typename T>
+__device__ inline T robin_ghost(const BoxLevel& G, const RobinDev& bc, int homogeneous, int loc,
+                                T uin) {
   const int n = loc >> 1;
-  const double h = G.dx[n], av = bc.a[loc], bv = bc.b[loc];
+  const T h = (T)G.dx[n], av = (T)bc.a[loc], bv = (T)bc.b[loc], g = (T)bc.g[loc];
+  const T half = (T)0.5;
   if (DIM == 3)
-    return ((homogeneous ? 0.0 : h * bc.g[loc]) + uin * (bv - 0.5 * h * av)) /
-           (bv + 0.5 * h * av);
-  if (homogeneous) return (uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
-  return (h * bc.g[loc] + uin * (bv - 0.5 * h * av)) / (bv + 0.5 * h * av);
+    return ((homogeneous ? (T)0 : h * g) + uin * (bv - half * h * av)) / (bv + half * h * av);
+  if (homogeneous) return (uin * (bv - half * h * av)) / (bv + half * h * av);
+  return (h * g + uin * (bv - half * h * av)) / (bv + half * h * av);
+}
+
+// Box arrays typed by their storage precision.
+template <typename T>
+inline BoxArraysT<T> typed(const BoxArrays& A) {
+  BoxArraysT<T> t;
+  t.ein = reinterpret_cast<const T*>(A.ein);
+  t.eout = reinterpret_cast<T*>(A.eout);
+  t.r = reinterpret_cast<const T*>(A.r);
+  for (int a = 0; a < 3; ++a) t.D[a] = reinterpret_cast<const T*>(A.D[a]);
+  t.C = reinterpret_cast<const T*>(A.C);
+  t.dconst = A.dconst;
+  t.cmode = A.cmode;
+  t.cconst = A.cconst;
+  return t;
 }
 
 // Per-cell inputs of one Gauss-Seidel update that do not change during a
 // sweep: right-hand side, the six face coefficients (variable D) and C.
+template <typename T>
 struct Coef {
-  double f, xl, xh, yl, yh, zl, zh, c;
+  T f, xl, xh, yl, yh, zl, zh, c;
 };
 
-template <int DIM, bool DVAR, int CMODE>
-__device__ inline Coef load_coef(const BoxLevel& G, const BoxArrays& A, int i, int j, int k) {
-  Coef q{};
+template <int DIM, bool DVAR, int CMODE, typename T>
+__device__ inline Coef<T> load_coef(const BoxLevel& G, const BoxArraysT<T>& A, int i, int j,
+                                    int k) {
+  Coef<T> q{};
   const int n0 = G.n[0], n1 = G.n[1];
   const long c = (long)i + (long)n0 * (j + (long)n1 * k);
   q.f = A.r[c];
@@ -73,15 +94,15 @@ __device__ inline Coef load_coef(const BoxLevel& G, const BoxArrays& A, int i, i
 // Gauss-Seidel update of one cell held in LDS at q (neighbours at q -+ 1,
 // q -+ SY, q - SZd and q + SZu), in the expression order of
 // rbgswithfluxmax*.  Returns |residual|.
-template <int DIM, bool DVAR, int CMODE>
-__device__ inline double cell_update(double* U, int q, int SY, int SZd, int SZu, const Coef& k,
-                                     const BoxLevel& G, const BoxArrays& A) {
-  const double dxi = 1.0 / G.dx[0], dyi = 1.0 / G.dx[1];
-  const double dzi = DIM == 3 ? 1.0 / G.dx[2] : 0.0;
-  const double dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
-  const double dc = A.dconst;
-  const double uc = U[q];
-  double Dxl, Dxh, Dyl, Dyh, Dzl = 0.0, Dzh = 0.0;
+template <int DIM, bool DVAR, int CMODE, typename T>
+__device__ inline T cell_update(T* U, int q, int SY, int SZd, int SZu, const Coef<T>& k,
+                                const BoxLevel& G, const BoxArraysT<T>& A) {
+  const T dxi = (T)(1.0 / G.dx[0]), dyi = (T)(1.0 / G.dx[1]);
+  const T dzi = DIM == 3 ? (T)(1.0 / G.dx[2]) : (T)0;
+  const T dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
+  const T dc = (T)A.dconst;
+  const T uc = U[q];
+  T Dxl, Dxh, Dyl, Dyh, Dzl = 0, Dzh = 0;
   if (DVAR) {
     Dxl = k.xl;
     Dxh = k.xh;
@@ -94,20 +115,20 @@ __device__ inline double cell_update(double* U, int q, int SY, int SZd, int SZu,
   } else {
     Dxl = Dxh = Dyl = Dyh = Dzl = Dzh = dc;
   }
-  const double fxl = dxi * Dxl * (uc - U[q - 1]);
-  const double fxh = dxi * Dxh * (U[q + 1] - uc);
-  const double fyl = dyi * Dyl * (uc - U[q - SY]);
-  const double fyh = dyi * Dyh * (U[q + SY] - uc);
-  double fzl = 0.0, fzh = 0.0;
+  const T fxl = dxi * Dxl * (uc - U[q - 1]);
+  const T fxh = dxi * Dxh * (U[q + 1] - uc);
+  const T fyl = dyi * Dyl * (uc - U[q - SY]);
+  const T fyh = dyi * Dyh * (U[q + SY] - uc);
+  T fzl = 0, fzh = 0;
   if (DIM == 3) {
     fzl = dzi * Dzl * (uc - U[q - SZd]);
     fzh = dzi * Dzh * (U[q + SZu] - uc);
   }
-  double scal;
+  T scal;
   if (CMODE == 2) scal = k.c;
-  else scal = CMODE == 1 ? A.cconst : 0.0;
-  const double fv = k.f;
-  double residual, diag;
+  else scal = CMODE == 1 ? (T)A.cconst : (T)0;
+  const T fv = k.f;
+  T residual, diag;
   if (DIM == 3) {
     residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
     if (DVAR)
@@ -121,11 +142,16 @@ __device__ inline double cell_update(double* U, int q, int SY, int SZd, int SZu,
     else
       diag = (dxi * dxi * (dc + dc) + dyi * dyi * (dc + dc)) - scal;
   }
-  const double dudr = 1.0 / diag;
-  const double du = -residual * dudr;
-  U[q] = uc + du * 1.0;
+  const T dudr = (T)1 / diag;
+  const T du = -residual * dudr;
+  U[q] = uc + du * (T)1;
   return fabs(residual);
 }
+
+__device__ inline bool same_bits(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
+__device__ inline bool same_bits(float a, float b) { return __float_as_int(a) == __float_as_int(b); }
 
 // Work-group max of v (kThreads threads); result valid in thread 0.
 __device__ inline double block_max(double v, double* wm) {
@@ -161,14 +187,14 @@ __device__ inline bool stop_sweep(const BoxCheck& K, int b, double* wm, int& s_s
   return s_stop != 0;
 }
 
-template <int DIM, bool DVAR, int CMODE, int TX, int TY, int TZ>
+template <int DIM, bool DVAR, int CMODE, int TX, int TY, int TZ, typename T>
 __global__ void __launch_bounds__(kThreads)
-k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, int ntx, int nty,
-           int ntiles, int per_xcd) {
+k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K, int ntx,
+           int nty, int ntiles, int per_xcd) {
   constexpr int LX = TX + 4, LY = TY + 4, LZ = DIM == 3 ? TZ + 4 : 1;
   constexpr int SY = LX, SZ = LX * LY;
   constexpr int ZH = DIM == 3 ? 2 : 0;  // LDS halo along z
-  __shared__ double U[LX * LY * LZ];
+  __shared__ T U[LX * LY * LZ];
   __shared__ double wm[kThreads / 64];
   __shared__ int s_stop;
   // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
@@ -214,18 +240,18 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
   // decomposition per column, then a fixed stride per plane)
   constexpr int NCOL = (LX * LY + kThreads - 1) / kThreads;
   const long plane = (long)n0 * n1;
-  double v[NCOL][LZ];
+  T v[NCOL][LZ];
 #pragma unroll
   for (int c = 0; c < NCOL; ++c) {
     const int col = threadIdx.x + c * kThreads;
     const int lx = col % LX, ly = col / LX;
     const int i = x0 - 2 + lx, j = y0 - 2 + ly;
     const bool xy = col < LX * LY && i >= 0 && i < n0 && j >= 0 && j < n1;
-    const double* src = A.ein + ((long)i + (long)n0 * j);
+    const T* src = A.ein + ((long)i + (long)n0 * j);
 #pragma unroll
     for (int lz = 0; lz < LZ; ++lz) {
       const int k = z0 - ZH + lz;
-      v[c][lz] = (xy && (DIM == 2 || (k >= 0 && k < n2))) ? src[plane * k] : 0.0;
+      v[c][lz] = (xy && (DIM == 2 || (k >= 0 && k < n2))) ? src[plane * k] : (T)0;
     }
   }
   // the cells this thread updates: LDS index, flags (1 update, 2 counts
@@ -235,7 +261,7 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
     return i >= G.lp_lo[0] && i <= G.lp_hi[0] && j >= G.lp_lo[1] && j <= G.lp_hi[1] &&
            (DIM == 2 || (k >= G.lp_lo[2] && k <= G.lp_hi[2]));
   };
-  Coef cr[NR], cb[NB];
+  Coef<T> cr[NR], cb[NB];
   int rq[NR], rf[NR], bq[NB], bf[NB];
 #pragma unroll
   for (int u = 0; u < NR; ++u) {
@@ -248,7 +274,7 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
                      (DIM == 2 || (lz >= 2 && lz < TZ + 2));
     rq[u] = lx + SY * ly + SZ * lz;
     rf[u] = (act ? 1 : 0) | ((act && lp_tile && own && in_lp(i, j, k)) ? 2 : 0);
-    cr[u] = act ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef{};
+    cr[u] = act ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef<T>{};
   }
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
@@ -259,7 +285,7 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
     const bool act = w < NBW && inside(i, j, k);
     bq[u] = lx + SY * ly + SZ * lz;
     bf[u] = (act ? 1 : 0) | ((act && lp_tile && in_lp(i, j, k)) ? 2 : 0);
-    cb[u] = act ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef{};
+    cb[u] = act ? load_coef<DIM, DVAR, CMODE>(G, A, i, j, k) : Coef<T>{};
   }
 #pragma unroll
   for (int c = 0; c < NCOL; ++c) {
@@ -291,7 +317,7 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
           if (DIM == 3 && (g2 < 0 || g2 >= G.n[t2])) continue;
           const int q = lc[0] + SY * lc[1] + SZ * lc[2];
           const int stride = a == 0 ? 1 : (a == 1 ? SY : SZ);
-          const double uin = U[side ? q - stride : q + stride];
+          const T uin = U[side ? q - stride : q + stride];
           U[q] = robin_ghost<DIM>(G, bc, homogeneous, 2 * a + side, uin);
         }
       }
@@ -305,8 +331,8 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
 #pragma unroll
   for (int u = 0; u < NR; ++u) {
     if (rf[u] & 1) {
-      const double ra = cell_update<DIM, DVAR, CMODE>(U, rq[u], SY, SZ, SZ, cr[u], G, A);
-      if (rf[u] & 2) mr = fmax(mr, ra);
+      const T ra = cell_update<DIM, DVAR, CMODE>(U, rq[u], SY, SZ, SZ, cr[u], G, A);
+      if (rf[u] & 2) mr = fmax(mr, (double)ra);
     }
   }
   __syncthreads();
@@ -317,8 +343,8 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     if (bf[u] & 1) {
-      const double ra = cell_update<DIM, DVAR, CMODE>(U, bq[u], SY, SZ, SZ, cb[u], G, A);
-      if (bf[u] & 2) mr = fmax(mr, ra);
+      const T ra = cell_update<DIM, DVAR, CMODE>(U, bq[u], SY, SZ, SZ, cb[u], G, A);
+      if (bf[u] & 2) mr = fmax(mr, (double)ra);
     }
   }
   __syncthreads();
@@ -329,8 +355,8 @@ k_box_rbgs(BoxLevel G, BoxArrays A, RobinDev bc, int homogeneous, BoxCheck K, in
     const int ox = col % TX, oy = col / TX;
     const int i = x0 + ox, j = y0 + oy;
     if (col >= TX * TY || i >= n0 || j >= n1) continue;
-    double* dst = A.eout + ((long)i + (long)n0 * j);
-    const double* src = U + (ox + 2) + SY * (oy + 2) + SZ * ZH;
+    T* dst = A.eout + ((long)i + (long)n0 * j);
+    const T* src = U + (ox + 2) + SY * (oy + 2) + SZ * ZH;
 #pragma unroll
     for (int oz = 0; oz < (DIM == 3 ? TZ : 1); ++oz)
       if (DIM == 2 || z0 + oz < n2) dst[plane * (z0 + oz)] = src[SZ * oz];
@@ -359,11 +385,11 @@ __global__ void __launch_bounds__(kThreads) k_box_final_check(BoxCheck K, int ns
   }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_box_cells(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* x, int g,
-            double* box0, double* box1, const int* count, int count_host, BoxLevel G,
-            int to_box) {
-  double* box = ((count ? *count : count_host) & 1) ? box1 : box0;
+            T* box0, T* box1, const int* count, int count_host, BoxLevel G, int to_box) {
+  T* box = ((count ? *count : count_host) & 1) ? box1 : box0;
   const Chunk ch = chunks[blockIdx.x];
   const BoxItem it = items[ch.item];
   const Box pb = L.box[it.p];
@@ -374,15 +400,16 @@ k_box_cells(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* cons
     if (!decode_box(it, (long)ch.start + t, c)) break;
     const long o = (long)(c[0] - G.lo[0]) +
                    (long)G.n[0] * ((c[1] - G.lo[1]) + (long)G.n[1] * (L.dim == 3 ? c[2] - G.lo[2] : 0));
-    if (to_box) box[o] = xp[v.at(c[0], c[1], c[2])];
-    else xp[v.at(c[0], c[1], c[2])] = box[o];
+    if (to_box) box[o] = (T)xp[v.at(c[0], c[1], c[2])];
+    else xp[v.at(c[0], c[1], c[2])] = (double)box[o];
   }
 }
 
 // grid (npatch, slices): faces of axis a of patch blockIdx.x
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_box_side(LevelGeom L, const double* const* x, int g, int a, double* box, BoxLevel G,
-           int* mismatch, int verify) {
+k_box_side(LevelGeom L, const double* const* x, int g, int a, T* box, BoxLevel G, int* mismatch,
+           int verify) {
   const int p = blockIdx.x;
   const Box pb = L.box[p];
   int e[3] = {1, 1, 1};
@@ -399,21 +426,21 @@ k_box_side(LevelGeom L, const double* const* x, int g, int a, double* box, BoxLe
     const int k = L.dim == 3 ? pb.lo[2] + (int)(t / ((long)e[0] * e[1])) : 0;
     const long o = (long)(i - G.lo[0]) +
                    (long)bn[0] * ((j - G.lo[1]) + (long)bn[1] * (L.dim == 3 ? k - G.lo[2] : 0));
-    const double val = xp[v.at(i, j, k)];
+    const T val = (T)xp[v.at(i, j, k)];
     if (!verify) box[o] = val;
-    else if (__double_as_longlong(box[o]) != __double_as_longlong(val))
+    else if (!same_bits(box[o], val))
       *mismatch = 1;
   }
 }
 
-template <int DIM, bool DVAR, int CMODE, int TX, int TY, int TZ>
-void launch_tile(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int homogeneous,
+template <int DIM, bool DVAR, int CMODE, int TX, int TY, int TZ, typename T>
+void launch_tile(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc, int homogeneous,
                  const BoxCheck& K, hipStream_t s) {
   const int ntx = (G.n[0] + TX - 1) / TX, nty = (G.n[1] + TY - 1) / TY;
   const int ntz = DIM == 3 ? (G.n[2] + TZ - 1) / TZ : 1;
   const int ntiles = ntx * nty * ntz;
   const int per_xcd = (ntiles + 7) / 8;
-  hipLaunchKernelGGL((k_box_rbgs<DIM, DVAR, CMODE, TX, TY, TZ>), dim3(8 * per_xcd),
+  hipLaunchKernelGGL((k_box_rbgs<DIM, DVAR, CMODE, TX, TY, TZ, T>), dim3(8 * per_xcd),
                      dim3(kThreads), 0, s, G, A, bc, homogeneous, K, ntx, nty, ntiles, per_xcd);
   check_launch();
 }
@@ -426,8 +453,8 @@ constexpr BoxTile kTiles2[kVariants] = {{64, 16, 1}, {32, 16, 1}, {32, 32, 1},
                                         {128, 8, 1}, {64, 8, 1},  {16, 16, 1},
                                         {64, 16, 1}, {64, 16, 1}, {64, 16, 1}};
 
-template <int DIM, bool DVAR, int CMODE>
-void launch_variant(int variant, const BoxLevel& G, const BoxArrays& A, const RobinDev& bc,
+template <int DIM, bool DVAR, int CMODE, typename T>
+void launch_variant(int variant, const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc,
                     int homogeneous, const BoxCheck& K, hipStream_t s) {
   if (DIM == 3) {
     switch (variant) {
@@ -453,8 +480,8 @@ void launch_variant(int variant, const BoxLevel& G, const BoxArrays& A, const Ro
   }
 }
 
-template <int DIM>
-void launch_dim(int variant, const BoxLevel& G, const BoxArrays& A, const RobinDev& bc,
+template <int DIM, typename T>
+void launch_dim(int variant, const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc,
                 int homogeneous, const BoxCheck& K, hipStream_t s) {
   const bool dvar = A.D[0] != nullptr;
   if (dvar) {
@@ -479,14 +506,14 @@ void launch_dim(int variant, const BoxLevel& G, const BoxArrays& A, const RobinD
 
 // u at box index (i, j, k) with homogeneous Robin ghosts one cell beyond the
 // domain along any axis (applied axis by axis).
-template <int DIM>
-__device__ inline double mg_at(const BoxLevel& G, const RobinDev& bc, const double* u, int i,
-                               int j, int k) {
+template <int DIM, typename T>
+__device__ inline T mg_at(const BoxLevel& G, const RobinDev& bc, const T* u, int i, int j,
+                          int k) {
   int c[3] = {i, j, k}, lo[3] = {0, 0, 0};
   for (int a = 0; a < DIM; ++a) {
     lo[a] = c[a] < 0 ? 0 : (c[a] >= G.n[a] ? G.n[a] - 1 : c[a]);
   }
-  double v = u[(long)lo[0] + (long)G.n[0] * (lo[1] + (long)G.n[1] * (DIM == 3 ? lo[2] : 0))];
+  T v = u[(long)lo[0] + (long)G.n[0] * (lo[1] + (long)G.n[1] * (DIM == 3 ? lo[2] : 0))];
   for (int a = 0; a < DIM; ++a) {
     if (c[a] < 0) v = robin_ghost<DIM>(G, bc, 1, 2 * a, v);
     else if (c[a] >= G.n[a]) v = robin_ghost<DIM>(G, bc, 1, 2 * a + 1, v);
@@ -496,42 +523,42 @@ __device__ inline double mg_at(const BoxLevel& G, const RobinDev& bc, const doub
 
 // Residual f - (div D grad u + C u) of one cell, in the smoother's
 // expression order (global-memory neighbours, Robin ghosts at the domain).
-template <int DIM, bool DVAR, int CMODE>
-__device__ inline double mg_residual(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc,
-                                     const double* u, int i, int j, int k) {
+template <int DIM, bool DVAR, int CMODE, typename T>
+__device__ inline T mg_residual(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc,
+                                const T* u, int i, int j, int k) {
   const int n0 = G.n[0], n1 = G.n[1];
   const long c = (long)i + (long)n0 * (j + (long)n1 * k);
-  const Coef q = load_coef<DIM, DVAR, CMODE>(G, A, i, j, k);
-  const double dxi = 1.0 / G.dx[0], dyi = 1.0 / G.dx[1];
-  const double dzi = DIM == 3 ? 1.0 / G.dx[2] : 0.0;
-  const double dc = A.dconst;
-  const double uc = u[c];
-  double Dxl, Dxh, Dyl, Dyh, Dzl = 0.0, Dzh = 0.0;
+  const Coef<T> q = load_coef<DIM, DVAR, CMODE>(G, A, i, j, k);
+  const T dxi = (T)(1.0 / G.dx[0]), dyi = (T)(1.0 / G.dx[1]);
+  const T dzi = DIM == 3 ? (T)(1.0 / G.dx[2]) : (T)0;
+  const T dc = (T)A.dconst;
+  const T uc = u[c];
+  T Dxl, Dxh, Dyl, Dyh, Dzl = 0, Dzh = 0;
   if (DVAR) {
     Dxl = q.xl; Dxh = q.xh; Dyl = q.yl; Dyh = q.yh;
     if (DIM == 3) { Dzl = q.zl; Dzh = q.zh; }
   } else {
     Dxl = Dxh = Dyl = Dyh = Dzl = Dzh = dc;
   }
-  const double uxl = i > 0 ? u[c - 1] : robin_ghost<DIM>(G, bc, 1, 0, uc);
-  const double uxh = i < n0 - 1 ? u[c + 1] : robin_ghost<DIM>(G, bc, 1, 1, uc);
-  const double uyl = j > 0 ? u[c - n0] : robin_ghost<DIM>(G, bc, 1, 2, uc);
-  const double uyh = j < n1 - 1 ? u[c + n0] : robin_ghost<DIM>(G, bc, 1, 3, uc);
-  const double fxl = dxi * Dxl * (uc - uxl);
-  const double fxh = dxi * Dxh * (uxh - uc);
-  const double fyl = dyi * Dyl * (uc - uyl);
-  const double fyh = dyi * Dyh * (uyh - uc);
-  double fzl = 0.0, fzh = 0.0;
+  const T uxl = i > 0 ? u[c - 1] : robin_ghost<DIM>(G, bc, 1, 0, uc);
+  const T uxh = i < n0 - 1 ? u[c + 1] : robin_ghost<DIM>(G, bc, 1, 1, uc);
+  const T uyl = j > 0 ? u[c - n0] : robin_ghost<DIM>(G, bc, 1, 2, uc);
+  const T uyh = j < n1 - 1 ? u[c + n0] : robin_ghost<DIM>(G, bc, 1, 3, uc);
+  const T fxl = dxi * Dxl * (uc - uxl);
+  const T fxh = dxi * Dxh * (uxh - uc);
+  const T fyl = dyi * Dyl * (uc - uyl);
+  const T fyh = dyi * Dyh * (uyh - uc);
+  T fzl = 0, fzh = 0;
   if (DIM == 3) {
     const long s2 = (long)n0 * n1;
-    const double uzl = k > 0 ? u[c - s2] : robin_ghost<DIM>(G, bc, 1, 4, uc);
-    const double uzh = k < G.n[2] - 1 ? u[c + s2] : robin_ghost<DIM>(G, bc, 1, 5, uc);
+    const T uzl = k > 0 ? u[c - s2] : robin_ghost<DIM>(G, bc, 1, 4, uc);
+    const T uzh = k < G.n[2] - 1 ? u[c + s2] : robin_ghost<DIM>(G, bc, 1, 5, uc);
     fzl = dzi * Dzl * (uc - uzl);
     fzh = dzi * Dzh * (uzh - uc);
   }
-  double scal;
+  T scal;
   if (CMODE == 2) scal = q.c;
-  else scal = CMODE == 1 ? A.cconst : 0.0;
+  else scal = CMODE == 1 ? (T)A.cconst : (T)0;
   if (DIM == 3)
     return q.f - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
   return q.f - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
@@ -539,10 +566,10 @@ __device__ inline double mg_residual(const BoxLevel& G, const BoxArrays& A, cons
 
 // fc = mean over the 2^dim children of the fine residual (one thread per
 // coarse cell; children summed in i, j, k order).
-template <int DIM, bool DVAR, int CMODE>
+template <int DIM, bool DVAR, int CMODE, typename T>
 __global__ void __launch_bounds__(kThreads)
-k_mg_restrict_residual(BoxLevel Gf, BoxArrays A, RobinDev bc, const double* u, BoxLevel Gc,
-                       double* fc, const int* stop) {
+k_mg_restrict_residual(BoxLevel Gf, BoxArraysT<T> A, RobinDev bc, const T* u, BoxLevel Gc, T* fc,
+                       const int* stop) {
   if (*stop) return;
   const long nc = (long)Gc.n[0] * Gc.n[1] * (DIM == 3 ? Gc.n[2] : 1);
   for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < nc;
@@ -550,21 +577,20 @@ k_mg_restrict_residual(BoxLevel Gf, BoxArrays A, RobinDev bc, const double* u, B
     const int I = (int)(t % Gc.n[0]);
     const int J = (int)((t / Gc.n[0]) % Gc.n[1]);
     const int K = DIM == 3 ? (int)(t / ((long)Gc.n[0] * Gc.n[1])) : 0;
-    double sum = 0.0;
+    T sum = 0;
     for (int c2 = 0; c2 < (DIM == 3 ? 2 : 1); ++c2)
       for (int c1 = 0; c1 < 2; ++c1)
         for (int c0 = 0; c0 < 2; ++c0)
           sum += mg_residual<DIM, DVAR, CMODE>(Gf, A, bc, u, 2 * I + c0, 2 * J + c1,
                                                DIM == 3 ? 2 * K + c2 : 0);
-    fc[t] = sum * (DIM == 3 ? 0.125 : 0.25);
+    fc[t] = sum * (T)(DIM == 3 ? 0.125 : 0.25);
   }
 }
 
 // ef += linear interpolation of ec (one thread per fine cell).
-template <int DIM>
+template <int DIM, typename T>
 __global__ void __launch_bounds__(kThreads)
-k_mg_prolong_add(BoxLevel Gc, RobinDev bc, const double* ec, BoxLevel Gf, double* ef,
-                 const int* stop) {
+k_mg_prolong_add(BoxLevel Gc, RobinDev bc, const T* ec, BoxLevel Gf, T* ef, const int* stop) {
   if (*stop) return;
   const long nf = (long)Gf.n[0] * Gf.n[1] * (DIM == 3 ? Gf.n[2] : 1);
   for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < nf;
@@ -574,21 +600,21 @@ k_mg_prolong_add(BoxLevel Gc, RobinDev bc, const double* ec, BoxLevel Gf, double
     const int k = DIM == 3 ? (int)(t / ((long)Gf.n[0] * Gf.n[1])) : 0;
     const int I = i >> 1, J = j >> 1, K = k >> 1;
     const int sx = (i & 1) ? 1 : -1, sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
-    double v;
+    const T w3 = (T)0.75, w1 = (T)0.25;
+    T v;
     if (DIM == 3) {
-      double z0 = 0.75 * (0.75 * mg_at<3>(Gc, bc, ec, I, J, K) +
-                          0.25 * mg_at<3>(Gc, bc, ec, I + sx, J, K)) +
-                  0.25 * (0.75 * mg_at<3>(Gc, bc, ec, I, J + sy, K) +
-                          0.25 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K));
-      double z1 = 0.75 * (0.75 * mg_at<3>(Gc, bc, ec, I, J, K + sz) +
-                          0.25 * mg_at<3>(Gc, bc, ec, I + sx, J, K + sz)) +
-                  0.25 * (0.75 * mg_at<3>(Gc, bc, ec, I, J + sy, K + sz) +
-                          0.25 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K + sz));
-      v = 0.75 * z0 + 0.25 * z1;
+      T z0 = w3 * (w3 * mg_at<3>(Gc, bc, ec, I, J, K) + w1 * mg_at<3>(Gc, bc, ec, I + sx, J, K)) +
+             w1 * (w3 * mg_at<3>(Gc, bc, ec, I, J + sy, K) +
+                   w1 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K));
+      T z1 = w3 * (w3 * mg_at<3>(Gc, bc, ec, I, J, K + sz) +
+                   w1 * mg_at<3>(Gc, bc, ec, I + sx, J, K + sz)) +
+             w1 * (w3 * mg_at<3>(Gc, bc, ec, I, J + sy, K + sz) +
+                   w1 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K + sz));
+      v = w3 * z0 + w1 * z1;
     } else {
-      v = 0.75 * (0.75 * mg_at<2>(Gc, bc, ec, I, J, 0) + 0.25 * mg_at<2>(Gc, bc, ec, I + sx, J, 0)) +
-          0.25 * (0.75 * mg_at<2>(Gc, bc, ec, I, J + sy, 0) +
-                  0.25 * mg_at<2>(Gc, bc, ec, I + sx, J + sy, 0));
+      v = w3 * (w3 * mg_at<2>(Gc, bc, ec, I, J, 0) + w1 * mg_at<2>(Gc, bc, ec, I + sx, J, 0)) +
+          w1 * (w3 * mg_at<2>(Gc, bc, ec, I, J + sy, 0) +
+                w1 * mg_at<2>(Gc, bc, ec, I + sx, J + sy, 0));
     }
     ef[t] = ef[t] + v;
   }
@@ -596,13 +622,12 @@ k_mg_prolong_add(BoxLevel Gc, RobinDev bc, const double* ec, BoxLevel Gf, double
 
 // Coefficients of level m+1 from level m: a coarse face takes the mean of
 // the 2^(dim-1) fine faces it covers; C the mean of the 2^dim children.
-template <int DIM>
+template <int DIM, typename T>
 __global__ void __launch_bounds__(kThreads)
-k_mg_coarsen_coef(BoxLevel Gf, BoxArrays Af, BoxLevel Gc, double* Dc0, double* Dc1, double* Dc2,
-                  double* Cc) {
+k_mg_coarsen_coef(BoxLevel Gf, BoxArraysT<T> Af, BoxLevel Gc, T* Dc0, T* Dc1, T* Dc2, T* Cc) {
   const int nc[3] = {Gc.n[0], Gc.n[1], DIM == 3 ? Gc.n[2] : 1};
   const int nf[3] = {Gf.n[0], Gf.n[1], DIM == 3 ? Gf.n[2] : 1};
-  double* Dc[3] = {Dc0, Dc1, Dc2};
+  T* Dc[3] = {Dc0, Dc1, Dc2};
   for (int a = 0; a < DIM; ++a) {
     if (!Dc[a]) continue;
     int e[3] = {nc[0], nc[1], nc[2]};
@@ -614,7 +639,7 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArrays Af, BoxLevel Gc, double* Dc0, double* D
     for (long t = (long)blockIdx.x * kThreads + threadIdx.x; t < n;
          t += (long)gridDim.x * kThreads) {
       int c[3] = {(int)(t % e[0]), (int)((t / e[0]) % e[1]), (int)(t / ((long)e[0] * e[1]))};
-      double sum = 0.0;
+      T sum = 0;
       for (int q2 = 0; q2 < (DIM == 3 ? 2 : 1); ++q2)
         for (int q1 = 0; q1 < 2; ++q1) {
           int f[3];
@@ -623,7 +648,7 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArrays Af, BoxLevel Gc, double* Dc0, double* D
           f[t2] = DIM == 3 ? 2 * c[t2] + q2 : 0;
           sum += Af.D[a][(long)f[0] + (long)ef[0] * (f[1] + (long)ef[1] * f[2])];
         }
-      Dc[a][t] = sum * (DIM == 3 ? 0.25 : 0.5);
+      Dc[a][t] = sum * (T)(DIM == 3 ? 0.25 : 0.5);
     }
   }
   if (Cc) {
@@ -632,13 +657,13 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArrays Af, BoxLevel Gc, double* Dc0, double* D
          t += (long)gridDim.x * kThreads) {
       const int I = (int)(t % nc[0]), J = (int)((t / nc[0]) % nc[1]);
       const int K = (int)(t / ((long)nc[0] * nc[1]));
-      double sum = 0.0;
+      T sum = 0;
       for (int c2 = 0; c2 < (DIM == 3 ? 2 : 1); ++c2)
         for (int c1 = 0; c1 < 2; ++c1)
           for (int c0 = 0; c0 < 2; ++c0)
             sum += Af.C[(long)(2 * I + c0) +
                         (long)nf[0] * ((2 * J + c1) + (long)nf[1] * (DIM == 3 ? 2 * K + c2 : 0))];
-      Cc[t] = sum * (DIM == 3 ? 0.125 : 0.25);
+      Cc[t] = sum * (T)(DIM == 3 ? 0.125 : 0.25);
     }
   }
 }
@@ -647,21 +672,21 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArrays Af, BoxLevel Gc, double* Dc0, double* D
 // `sweeps` red-black sweeps with the whole grid and its ghosts in LDS and
 // each thread's cell coefficients in registers; one work-group.
 constexpr int kMGSmallEdge = 4;
-template <int DIM, bool DVAR, int CMODE>
+template <int DIM, bool DVAR, int CMODE, typename T>
 __global__ void __launch_bounds__(kThreads)
-k_mg_small_solve(BoxLevel G, BoxArrays A, RobinDev bc, double* u, int sweeps, const int* stop) {
+k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, const int* stop) {
   constexpr int L = kMGSmallEdge + 2;
   constexpr int SY = L, SZ = L * L;
   constexpr int NMAX = DIM == 3 ? kMGSmallEdge * kMGSmallEdge * kMGSmallEdge
                                 : kMGSmallEdge * kMGSmallEdge;
   constexpr int NC = (NMAX + kThreads - 1) / kThreads;
-  __shared__ double U[DIM == 3 ? L * L * L : L * L];
+  __shared__ T U[DIM == 3 ? L * L * L : L * L];
   if (*stop) return;
   const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
   const int ncell = n0 * n1 * n2;
-  for (int q = threadIdx.x; q < (DIM == 3 ? L * L * L : L * L); q += kThreads) U[q] = 0.0;
+  for (int q = threadIdx.x; q < (DIM == 3 ? L * L * L : L * L); q += kThreads) U[q] = (T)0;
   int qc[NC], col[NC];
-  Coef cf[NC];
+  Coef<T> cf[NC];
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
     const int t = threadIdx.x + v * kThreads;
@@ -706,10 +731,10 @@ k_mg_small_solve(BoxLevel G, BoxArrays A, RobinDev bc, double* u, int sweeps, co
 
 // Partial sums of the squared residual (x == nullptr) or of x^2 over a
 // level, one per work-group of a fixed grid.
-template <int DIM, bool DVAR, int CMODE>
+template <int DIM, bool DVAR, int CMODE, typename T>
 __global__ void __launch_bounds__(kThreads)
-k_mg_sumsq(BoxLevel G, BoxArrays A, RobinDev bc, const double* u, const double* x,
-           double* partial, const int* stop) {
+k_mg_sumsq(BoxLevel G, BoxArraysT<T> A, RobinDev bc, const T* u, const T* x, double* partial,
+           const int* stop) {
   __shared__ double sm[kThreads / 64];
   if (*stop) return;
   const long n = (long)G.n[0] * G.n[1] * (DIM == 3 ? G.n[2] : 1);
@@ -718,11 +743,11 @@ k_mg_sumsq(BoxLevel G, BoxArrays A, RobinDev bc, const double* u, const double* 
        t += (long)gridDim.x * kThreads) {
     double v;
     if (x) {
-      v = x[t];
+      v = (double)x[t];
     } else {
       const int i = (int)(t % G.n[0]), j = (int)((t / G.n[0]) % G.n[1]);
       const int k = DIM == 3 ? (int)(t / ((long)G.n[0] * G.n[1])) : 0;
-      v = mg_residual<DIM, DVAR, CMODE>(G, A, bc, u, i, j, k);
+      v = (double)mg_residual<DIM, DVAR, CMODE>(G, A, bc, u, i, j, k);
     }
     acc += v * v;
   }
@@ -767,8 +792,15 @@ BoxTile box_tile(int dim, int variant) {
 
 void launch_box_rbgs(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, int homogeneous,
                      const BoxCheck& K, int variant, hipStream_t s) {
-  if (G.dim == 3) launch_dim<3>(variant, G, A, bc, homogeneous, K, s);
-  else launch_dim<2>(variant, G, A, bc, homogeneous, K, s);
+  if (A.fp32) {
+    const BoxArraysT<float> At = typed<float>(A);
+    if (G.dim == 3) launch_dim<3>(variant, G, At, bc, homogeneous, K, s);
+    else launch_dim<2>(variant, G, At, bc, homogeneous, K, s);
+  } else {
+    const BoxArraysT<double> At = typed<double>(A);
+    if (G.dim == 3) launch_dim<3>(variant, G, At, bc, homogeneous, K, s);
+    else launch_dim<2>(variant, G, At, bc, homogeneous, K, s);
+  }
 }
 
 void launch_box_final_check(const BoxCheck& K, int nsweeps, hipStream_t s) {
@@ -778,22 +810,32 @@ void launch_box_final_check(const BoxCheck& K, int nsweeps, hipStream_t s) {
 
 void launch_box_cells(const GhostLaunch& cells, const LevelGeom& L, double* const* x, int g,
                       double* box0, double* box1, const int* count, int count_host,
-                      const BoxLevel& G, int to_box, hipStream_t s) {
+                      const BoxLevel& G, int to_box, hipStream_t s, int fp32) {
   if (cells.nchunks == 0) return;
-  hipLaunchKernelGGL(k_box_cells, dim3(cells.nchunks), dim3(kThreads), 0, s, cells.items,
-                     cells.chunks, L, x, g, box0, box1, count, count_host, G, to_box);
+  if (fp32)
+    hipLaunchKernelGGL(k_box_cells<float>, dim3(cells.nchunks), dim3(kThreads), 0, s,
+                       cells.items, cells.chunks, L, x, g, reinterpret_cast<float*>(box0),
+                       reinterpret_cast<float*>(box1), count, count_host, G, to_box);
+  else
+    hipLaunchKernelGGL(k_box_cells<double>, dim3(cells.nchunks), dim3(kThreads), 0, s,
+                       cells.items, cells.chunks, L, x, g, box0, box1, count, count_host, G,
+                       to_box);
   check_launch();
 }
 
 void launch_box_side(const LevelGeom& L, const double* const* x, int g, int a, double* box,
-                     const BoxLevel& G, int* mismatch, hipStream_t s, int verify) {
+                     const BoxLevel& G, int* mismatch, hipStream_t s, int verify, int fp32) {
   if (L.npatch == 0) return;
   dim3 grid(L.npatch, 64);
-  hipLaunchKernelGGL(k_box_side, grid, dim3(kThreads), 0, s, L, x, g, a, box, G, mismatch, 0);
-  check_launch();
-  if (!verify) return;
-  hipLaunchKernelGGL(k_box_side, grid, dim3(kThreads), 0, s, L, x, g, a, box, G, mismatch, 1);
-  check_launch();
+  for (int pass = 0; pass < (verify ? 2 : 1); ++pass) {
+    if (fp32)
+      hipLaunchKernelGGL(k_box_side<float>, grid, dim3(kThreads), 0, s, L, x, g, a,
+                         reinterpret_cast<float*>(box), G, mismatch, pass);
+    else
+      hipLaunchKernelGGL(k_box_side<double>, grid, dim3(kThreads), 0, s, L, x, g, a, box, G,
+                         mismatch, pass);
+    check_launch();
+  }
 }
 
 
@@ -807,8 +849,8 @@ inline int mg_blocks(long n) {
   return (int)(b < kMGGrid ? (b > 0 ? b : 1) : kMGGrid);
 }
 
-template <template <int, bool, int> class F, class... Args>
-void dispatch3(const BoxLevel& G, const BoxArrays& A, Args&&... args) {
+template <template <int, bool, int> class F, typename T, class... Args>
+void dispatch3(const BoxLevel& G, const BoxArraysT<T>& A, Args&&... args) {
   const bool dvar = A.D[0] != nullptr;
   if (G.dim == 3) {
     if (dvar) {
@@ -835,10 +877,11 @@ void dispatch3(const BoxLevel& G, const BoxArrays& A, Args&&... args) {
 
 template <int DIM, bool DVAR, int CMODE>
 struct RestrictRun {
-  static void run(const BoxLevel& Gf, const BoxArrays& A, const RobinDev& bc, const double* u,
-                  const BoxLevel& Gc, double* fc, const int* stop, hipStream_t s) {
+  template <typename T>
+  static void run(const BoxLevel& Gf, const BoxArraysT<T>& A, const RobinDev& bc, const T* u,
+                  const BoxLevel& Gc, T* fc, const int* stop, hipStream_t s) {
     const long nc = (long)Gc.n[0] * Gc.n[1] * (DIM == 3 ? Gc.n[2] : 1);
-    hipLaunchKernelGGL((k_mg_restrict_residual<DIM, DVAR, CMODE>), dim3(mg_blocks(nc)),
+    hipLaunchKernelGGL((k_mg_restrict_residual<DIM, DVAR, CMODE, T>), dim3(mg_blocks(nc)),
                        dim3(kThreads), 0, s, Gf, A, bc, u, Gc, fc, stop);
     check_launch();
   }
@@ -846,23 +889,30 @@ struct RestrictRun {
 
 template <int DIM, bool DVAR, int CMODE>
 struct SmallRun {
-  static void run(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, double* u,
+  template <typename T>
+  static void run(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc, T* u,
                   int sweeps, const int* stop, hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_small_solve<DIM, DVAR, CMODE>), dim3(1), dim3(kThreads), 0, s, G, A,
-                       bc, u, sweeps, stop);
+    hipLaunchKernelGGL((k_mg_small_solve<DIM, DVAR, CMODE, T>), dim3(1), dim3(kThreads), 0, s, G,
+                       A, bc, u, sweeps, stop);
     check_launch();
   }
 };
 
 template <int DIM, bool DVAR, int CMODE>
 struct SumsqRun {
-  static void run(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, const double* u,
-                  const double* x, double* partial, const int* stop, hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_sumsq<DIM, DVAR, CMODE>), dim3(kMGGrid), dim3(kThreads), 0, s, G, A,
-                       bc, u, x, partial, stop);
+  template <typename T>
+  static void run(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc, const T* u,
+                  const T* x, double* partial, const int* stop, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_sumsq<DIM, DVAR, CMODE, T>), dim3(kMGGrid), dim3(kThreads), 0, s, G,
+                       A, bc, u, x, partial, stop);
     check_launch();
   }
 };
+
+template <typename T>
+const T* as(const double* p) { return reinterpret_cast<const T*>(p); }
+template <typename T>
+T* as(double* p) { return reinterpret_cast<T*>(p); }
 }  // namespace
 
 int mg_small_edge() { return kMGSmallEdge; }
@@ -871,41 +921,67 @@ int mg_partials() { return kMGGrid; }
 void launch_mg_restrict_residual(const BoxLevel& Gf, const BoxArrays& A, const RobinDev& bc,
                                  const double* u, const BoxLevel& Gc, double* fc,
                                  const int* stop, hipStream_t s) {
-  dispatch3<RestrictRun>(Gf, A, bc, u, Gc, fc, stop, s);
+  if (A.fp32)
+    dispatch3<RestrictRun>(Gf, typed<float>(A), bc, as<float>(u), Gc, as<float>(fc), stop, s);
+  else
+    dispatch3<RestrictRun>(Gf, typed<double>(A), bc, u, Gc, fc, stop, s);
+}
+
+template <typename T>
+static void mg_prolong(const BoxLevel& Gc, const RobinDev& bc, const T* ec, const BoxLevel& Gf,
+                       T* ef, const int* stop, hipStream_t s) {
+  const long nf = (long)Gf.n[0] * Gf.n[1] * (Gf.dim == 3 ? Gf.n[2] : 1);
+  if (Gf.dim == 3)
+    hipLaunchKernelGGL((k_mg_prolong_add<3, T>), dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gc,
+                       bc, ec, Gf, ef, stop);
+  else
+    hipLaunchKernelGGL((k_mg_prolong_add<2, T>), dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gc,
+                       bc, ec, Gf, ef, stop);
+  check_launch();
 }
 
 void launch_mg_prolong_add(const BoxLevel& Gc, const RobinDev& bc, const double* ec,
-                           const BoxLevel& Gf, double* ef, const int* stop, hipStream_t s) {
+                           const BoxLevel& Gf, double* ef, const int* stop, hipStream_t s,
+                           int fp32) {
+  if (fp32) mg_prolong<float>(Gc, bc, as<float>(ec), Gf, as<float>(ef), stop, s);
+  else mg_prolong<double>(Gc, bc, ec, Gf, ef, stop, s);
+}
+
+template <typename T>
+static void mg_coarsen(const BoxLevel& Gf, const BoxArraysT<T>& Af, const BoxLevel& Gc,
+                       T* const* Dc, T* Cc, hipStream_t s) {
   const long nf = (long)Gf.n[0] * Gf.n[1] * (Gf.dim == 3 ? Gf.n[2] : 1);
   if (Gf.dim == 3)
-    hipLaunchKernelGGL(k_mg_prolong_add<3>, dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gc, bc, ec,
-                       Gf, ef, stop);
+    hipLaunchKernelGGL((k_mg_coarsen_coef<3, T>), dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gf,
+                       Af, Gc, Dc[0], Dc[1], Dc[2], Cc);
   else
-    hipLaunchKernelGGL(k_mg_prolong_add<2>, dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gc, bc, ec,
-                       Gf, ef, stop);
+    hipLaunchKernelGGL((k_mg_coarsen_coef<2, T>), dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gf,
+                       Af, Gc, Dc[0], Dc[1], (T*)nullptr, Cc);
   check_launch();
 }
 
 void launch_mg_coarsen_coef(const BoxLevel& Gf, const BoxArrays& Af, const BoxLevel& Gc,
                             double* const* Dc, double* Cc, hipStream_t s) {
-  const long nf = (long)Gf.n[0] * Gf.n[1] * (Gf.dim == 3 ? Gf.n[2] : 1);
-  if (Gf.dim == 3)
-    hipLaunchKernelGGL(k_mg_coarsen_coef<3>, dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gf, Af, Gc,
-                       Dc[0], Dc[1], Dc[2], Cc);
-  else
-    hipLaunchKernelGGL(k_mg_coarsen_coef<2>, dim3(mg_blocks(nf)), dim3(kThreads), 0, s, Gf, Af, Gc,
-                       Dc[0], Dc[1], nullptr, Cc);
-  check_launch();
+  if (Af.fp32) {
+    float* D32[3] = {as<float>(Dc[0]), as<float>(Dc[1]), as<float>(Dc[2])};
+    mg_coarsen<float>(Gf, typed<float>(Af), Gc, D32, as<float>(Cc), s);
+  } else {
+    mg_coarsen<double>(Gf, typed<double>(Af), Gc, Dc, Cc, s);
+  }
 }
 
 void launch_mg_small_solve(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, double* u,
                            int sweeps, const int* stop, hipStream_t s) {
-  dispatch3<SmallRun>(G, A, bc, u, sweeps, stop, s);
+  if (A.fp32) dispatch3<SmallRun>(G, typed<float>(A), bc, as<float>(u), sweeps, stop, s);
+  else dispatch3<SmallRun>(G, typed<double>(A), bc, u, sweeps, stop, s);
 }
 
 void launch_mg_sumsq(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, const double* u,
                      const double* x, double* partial, const int* stop, hipStream_t s) {
-  dispatch3<SumsqRun>(G, A, bc, u, x, partial, stop, s);
+  if (A.fp32)
+    dispatch3<SumsqRun>(G, typed<float>(A), bc, as<float>(u), as<float>(x), partial, stop, s);
+  else
+    dispatch3<SumsqRun>(G, typed<double>(A), bc, u, x, partial, stop, s);
 }
 
 void launch_mg_check(const double* rr, const double* bb, double tol, int it, int* stop,

@@ -392,6 +392,13 @@ int sfac_solver_fused_tile_timings(sfac_solver* s, double* ms_per_sweep, int max
   });
 }
 
+int sfac_solver_set_coarse_precision(sfac_solver* s, int bits) {
+  return guarded([&] {
+    need(s, "solver");
+    s->s->ops().setCoarsePrecision(bits);
+  });
+}
+
 int sfac_solver_set_algorithm_choice(sfac_solver* s, const char* choice) {
   return guarded([&] {
     need(s, "solver");

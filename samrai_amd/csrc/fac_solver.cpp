@@ -223,6 +223,15 @@ void CellPoissonFACOps::deallocateOperatorState() {
 
 // ---------------------------------------------- single-rectangle coarsest level
 
+void CellPoissonFACOps::setCoarsePrecision(int bits) {
+  if (bits != 32 && bits != 64) fail("CellPoissonFACOps: coarse precision must be 32 or 64");
+  const bool fp32 = bits == 32;
+  if (fp32 == d_coarse_fp32) return;
+  d_coarse_fp32 = fp32;
+  if (d_box.ok) setupBoxPath();
+  ++d_generation;
+}
+
 void CellPoissonFACOps::setFusedCoarsest(bool on, int tile_variant) {
   if (tile_variant < -1 || tile_variant >= sfac::box_tile_variants(d_h.dim()))
     fail("CellPoissonFACOps: tile variant out of range");
@@ -240,10 +249,10 @@ int CellPoissonFACOps::tuneBoxVariant() {
   const int dim = d_h.dim();
   const sfac::BoxLevel& G = B.G;
   const long vol = (long)G.n[0] * G.n[1] * (dim == 3 ? G.n[2] : 1);
-  const std::array<long, 6> key = {dim, G.n[0], G.n[1], G.n[2], d_spec.dIsConstant() ? 0 : 1,
-                                   d_spec.cMode()};
+  const std::array<long, 7> key = {dim, G.n[0], G.n[1], G.n[2], d_spec.dIsConstant() ? 0 : 1,
+                                   d_spec.cMode(), B.fp32};
   static std::mutex mu;
-  static std::map<std::array<long, 6>, std::pair<int, std::vector<float>>> cache;
+  static std::map<std::array<long, 7>, std::pair<int, std::vector<float>>> cache;
   {
     std::lock_guard<std::mutex> lk(mu);
     auto it = cache.find(key);
@@ -254,6 +263,7 @@ int CellPoissonFACOps::tuneBoxVariant() {
   }
   std::vector<float> times;
   sfac::BoxArrays A{};
+  A.fp32 = B.fp32;
   A.dconst = d_spec.getDConstant();
   A.cmode = d_spec.cMode();
   A.cconst = d_spec.getCConstant();
@@ -367,6 +377,9 @@ void CellPoissonFACOps::setupBoxPath() {
     B.G.lp_hi[d] = d < dim ? last.hi[d] - bb.lo[d] : 0;
   }
   B.G.shift = shift;
+  B.fp32 = d_coarse_fp32 ? 1 : 0;
+  // FP64-sized buffers in either precision (fills and gathers see the same
+  // byte extents)
   hip_check(hipMalloc(&B.e[0], sizeof(double) * vol), "hipMalloc(box e)");
   hip_check(hipMalloc(&B.e[1], sizeof(double) * vol), "hipMalloc(box e)");
   hip_check(hipMalloc(&B.r, sizeof(double) * vol), "hipMalloc(box r)");
@@ -395,7 +408,7 @@ void CellPoissonFACOps::setupBoxPath() {
     sfac::launch_fill_int(B.mismatch, 1, 0, d_stream);
     for (int a = 0; a < dim; ++a)
       sfac::launch_box_side(T.geom, D.table(ln, a), D.ghost(), a, B.D[a], B.G, B.mismatch,
-                            d_stream, 1);
+                            d_stream, 1, B.fp32);
     int bad = 0;
     hip_check(hipMemcpyAsync(&bad, B.mismatch, sizeof(int), hipMemcpyDeviceToHost, d_stream),
               "hipMemcpyAsync");
@@ -420,6 +433,7 @@ sfac::BoxArrays CellPoissonFACOps::gatherBox(PatchDataSet& e, const PatchDataSet
   const auto& T = d_h.tables(ln);
   const sfac::BoxLevel& G = B.G;
   sfac::BoxArrays A{};
+  A.fp32 = B.fp32;
   A.dconst = d_spec.getDConstant();
   A.cmode = d_spec.cMode();
   A.cconst = d_spec.getCConstant();
@@ -427,20 +441,20 @@ sfac::BoxArrays CellPoissonFACOps::gatherBox(PatchDataSet& e, const PatchDataSet
     const PatchDataSet& D = *d_spec.getD();
     for (int a = 0; a < dim; ++a) {
       sfac::launch_box_side(T.geom, D.table(ln, a), D.ghost(), a, B.D[a], G, nullptr, d_stream,
-                            0);
+                            0, B.fp32);
       A.D[a] = B.D[a];
     }
   }
   if (A.cmode == 2) {
     const PatchDataSet& Cd = *d_spec.getC();
     sfac::launch_box_cells(T.cells.launch(), T.geom, const_cast<double* const*>(Cd.table(ln)),
-                           Cd.ghost(), B.C, B.C, nullptr, 0, G, 1, d_stream);
+                           Cd.ghost(), B.C, B.C, nullptr, 0, G, 1, d_stream, B.fp32);
     A.C = B.C;
   }
   sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), B.e[0], B.e[0],
-                         nullptr, 0, G, 1, d_stream);
+                         nullptr, 0, G, 1, d_stream, B.fp32);
   sfac::launch_box_cells(T.cells.launch(), T.geom, const_cast<double* const*>(r.table(ln)),
-                         r.ghost(), B.r, B.r, nullptr, 0, G, 1, d_stream);
+                         r.ghost(), B.r, B.r, nullptr, 0, G, 1, d_stream, B.fp32);
   A.r = B.r;
   return A;
 }
@@ -527,6 +541,7 @@ void CellPoissonFACOps::setupMG() {
 sfac::BoxArrays CellPoissonFACOps::mgArrays(int m) const {
   const MGLevel& L = d_mg[m];
   sfac::BoxArrays A{};
+  A.fp32 = d_box.fp32;
   A.dconst = d_spec.getDConstant();
   A.cmode = d_spec.cMode();
   A.cconst = d_spec.getCConstant();
@@ -570,7 +585,7 @@ void CellPoissonFACOps::mgVcycle(int m) {
   MGLevel& C = d_mg[m + 1];
   sfac::launch_mg_restrict_residual(L.G, mgArrays(m), bc, L.e[0], C.G, C.f, d_mg_flag, d_stream);
   mgVcycle(m + 1);
-  sfac::launch_mg_prolong_add(C.G, bc, C.e[0], L.G, L.e[0], d_mg_flag, d_stream);
+  sfac::launch_mg_prolong_add(C.G, bc, C.e[0], L.G, L.e[0], d_mg_flag, d_stream, d_box.fp32);
   mgSweeps(m, kMGPost);
 }
 
@@ -594,18 +609,20 @@ void CellPoissonFACOps::solveBoxMG(PatchDataSet& e, const PatchDataSet& r) {
   sfac::launch_mg_sumsq(d_mg[0].G, mgArrays(0), bc, nullptr, d_mg[0].f, d_mg_partial, d_mg_flag,
                         d_stream);
   sfac::launch_sum(d_mg_partial, P, d_mg_scal, 0, d_stream);
-  sfac::launch_mg_check(d_mg_scal + 1, d_mg_scal, d_coarse_tol, -1, d_mg_flag, d_mg_flag + 1,
+  // FP32 cannot meet relative tolerances near its rounding level
+  const double tol = d_box.fp32 ? std::max(d_coarse_tol, kCoarseFP32Tol) : d_coarse_tol;
+  sfac::launch_mg_check(d_mg_scal + 1, d_mg_scal, tol, -1, d_mg_flag, d_mg_flag + 1,
                         d_mg_scal + 2, d_stream);
   for (int it = 0; it < d_coarse_max; ++it) {
     mgVcycle(0);
     sfac::launch_mg_sumsq(d_mg[0].G, mgArrays(0), bc, d_mg[0].e[0], nullptr, d_mg_partial,
                           d_mg_flag, d_stream);
     sfac::launch_sum(d_mg_partial, P, d_mg_scal + 1, 0, d_stream);
-    sfac::launch_mg_check(d_mg_scal + 1, d_mg_scal, d_coarse_tol, it, d_mg_flag, d_mg_flag + 1,
+    sfac::launch_mg_check(d_mg_scal + 1, d_mg_scal, tol, it, d_mg_flag, d_mg_flag + 1,
                           d_mg_scal + 2, d_stream);
   }
   sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), d_box.e[0],
-                         d_box.e[0], nullptr, 0, d_box.G, 0, d_stream);
+                         d_box.e[0], nullptr, 0, d_box.G, 0, d_stream, d_box.fp32);
   (void)dim;
   if (d_logging) {
     int it = 0;
@@ -667,7 +684,8 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
     K.sweeps = d_stop + 1;
     K.maxres_out = d_scalars + 1;
   }
-  const double launch_bytes = (d_spec.dIsConstant() ? 24.0 : 48.0) * (double)vol;
+  const double launch_bytes =
+      (d_spec.dIsConstant() ? 24.0 : 48.0) * (B.fp32 ? 0.5 : 1.0) * (double)vol;
   const sfac::RobinDev bc = robinDev();
   // timing: one event pair around the back-to-back sweep launches (no host
   // work between them), counted as num_sweeps launches
@@ -700,7 +718,7 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
   // select the buffer holding the result
   sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), B.e[0], B.e[1],
                          (check && num_sweeps > 0) ? d_stop + 1 : nullptr, num_sweeps, G, 0,
-                         d_stream);
+                         d_stream, B.fp32);
   // edges/corners too, as the reference's last NoCoarse fill does
   ghostFill(e, ln, false, true, false);
   if (check && (maxres_out || d_logging)) {

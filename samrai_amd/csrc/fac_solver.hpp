@@ -197,6 +197,14 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   const std::vector<float>& fusedTileTimings() const { return d_box_timings; }
   // V-cycles and final relative residual of the last multigrid coarse solve
   void mgStats(int* iterations, double* relres);
+  // Storage and arithmetic of the single-rectangle coarsest-level solve
+  // (red-black sweeps or multigrid): 64 (default) or 32 bits.  At 32 bits the
+  // error equation's coarse solve runs in FP32 inside the FP64 FAC iteration
+  // (the residuals the outer loop tests stay FP64), and the multigrid
+  // tolerance is floored at kCoarseFP32Tol.
+  void setCoarsePrecision(int bits);
+  int coarsePrecision() const { return d_coarse_fp32 ? 32 : 64; }
+  static constexpr double kCoarseFP32Tol = 1e-6;
 
   // HIP-event timing of the smoother launches (on this stream), with the
   // algorithmic bytes of each launch: a colour pass moves 12 B per cell of
@@ -235,8 +243,10 @@ class CellPoissonFACOps : public FACOperatorStrategy {
     long nslots = 0;
     int* mismatch = nullptr;
     int variant = 0;
+    int fp32 = 0;  // arrays hold float (mixed-precision coarse solve)
   } d_box;
   bool d_box_enabled = true;
+  bool d_coarse_fp32 = false;
   int tuneBoxVariant();
   std::vector<float> d_box_timings;
   sfac::BoxArrays gatherBox(pdat::PatchDataSet& e, const pdat::PatchDataSet& r);

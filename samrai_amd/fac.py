@@ -84,6 +84,7 @@ def lib():
     L.sfac_solver_coarse_mg_stats.argtypes = [_vp, _ip, _dp]
     L.sfac_solver_set_graphs.argtypes = [_vp, C.c_int]
     L.sfac_solver_set_algorithm_choice.argtypes = [_vp, C.c_char_p]
+    L.sfac_solver_set_coarse_precision.argtypes = [_vp, C.c_int]
     L.sfac_fused_tile_variants.argtypes = [C.c_int]
     L.sfac_fused_tile_shape.argtypes = [C.c_int, C.c_int, _ip]
     L.sfac_ops_internal_vector.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
@@ -328,6 +329,11 @@ class CellPoissonFACSolver:
         """FACPreconditioner::setAlgorithmChoice: "default", "mccormick-s4.3"
         or "pernice"."""
         _check(lib().sfac_solver_set_algorithm_choice(self.s, choice.encode()))
+
+    def set_coarse_precision(self, bits):
+        """64 (default) or 32: precision of the coarsest-level solve of the
+        error equation inside the FP64 FAC iteration."""
+        _check(lib().sfac_solver_set_coarse_precision(self.s, int(bits)))
 
     def set_graphs(self, on=True):
         """Capture each FAC cycle as a HIP graph and replay it (same results)."""
