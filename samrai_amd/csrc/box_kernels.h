@@ -60,13 +60,13 @@ struct BoxArraysT {
   double cconst;
 };
 
-// Early-exit test of the sweep before this launch.  prev: per-tile max
-// |residual| over the last patch written by the previous sweep (nullptr for
-// the first sweep or without a tolerance); cur: this sweep's slots.
+// Early-exit test of the sweep before this launch.  slots: three words
+// (nullptr without a tolerance); sweep s raises slot s % 3 to the max
+// |residual| over the last patch (an atomic max of the bit patterns, which
+// order like the values for non-negative doubles), reads slot (s + 2) % 3
+// (the previous sweep's) and clears slot (s + 1) % 3 for the next sweep.
 struct BoxCheck {
-  const double* prev;
-  double* cur;
-  int nlp;              // slots (tiles meeting the last patch)
+  unsigned long long* slots;
   int tlo[3], tn[3];    // tile range of the last patch
   int sweep;            // index of this sweep
   double tol;

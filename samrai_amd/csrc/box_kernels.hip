@@ -3,8 +3,8 @@
 // Per work-group: one TX x TY x TZ tile of owned cells.  LDS holds the tile
 // grown by two cells on every side (clipped to the level, plus the physical
 // ghost layer).  Per launch:
-//   1. early-exit test of the previous sweep (every work-group reduces the
-//      same per-tile maxima, so all of them take the same decision);
+//   1. early-exit test of the previous sweep (every work-group reads the
+//      same max-residual word, so all of them take the same decision);
 //   2. load u; compute the face ghosts of the physical boundary in LDS;
 //   3. red update on the tile grown by one (halo reds are recomputed from the
 //      same inputs as their owning tile, so they are bit-identical);
@@ -164,24 +164,26 @@ __device__ inline double block_max(double v, double* wm) {
 }
 
 // Early-exit test of the previous sweep (see BoxCheck): every work-group
-// reduces the same per-tile maxima, so all take the same decision, and
+// reads the same max-residual word, so all take the same decision, and
 // work-group 0 records it.  Called by all threads; true: skip this sweep.
-__device__ inline bool stop_sweep(const BoxCheck& K, int b, double* wm, int& s_stop) {
+__device__ inline bool stop_sweep(const BoxCheck& K, int b, int& s_stop) {
   if (!K.stop) return false;
-  if (threadIdx.x == 0) s_stop = *K.stop;
-  __syncthreads();
-  if (!s_stop && K.prev) {
-    double m = 0.0;
-    for (int x = threadIdx.x; x < K.nlp; x += kThreads) m = fmax(m, K.prev[x]);
-    m = block_max(m, wm);
-    if (threadIdx.x == 0 && !(m > K.tol)) {
-      s_stop = 1;
-      if (b == 0) {
-        *K.stop = 1;
-        *K.sweeps = K.sweep;
-        if (K.maxres_out) *K.maxres_out = m;
+  if (threadIdx.x == 0) {
+    int stop = *K.stop;
+    if (!stop && K.slots && K.sweep > 0) {
+      const double m = __longlong_as_double((long long)K.slots[(K.sweep + 2) % 3]);
+      if (!(m > K.tol)) {
+        stop = 1;
+        if (b == 0) {
+          *K.stop = 1;
+          *K.sweeps = K.sweep;
+          if (K.maxres_out) *K.maxres_out = m;
+        }
       }
     }
+    // the slot the next sweep raises (its last reader was the previous sweep)
+    if (!stop && K.slots && b == 0) K.slots[(K.sweep + 1) % 3] = 0ull;
+    s_stop = stop;
   }
   __syncthreads();
   return s_stop != 0;
@@ -207,7 +209,7 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
   const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
 
   // 1. early exit
-  if (stop_sweep(K, b, wm, s_stop)) return;
+  if (stop_sweep(K, b, s_stop)) return;
 
   // colour parity: red <=> (ilev + jlev + klev + shift) even; in LDS
   // coordinates the sum is base + lx + ly + lz
@@ -256,7 +258,7 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
   }
   // the cells this thread updates: LDS index, flags (1 update, 2 counts
   // for the early-exit maximum) and coefficients
-  const bool lp_tile = K.cur != nullptr;
+  const bool lp_tile = K.slots != nullptr;
   auto in_lp = [&](int i, int j, int k) {
     return i >= G.lp_lo[0] && i <= G.lp_hi[0] && j >= G.lp_lo[1] && j <= G.lp_hi[1] &&
            (DIM == 2 || (k >= G.lp_lo[2] && k <= G.lp_hi[2]));
@@ -367,17 +369,15 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
                        ti[2] >= 0 && ti[2] < K.tn[2];
     if (meets) {  // uniform over the work-group
       mr = block_max(mr, wm);
-      if (threadIdx.x == 0) K.cur[ti[0] + K.tn[0] * (ti[1] + K.tn[1] * ti[2])] = mr;
+      if (threadIdx.x == 0)
+        atomicMax(K.slots + K.sweep % 3, (unsigned long long)__double_as_longlong(mr));
     }
   }
 }
 
 __global__ void __launch_bounds__(kThreads) k_box_final_check(BoxCheck K, int nsweeps) {
-  __shared__ double wm[kThreads / 64];
   if (*K.stop) return;  // stopped early: already recorded
-  double m = 0.0;
-  for (int x = threadIdx.x; x < K.nlp; x += kThreads) m = fmax(m, K.cur[x]);
-  m = block_max(m, wm);
+  const double m = __longlong_as_double((long long)K.slots[(nsweeps - 1) % 3]);
   if (threadIdx.x == 0) {
     *K.sweeps = nsweeps;
     if (K.maxres_out) *K.maxres_out = m;

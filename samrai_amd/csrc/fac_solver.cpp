@@ -383,16 +383,8 @@ void CellPoissonFACOps::setupBoxPath() {
   hip_check(hipMalloc(&B.e[0], sizeof(double) * vol), "hipMalloc(box e)");
   hip_check(hipMalloc(&B.e[1], sizeof(double) * vol), "hipMalloc(box e)");
   hip_check(hipMalloc(&B.r, sizeof(double) * vol), "hipMalloc(box r)");
-  // per-tile early-exit slots, double buffered, sized for the smallest tile
-  long maxtiles = 1;
-  for (int v = 0; v < sfac::box_tile_variants(dim); ++v) {
-    sfac::BoxTile t = sfac::box_tile(dim, v);
-    long nt = ((n[0] + t.tx - 1) / t.tx) * ((n[1] + t.ty - 1) / t.ty) *
-              (dim == 3 ? (n[2] + t.tz - 1) / t.tz : 1);
-    maxtiles = std::max(maxtiles, nt);
-  }
-  B.nslots = maxtiles;
-  hip_check(hipMalloc(&B.slots, sizeof(double) * 2 * maxtiles), "hipMalloc(box slots)");
+  // early-exit maxima of three consecutive sweeps (BoxCheck)
+  hip_check(hipMalloc(&B.slots, sizeof(double) * 4), "hipMalloc(box slots)");
   hip_check(hipMalloc(&B.mismatch, sizeof(int)), "hipMalloc(box flag)");
   if (d_spec.cMode() == 2) hip_check(hipMalloc(&B.C, sizeof(double) * vol), "hipMalloc(box C)");
   B.ok = true;
@@ -665,7 +657,6 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
   sfac::BoxCheck K{};
   const sfac::BoxTile tile = sfac::box_tile(dim, B.variant);
   const int tsz[3] = {tile.tx, tile.ty, dim == 3 ? tile.tz : 1};
-  K.nlp = 1;
   for (int d = 0; d < 3; ++d) {
     if (d < dim) {
       K.tlo[d] = G.lp_lo[d] / tsz[d];
@@ -674,12 +665,12 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
       K.tlo[d] = 0;
       K.tn[d] = 1;
     }
-    K.nlp *= K.tn[d];
   }
   K.tol = tol;
   const bool check = tol >= 0.0;
   if (check) {
     sfac::launch_fill_int(d_stop, 2, 0, d_stream);
+    sfac::launch_fill(B.slots, 3, 0.0, d_stream);  // the first sweep's slot
     K.stop = d_stop;
     K.sweeps = d_stop + 1;
     K.maxres_out = d_scalars + 1;
@@ -698,8 +689,7 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
     K.sweep = s;
     A.ein = B.e[s & 1];
     A.eout = B.e[(s + 1) & 1];
-    K.prev = (check && s > 0) ? B.slots + (long)((s - 1) & 1) * B.nslots : nullptr;
-    K.cur = check ? B.slots + (long)(s & 1) * B.nslots : nullptr;
+    K.slots = check ? reinterpret_cast<unsigned long long*>(B.slots) : nullptr;
     sfac::launch_box_rbgs(G, A, bc, 1, K, B.variant, d_stream);
   }
   if (d_timing && num_sweeps > 0) {
@@ -711,7 +701,6 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
     d_timed.push_back(tl);
   }
   if (check && num_sweeps > 0) {
-    K.cur = B.slots + (long)((num_sweeps - 1) & 1) * B.nslots;
     sfac::launch_box_final_check(K, num_sweeps, d_stream);
   }
   // the sweeps done (known on the device when the loop may stop early)

@@ -240,7 +240,6 @@ class CellPoissonFACOps : public FACOperatorStrategy {
     double* D[3] = {nullptr, nullptr, nullptr};
     double* C = nullptr;
     double* slots = nullptr;
-    long nslots = 0;
     int* mismatch = nullptr;
     int variant = 0;
     int fp32 = 0;  // arrays hold float (mixed-precision coarse solve)
