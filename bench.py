@@ -120,6 +120,20 @@ def setup_c2(fac, decks, torch, params, seed=0):
     return deck, h, u, f, D, s
 
 
+def measured_traffic(tile):
+    """L2-miss bytes per launch of the fused sweep with this tile shape on the
+    C1 coarsest level, from the committed counter summary (None if absent)."""
+    path = os.path.join(ROOT, "profiles", "k_box_rbgs_c1_hbm_traffic.txt")
+    if tile is None or not os.path.exists(path):
+        return None
+    key = "x".join(map(str, tile))
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == key:
+            return int(f[3])
+    return None
+
+
 def level_errors(deck, ua, ub):
     """Per-level RMS and max |ua - ub| over interior cells (ub: FP64 reference)."""
     out = []
@@ -165,9 +179,15 @@ def main():
     ap.add_argument("--smoother-sweeps", type=int, default=200)
     ap.add_argument("--no-c2", action="store_true",
                     help="skip the variable-coefficient 4-level deck (C2) section")
-    ap.add_argument("--mode", choices=["solve", "smoother"], default="solve",
+    ap.add_argument("--mode", choices=["solve", "smoother", "profile"], default="solve",
                     help="smoother: only a few smoother launches and a 256 MiB copy of known "
-                         "size (for hardware-counter passes); prints no benchmark line")
+                         "size (for hardware-counter passes); profile: only the solves of "
+                         "--deck with --coarse (for kernel traces); neither prints the "
+                         "benchmark line")
+    ap.add_argument("--deck", choices=["c1", "c2"], default="c1",
+                    help="deck of --mode profile")
+    ap.add_argument("--coarse", choices=["redblack", "mg"], default="redblack",
+                    help="coarsest-level solver of --mode profile")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -179,6 +199,22 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+    if args.mode == "profile":
+        P = MG_PARAMS if args.coarse == "mg" else PARAMS
+        if args.deck == "c2":
+            deck, h, u, f, D, s = setup_c2(fac, decks, torch, P)
+        else:
+            deck = decks.deck_3d_c1()
+            h, u, f, s = build_problem(fac, decks, deck, P)
+        for k in range(args.warmup + args.steps):
+            u.fill(0.0)
+            t0 = time.perf_counter()
+            s.solve_system(u, f)
+            s.synchronize()
+            print("bench: profile %s %s solve %d: %d cycles, %.4f s" %
+                  (args.deck, args.coarse, k, s.number_of_iterations(),
+                   time.perf_counter() - t0), file=sys.stderr, flush=True)
+        return
     deck = decks.deck_3d_c1()
     ncells = decks.count_cells(deck)
     npatch = sum(len(l) for l in deck["levels"])
@@ -264,8 +300,12 @@ def main():
                            achieved_GBs=round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else 0.0))
     dom = max(levels, key=lambda d: d["total_ms"])
     achieved = dom["achieved_GBs"]
+    traffic = measured_traffic(tile) if dom["level"] == 0 and fused else None
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                    traffic_source=("FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch "
+                                    "(profiles/k_box_rbgs_c1_hbm_traffic.txt)"
+                                    if traffic else None),
                     kernel="%s, level %d" % (dom["kernel"], dom["level"]),
                     launches=dom["launches"], avg_launch_us=dom["avg_launch_us"],
                     bytes_per_launch=dom["bytes_per_launch"], per_level=levels)
