@@ -51,7 +51,30 @@ def case_2d_odd_split():
     return dict(name="2doddsplit", dim=2, levels=[L0, L1], ratio=2, dlo=(0, 0), dhi=(15, 15))
 
 
-ALL = [case_2d, case_3d, case_3d_boundary]
+def case_2d_ratio4():
+    """Refinement ratio 4, a fine patch two coarse cells wide (8 fine cells)
+    next to a wider one."""
+    L0 = [((0, 0), (3, 7)), ((4, 0), (7, 7))]
+    L1 = [((8, 8), (19, 23)), ((20, 8), (27, 15))]
+    return dict(name="2dratio4", dim=2, levels=[L0, L1], ratio=4, dlo=(0, 0), dhi=(7, 7))
+
+
+def case_3d_ratio4():
+    """3-D at refinement ratio 4 over a patch split level 0."""
+    L0 = [((0, 0, 0), (3, 7, 7)), ((4, 0, 0), (7, 7, 7))]
+    L1 = [((8, 8, 8), (23, 19, 27))]
+    return dict(name="3dratio4", dim=3, levels=[L0, L1], ratio=4, dlo=(0, 0, 0), dhi=(7, 7, 7))
+
+
+def case_2d_thin():
+    """A fine patch one coarse cell wide and a fine patch touching two
+    physical boundaries (C/F and physical ghosts at the same corner)."""
+    L0 = [((0, 0), (15, 15))]
+    L1 = [((10, 8), (11, 23)), ((0, 0), (7, 5))]
+    return dict(name="2dthin", dim=2, levels=[L0, L1], ratio=2, dlo=(0, 0), dhi=(15, 15))
+
+
+ALL = [case_2d, case_3d, case_3d_boundary, case_2d_ratio4, case_3d_ratio4, case_2d_thin]
 
 
 def cell_shape(box, g, dim, axis=-1):
