@@ -390,7 +390,8 @@ __global__ void __launch_bounds__(kThreads)
 k_box_cells(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* x, int g,
             T* box0, T* box1, const int* count, int count_host, BoxLevel G, int to_box) {
   T* box = ((count ? *count : count_host) & 1) ? box1 : box0;
-  const Chunk ch = chunks[blockIdx.x];
+  const Chunk ch = chunks[xcd_chunk()];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const Box pb = L.box[it.p];
   const View v = make_view(pb.lo, pb.hi, g, L.dim);
@@ -813,11 +814,11 @@ void launch_box_cells(const GhostLaunch& cells, const LevelGeom& L, double* cons
                       const BoxLevel& G, int to_box, hipStream_t s, int fp32) {
   if (cells.nchunks == 0) return;
   if (fp32)
-    hipLaunchKernelGGL(k_box_cells<float>, dim3(cells.nchunks), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_box_cells<float>, dim3(xcd_grid(cells.nchunks)), dim3(kThreads), 0, s,
                        cells.items, cells.chunks, L, x, g, reinterpret_cast<float*>(box0),
                        reinterpret_cast<float*>(box1), count, count_host, G, to_box);
   else
-    hipLaunchKernelGGL(k_box_cells<double>, dim3(cells.nchunks), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL(k_box_cells<double>, dim3(xcd_grid(cells.nchunks)), dim3(kThreads), 0, s,
                        cells.items, cells.chunks, L, x, g, box0, box1, count, count_host, G,
                        to_box);
   check_launch();

@@ -203,10 +203,12 @@ DevList PatchHierarchy::uploadList(const std::vector<BoxItem>& items, bool colou
     }
     for (long s = 0; s < n; s += sfac::kCellsPerChunk) chunks.push_back(Chunk{(int)i, (int)s});
   }
+  L.nchunks = (int)chunks.size();
+  // pad to the XCD-aware launch grid (level_device.h); item -1 returns at once
+  while (chunks.size() % 8) chunks.push_back(Chunk{-1, 0});
   L.items = upload(items);
   L.chunks = upload(chunks);
   L.nitems = (int)items.size();
-  L.nchunks = (int)chunks.size();
   return L;
 }
 

@@ -14,6 +14,17 @@ namespace sfac {
 constexpr int kThreads = 256;       // work-group size of every level kernel
 constexpr int kCellsPerChunk = 512; // cells a work-group handles per chunk
 
+// Chunk tables are padded with item -1 to a multiple of 8 entries and
+// launched with one work-group per entry in an XCD-aware order: work-groups
+// are dealt round-robin to the 8 XCDs, so XCD x runs the consecutive chunks
+// [x*per, (x+1)*per) and neighbouring rows and planes of a patch share that
+// XCD's L2.
+inline int xcd_grid(int nchunks) { return 8 * ((nchunks + 7) / 8); }
+__device__ inline int xcd_chunk() {
+  const int per = (int)(gridDim.x >> 3);
+  return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
 // One rectangular piece of work: destination patch p, source patch q,
 // index box [lo, hi] (in the index space the kernel documents) and a tag.
 struct BoxItem {

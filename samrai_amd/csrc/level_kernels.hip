@@ -194,7 +194,9 @@ k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* co
               RobinDev bc, const int* stop, SweepCheck chk, CfRefine cfr) {
   if (stop && *stop) return;
   if (chk.stop && blockIdx.x == 0) sweep_check(chk);
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const int kind = it.tag & 15, loc = it.tag >> 4;
   const Box b = L.box[it.p];
@@ -280,7 +282,9 @@ k_rbgs(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* e,
        const double* const* rhs, int rg, PoissonDev ps, int colour, double* maxres,
        const int* stop) {
   if (stop && *stop) return;
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const int p = items[ch.item].p;
   const Box b = L.box[p];
   const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
@@ -392,7 +396,7 @@ k_rbgs(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* e,
     if (threadIdx.x == 0) {
       double v = wm[0];
       for (int w = 1; w < kThreads / 64; ++w) v = fmax(v, wm[w]);
-      maxres[blockIdx.x] = v;
+      maxres[ci] = v;
     }
   }
 }
@@ -403,7 +407,9 @@ template <int DIM, bool DVAR>
 __global__ void __launch_bounds__(kThreads)
 k_flux(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* flux,
        const double* const* uu, int ug, PoissonDev ps, int ewing) {
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const int p = it.p;
   const Box b = L.box[p];
@@ -452,7 +458,9 @@ __global__ void __launch_bounds__(kThreads)
 k_ewing_faces(const BoxItem* items, const Chunk* chunks, LevelGeom L, const double* const* uu,
               int ug, PoissonDev ps, const int* stop) {
   if (stop && *stop) return;
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const int p = it.p, face = it.q, a = face >> 1, side = face & 1;
   const Box b = L.box[p];
@@ -502,7 +510,9 @@ __global__ void __launch_bounds__(kThreads)
 k_residual(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* rr, int rg,
            const double* const* ff, int fg, const double* const* uu, int ug,
            const double* const* flux, PoissonDev ps, double* const* oflux) {
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const int p = it.p;
   const Box b = L.box[p];
@@ -552,7 +562,9 @@ template <int DIM>
 __global__ void __launch_bounds__(kThreads)
 k_flux_coarsen(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
                double* const* cflux, const double* const* foflux) {
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const int a = it.tag & 3, side = it.tag >> 2;
   const Box cb = C.box[it.p], fb = Fl.box[it.q];
@@ -591,7 +603,9 @@ template <int DIM>
 __global__ void __launch_bounds__(kThreads)
 k_restrict(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
            double* const* dst, int dg, const double* const* src, int sg) {
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const View vc = vview(C.box[it.p], dg, DIM), vf = vview(Fl.box[it.q], sg, DIM);
   const double* x = src[it.q];
@@ -615,7 +629,9 @@ template <int DIM>
 __global__ void __launch_bounds__(kThreads)
 k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
               const double* const* ce, int cg, double* const* fe, int fg, int method) {
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const View vc = vview(C.box[it.q], cg, DIM), vf = vview(Fl.box[it.p], fg, DIM);
   const double* uc_ = ce[it.q];
@@ -636,7 +652,9 @@ __global__ void __launch_bounds__(kThreads)
 k_dot_partial(const BoxItem* items, const Chunk* chunks, LevelGeom L, const double* const* xx,
               int xg, const double* const* yy, int yg, const double* const* ww,
               double* partial) {
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const Box b = L.box[it.p];
   const View vx = vview(b, xg, DIM), vy = vview(b, yg, DIM), vw = vview(b, 0, DIM);
@@ -658,7 +676,7 @@ k_dot_partial(const BoxItem* items, const Chunk* chunks, LevelGeom L, const doub
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int w = 0; w < kThreads / 64; ++w) s += sm[w];
-    partial[blockIdx.x] = s;
+    partial[ci] = s;
   }
 }
 
@@ -681,7 +699,9 @@ template <int DIM>
 __global__ void __launch_bounds__(kThreads)
 k_axpy(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* yy, int yg,
        const double* const* xx, int xg) {
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const Box b = L.box[it.p];
   const View vx = vview(b, xg, DIM), vy = vview(b, yg, DIM);
@@ -705,7 +725,9 @@ __global__ void k_fill(double* p, long n, double v) {
 __global__ void __launch_bounds__(kThreads)
 k_set_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* dst, int g,
             double v) {
-  const Chunk ch = chunks[blockIdx.x];
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
   const BoxItem it = items[ch.item];
   const View vd = vview(L.box[it.p], g, L.dim);
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
@@ -737,7 +759,7 @@ void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const*
     if (check) launch_sweep_check(*check, s);
     return;
   }
-  hipLaunchKernelGGL(k_ghost_items, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L,
+  hipLaunchKernelGGL(k_ghost_items, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items, w.chunks, L,
                      dst, dg, coarse, cg, coarse_box, homogeneous, bc, stop, chk, cfr);
   check_launch();
 }
@@ -747,10 +769,10 @@ static void rbgs_dispatch_e(const SmoothLaunch& w, const LevelGeom& L, double* c
                             const double* const* rhs, int rg, const PoissonDev& ps, int ewing,
                             int colour, double* maxres, const int* stop, hipStream_t s) {
   if (ewing)
-    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, CMODE, true>), dim3(w.nchunks), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, CMODE, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
                        w.items, w.chunks, L, e, eg, rhs, rg, ps, colour, maxres, stop);
   else
-    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, CMODE, false>), dim3(w.nchunks), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL((k_rbgs<DIM, DVAR, CMODE, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
                        w.items, w.chunks, L, e, eg, rhs, rg, ps, colour, maxres, stop);
 }
 
@@ -788,17 +810,17 @@ void launch_flux(const GhostLaunch& w, const LevelGeom& L, double* const* flux,
   bool dvar = ps.D[0] != nullptr;
   if (L.dim == 3) {
     if (dvar)
-      hipLaunchKernelGGL((k_flux<3, true>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+      hipLaunchKernelGGL((k_flux<3, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                          w.chunks, L, flux, u, ug, ps, ewing);
     else
-      hipLaunchKernelGGL((k_flux<3, false>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+      hipLaunchKernelGGL((k_flux<3, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                          w.chunks, L, flux, u, ug, ps, ewing);
   } else {
     if (dvar)
-      hipLaunchKernelGGL((k_flux<2, true>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+      hipLaunchKernelGGL((k_flux<2, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                          w.chunks, L, flux, u, ug, ps, ewing);
     else
-      hipLaunchKernelGGL((k_flux<2, false>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+      hipLaunchKernelGGL((k_flux<2, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                          w.chunks, L, flux, u, ug, ps, ewing);
   }
   check_launch();
@@ -810,17 +832,17 @@ void launch_ewing_faces(const GhostLaunch& w, const LevelGeom& L, const double* 
   bool dvar = ps.D[0] != nullptr;
   if (L.dim == 3) {
     if (dvar)
-      hipLaunchKernelGGL((k_ewing_faces<3, true>), dim3(w.nchunks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_ewing_faces<3, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
                          w.items, w.chunks, L, u, ug, ps, stop);
     else
-      hipLaunchKernelGGL((k_ewing_faces<3, false>), dim3(w.nchunks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_ewing_faces<3, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
                          w.items, w.chunks, L, u, ug, ps, stop);
   } else {
     if (dvar)
-      hipLaunchKernelGGL((k_ewing_faces<2, true>), dim3(w.nchunks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_ewing_faces<2, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
                          w.items, w.chunks, L, u, ug, ps, stop);
     else
-      hipLaunchKernelGGL((k_ewing_faces<2, false>), dim3(w.nchunks), dim3(kThreads), 0, s,
+      hipLaunchKernelGGL((k_ewing_faces<2, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
                          w.items, w.chunks, L, u, ug, ps, stop);
   }
   check_launch();
@@ -830,10 +852,10 @@ void launch_flux_coarsen(const GhostLaunch& w, const LevelGeom& coarse, const Le
                          double* const* cflux, const double* const* foflux, hipStream_t s) {
   if (w.nchunks == 0) return;
   if (coarse.dim == 3)
-    hipLaunchKernelGGL(k_flux_coarsen<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL(k_flux_coarsen<3>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, coarse, fine, cflux, foflux);
   else
-    hipLaunchKernelGGL(k_flux_coarsen<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL(k_flux_coarsen<2>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, coarse, fine, cflux, foflux);
   check_launch();
 }
@@ -844,13 +866,13 @@ static void residual_dispatch(const GhostLaunch& w, const LevelGeom& L, double* 
                               const double* const* flux, const PoissonDev& ps,
                               double* const* oflux, hipStream_t s) {
   if (ps.cmode == 0)
-    hipLaunchKernelGGL((k_residual<DIM, 0>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL((k_residual<DIM, 0>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
   else if (ps.cmode == 1)
-    hipLaunchKernelGGL((k_residual<DIM, 1>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL((k_residual<DIM, 1>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
   else
-    hipLaunchKernelGGL((k_residual<DIM, 2>), dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL((k_residual<DIM, 2>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
 }
 
@@ -869,10 +891,10 @@ void launch_restrict(const GhostLaunch& w, const LevelGeom& coarse, const LevelG
                      hipStream_t s) {
   if (w.nchunks == 0) return;
   if (coarse.dim == 3)
-    hipLaunchKernelGGL(k_restrict<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks,
+    hipLaunchKernelGGL(k_restrict<3>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items, w.chunks,
                        coarse, fine, dst, dg, src, sg);
   else
-    hipLaunchKernelGGL(k_restrict<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks,
+    hipLaunchKernelGGL(k_restrict<2>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items, w.chunks,
                        coarse, fine, dst, dg, src, sg);
   check_launch();
 }
@@ -882,10 +904,10 @@ void launch_prolong_add(const GhostLaunch& w, const LevelGeom& coarse, const Lev
                         int method, hipStream_t s) {
   if (w.nchunks == 0) return;
   if (coarse.dim == 3)
-    hipLaunchKernelGGL(k_prolong_add<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL(k_prolong_add<3>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, coarse, fine, ce, cg, fe, fg, method);
   else
-    hipLaunchKernelGGL(k_prolong_add<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL(k_prolong_add<2>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, coarse, fine, ce, cg, fe, fg, method);
   check_launch();
 }
@@ -895,10 +917,10 @@ void launch_dot_partial(const GhostLaunch& w, const LevelGeom& L, const double* 
                         double* partial, hipStream_t s) {
   if (w.nchunks == 0) return;
   if (L.dim == 3)
-    hipLaunchKernelGGL(k_dot_partial<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL(k_dot_partial<3>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, L, x, xg, y, yg, wts, partial);
   else
-    hipLaunchKernelGGL(k_dot_partial<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items,
+    hipLaunchKernelGGL(k_dot_partial<2>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
                        w.chunks, L, x, xg, y, yg, wts, partial);
   check_launch();
 }
@@ -912,10 +934,10 @@ void launch_axpy(const GhostLaunch& w, const LevelGeom& L, double* const* y, int
                  const double* const* x, int xg, hipStream_t s) {
   if (w.nchunks == 0) return;
   if (L.dim == 3)
-    hipLaunchKernelGGL(k_axpy<3>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L, y,
+    hipLaunchKernelGGL(k_axpy<3>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items, w.chunks, L, y,
                        yg, x, xg);
   else
-    hipLaunchKernelGGL(k_axpy<2>, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L, y,
+    hipLaunchKernelGGL(k_axpy<2>, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items, w.chunks, L, y,
                        yg, x, xg);
   check_launch();
 }
@@ -936,7 +958,7 @@ void launch_fill_int(int* p, int n, int v, hipStream_t s) {
 void launch_set_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int g,
                       double v, hipStream_t s) {
   if (w.nchunks == 0) return;
-  hipLaunchKernelGGL(k_set_items, dim3(w.nchunks), dim3(kThreads), 0, s, w.items, w.chunks, L,
+  hipLaunchKernelGGL(k_set_items, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items, w.chunks, L,
                      dst, g, v);
   check_launch();
 }
