@@ -193,7 +193,7 @@ DevList PatchHierarchy::uploadList(const std::vector<BoxItem>& items, bool colou
       int nx = it.hi[0] - it.lo[0] + 1;
       int half = (nx + 1) / 2;
       if (half > sfac::kThreads) fail("PatchHierarchy: patch wider than 512 cells in x");
-      int rpc = std::max(1, sfac::kCellsPerChunk / half);
+      int rpc = std::max(1, sfac::kColourCellsPerChunk / half);
       long rows = (long)(it.hi[1] - it.lo[1] + 1) * (it.hi[2] - it.lo[2] + 1);
       for (long s = 0; s < rows; s += rpc) chunks.push_back(Chunk{(int)i, (int)s});
       continue;

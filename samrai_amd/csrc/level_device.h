@@ -12,7 +12,10 @@
 namespace sfac {
 
 constexpr int kThreads = 256;       // work-group size of every level kernel
-constexpr int kCellsPerChunk = 512; // cells a work-group handles per chunk
+constexpr int kCellsPerChunk = 256; // cells a work-group handles per chunk
+// colour-space chunks of the red-black colour pass: whole rows, about this
+// many cells (both colours) per chunk
+constexpr int kColourCellsPerChunk = 512;
 
 // Chunk tables are padded with item -1 to a multiple of 8 entries and
 // launched with one work-group per entry in an XCD-aware order: work-groups

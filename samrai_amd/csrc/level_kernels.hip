@@ -292,7 +292,7 @@ k_rbgs(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* e,
   const int half = (nx + 1) >> 1;
   const int rows_per_iter = kThreads / half;  // half <= kThreads (checked on the host)
   const int nrow_total = ny * nz;
-  const int rpc = kCellsPerChunk / half > 0 ? kCellsPerChunk / half : 1;
+  const int rpc = kColourCellsPerChunk / half > 0 ? kColourCellsPerChunk / half : 1;
   const int row_end = min(ch.start + rpc, nrow_total);
   // array strides (ghost widths eg, rg; side arrays get +1 on their axis)
   const int us1 = nx + 2 * eg, us2 = us1 * (ny + 2 * eg);
