@@ -293,6 +293,19 @@ int sfac_solver_coarse_mg_stats(sfac_solver* s, int* iterations, double* relres)
  * .C:678-763), case-insensitive; any other name is an error. */
 int sfac_solver_set_algorithm_choice(sfac_solver* s, const char* choice);
 
+/* Berger-Rigoutsos clustering for building levels (mesh::BergerRigoutsos,
+ * BergerRigoutsosNode.C:725-760 recombination, :1251-1325 minimal box,
+ * :1326-1630 accept or split, :1636-1716 zero-swath cut, :1718-1812
+ * inflection cut), one process: boxes covering the nonzero cells of tags
+ * (one byte per cell of [lo, hi], first index fastest).  Boxes larger than
+ * max_box in some direction are split; none is cut below min_box.  On
+ * return *nboxes holds the count; boxes (capacity max_boxes, 2*dim ints per
+ * box: lo..., hi...) may be NULL to query it.  Host function. */
+int sfac_berger_rigoutsos(int dim, const int* lo, const int* hi, const unsigned char* tags,
+                          double efficiency_tolerance, double combine_efficiency,
+                          const int* min_box, const int* max_box, int* boxes, int max_boxes,
+                          int* nboxes);
+
 /* Mixed precision, no reference counterpart: bits = 32 runs the coarsest-level
  * solve of the error equation (red-black sweeps or multigrid on a coarsest
  * level that tiles the domain) in FP32 storage and arithmetic; the FAC

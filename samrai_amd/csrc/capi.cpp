@@ -3,6 +3,7 @@
 #include <memory>
 #include <string>
 
+#include "clustering.hpp"
 #include "fac_solver.hpp"
 #include "samrai_fac.h"
 
@@ -389,6 +390,23 @@ int sfac_solver_fused_tile_timings(sfac_solver* s, double* ms_per_sweep, int max
     const auto& t = s->s->ops().fusedTileTimings();
     *n = (int)t.size();
     for (int i = 0; i < (int)t.size() && i < max; ++i) ms_per_sweep[i] = t[i];
+  });
+}
+
+int sfac_berger_rigoutsos(int dim, const int* lo, const int* hi, const unsigned char* tags,
+                          double efficiency_tolerance, double combine_efficiency,
+                          const int* min_box, const int* max_box, int* boxes, int max_boxes,
+                          int* nboxes) {
+  return guarded([&] {
+    need(nboxes, "nboxes");
+    std::vector<int> b = mesh::bergerRigoutsos(dim, lo, hi, tags, efficiency_tolerance,
+                                               combine_efficiency, min_box, max_box);
+    const int n = (int)(b.size() / (2 * dim));
+    *nboxes = n;
+    if (boxes) {
+      if (n > max_boxes) fail("sfac_berger_rigoutsos: more boxes than max_boxes");
+      std::copy(b.begin(), b.end(), boxes);
+    }
   });
 }
 

@@ -91,9 +91,38 @@ def _greedy_boxes(blocks, maxb):
     return out
 
 
+def _carve(box, allowed, tags):
+    """Pieces of box lying in the allowed mask that hold every tag of box:
+    the box itself when it is allowed throughout, else its two halves
+    (longest side) carved in turn; pieces without tags are dropped."""
+    lo, hi = box
+    sl = tuple(slice(l, h + 1) for l, h in zip(lo, hi))
+    if not tags[sl].any():
+        return []
+    if allowed[sl].all():
+        return [box]
+    ext = [h - l + 1 for l, h in zip(lo, hi)]
+    d = int(np.argmax(ext))
+    mid = lo[d] + ext[d] // 2
+    hi1 = list(hi)
+    hi1[d] = mid - 1
+    lo2 = list(lo)
+    lo2[d] = mid
+    return _carve((lo, tuple(hi1)), allowed, tags) + _carve((tuple(lo2), hi), allowed, tags)
+
+
 def deck_3d_c1(base=128, nlevels=3, percentile=70.0, min_size=16, max_size=64,
-               base_patch=64):
-    """The 3-D plain-FP64 deck (sine solution, D = 1)."""
+               base_patch=64, clustering="berger_rigoutsos"):
+    """The 3-D plain-FP64 deck (sine solution, D = 1).
+
+    Level ln+1 is built on level ln's index space: cells where |grad u|
+    exceeds the percentile (over the cells level ln covers) are tagged inside
+    the proper-nesting region (level ln's cover eroded by one cell), the tags
+    are clustered by Berger-Rigoutsos (efficiency and combine tolerances 0.8,
+    boxes of min_size / r .. max_size / r coarse cells; the library's
+    sfac_berger_rigoutsos), boxes leaving the nesting region are carved to
+    it, and the boxes are refined by r.  clustering="greedy" instead tiles
+    tagged min_size blocks into boxes (no library call)."""
     r = 2
     levels = [[((lo0, lo1, lo2), (hi0, hi1, hi2))
                for (lo2, hi2) in _split(base, base_patch)
@@ -108,12 +137,21 @@ def deck_3d_c1(base=128, nlevels=3, percentile=70.0, min_size=16, max_size=64,
         thr = np.percentile(g[cover], percentile)
         tags = (g > thr) & cover
         allowed = _erode1(cover)
-        nb = n // blk
-        tb = tags.reshape(nb, blk, nb, blk, nb, blk).any(axis=(1, 3, 5))
-        eb = allowed.reshape(nb, blk, nb, blk, nb, blk).all(axis=(1, 3, 5))
-        bx = _greedy_boxes(tb & eb, maxb)
-        fine = [(tuple(c * min_size for c in lo), tuple((c + 1) * min_size - 1 for c in hi))
-                for lo, hi in bx]
+        if clustering == "berger_rigoutsos":
+            from samrai_amd import fac
+            tags &= allowed
+            bx = fac.berger_rigoutsos(tags, (0, 0, 0), 0.8, 0.8, (blk,) * 3,
+                                      (max_size // r,) * 3)
+            coarse = [p for b in bx for p in _carve(b, allowed, tags)]
+            fine = [(tuple(c * r for c in lo), tuple((c + 1) * r - 1 for c in hi))
+                    for lo, hi in coarse]
+        else:
+            nb = n // blk
+            tb = tags.reshape(nb, blk, nb, blk, nb, blk).any(axis=(1, 3, 5))
+            eb = allowed.reshape(nb, blk, nb, blk, nb, blk).all(axis=(1, 3, 5))
+            bx = _greedy_boxes(tb & eb, maxb)
+            fine = [(tuple(c * min_size for c in lo), tuple((c + 1) * min_size - 1 for c in hi))
+                    for lo, hi in bx]
         levels.append(fine)
         n *= r
         cover = _cover_mask(fine, n)

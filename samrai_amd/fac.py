@@ -85,6 +85,8 @@ def lib():
     L.sfac_solver_set_graphs.argtypes = [_vp, C.c_int]
     L.sfac_solver_set_algorithm_choice.argtypes = [_vp, C.c_char_p]
     L.sfac_solver_set_coarse_precision.argtypes = [_vp, C.c_int]
+    L.sfac_berger_rigoutsos.argtypes = [C.c_int, _ip, _ip, C.c_void_p, C.c_double, C.c_double,
+                                        _ip, _ip, _ip, C.c_int, _ip]
     L.sfac_fused_tile_variants.argtypes = [C.c_int]
     L.sfac_fused_tile_shape.argtypes = [C.c_int, C.c_int, _ip]
     L.sfac_ops_internal_vector.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
@@ -428,3 +430,25 @@ def fused_tile_shapes(dim):
         _check(L.sfac_fused_tile_shape(dim, v, a))
         out.append(tuple(a))
     return out
+
+
+def berger_rigoutsos(tags, lo=None, efficiency_tolerance=0.8, combine_efficiency=0.8,
+                     min_box=None, max_box=None):
+    """mesh::BergerRigoutsos clustering (one process) of a boolean tag array
+    indexed [i, j(, k)] whose cell (0, 0(, 0)) has index lo: boxes
+    ((lo...), (hi...)) covering every tagged cell, none larger than max_box,
+    none cut below min_box (host function)."""
+    t = np.asfortranarray(np.asarray(tags, dtype=np.uint8))
+    dim = t.ndim
+    lo = _arr(lo if lo is not None else (0,) * dim, np.int32)
+    hi = _arr([l + n - 1 for l, n in zip(lo, t.shape)], np.int32)
+    mn = _arr(min_box if min_box is not None else (1,) * dim, np.int32)
+    mx = _arr(max_box if max_box is not None else (1 << 30,) * dim, np.int32)
+    n = C.c_int()
+    args = (dim, lo.ctypes.data_as(_ip), hi.ctypes.data_as(_ip), t.ctypes.data_as(C.c_void_p),
+            efficiency_tolerance, combine_efficiency, mn.ctypes.data_as(_ip), mx.ctypes.data_as(_ip))
+    _check(lib().sfac_berger_rigoutsos(*args, None, 0, C.byref(n)))
+    out = np.zeros(max(n.value, 1) * 2 * dim, dtype=np.int32)
+    _check(lib().sfac_berger_rigoutsos(*args, out.ctypes.data_as(_ip), n.value, C.byref(n)))
+    b = out[: n.value * 2 * dim].reshape(n.value, 2, dim)
+    return [(tuple(int(v) for v in x[0]), tuple(int(v) for v in x[1])) for x in b]
