@@ -163,30 +163,26 @@ __device__ inline double block_max(double v, double* wm) {
   return v;
 }
 
-// Early-exit test of the previous sweep (see BoxCheck): every work-group
-// reads the same max-residual word, so all take the same decision, and
-// work-group 0 records it.  Called by all threads; true: skip this sweep.
-__device__ inline bool stop_sweep(const BoxCheck& K, int b, int& s_stop) {
+// Early-exit test of the previous sweep (see BoxCheck).  Every wave reads
+// the stop flag and the previous sweep's max-residual word itself (no
+// barrier): all read the same words and take the same decision (a stop that
+// work-group 0 records during this sweep follows from the same test), and
+// work-group 0 records it.  true: skip this sweep.
+__device__ inline bool stop_sweep(const BoxCheck& K, int b) {
   if (!K.stop) return false;
-  if (threadIdx.x == 0) {
-    int stop = *K.stop;
-    if (!stop && K.slots && K.sweep > 0) {
-      const double m = __longlong_as_double((long long)K.slots[(K.sweep + 2) % 3]);
-      if (!(m > K.tol)) {
-        stop = 1;
-        if (b == 0) {
-          *K.stop = 1;
-          *K.sweeps = K.sweep;
-          if (K.maxres_out) *K.maxres_out = m;
-        }
-      }
+  bool stop = *K.stop != 0;
+  if (!stop && K.slots && K.sweep > 0) {
+    const double m = __longlong_as_double((long long)K.slots[(K.sweep + 2) % 3]);
+    stop = !(m > K.tol);
+    if (stop && b == 0 && threadIdx.x == 0) {
+      *K.stop = 1;
+      *K.sweeps = K.sweep;
+      if (K.maxres_out) *K.maxres_out = m;
     }
-    // the slot the next sweep raises (its last reader was the previous sweep)
-    if (!stop && K.slots && b == 0) K.slots[(K.sweep + 1) % 3] = 0ull;
-    s_stop = stop;
   }
-  __syncthreads();
-  return s_stop != 0;
+  // the slot the next sweep raises (its last reader was the previous sweep)
+  if (!stop && K.slots && b == 0 && threadIdx.x == 0) K.slots[(K.sweep + 1) % 3] = 0ull;
+  return stop;
 }
 
 template <int DIM, bool DVAR, int CMODE, int TX, int TY, int TZ, typename T>
@@ -198,7 +194,6 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
   constexpr int ZH = DIM == 3 ? 2 : 0;  // LDS halo along z
   __shared__ T U[LX * LY * LZ];
   __shared__ double wm[kThreads / 64];
-  __shared__ int s_stop;
   // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
   // so consecutive tiles (one slab of the level) share an XCD's L2.
   const int b = blockIdx.x;
@@ -209,7 +204,7 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
   const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
 
   // 1. early exit
-  if (stop_sweep(K, b, s_stop)) return;
+  if (stop_sweep(K, b)) return;
 
   // colour parity: red <=> (ilev + jlev + klev + shift) even; in LDS
   // coordinates the sum is base + lx + ly + lz
