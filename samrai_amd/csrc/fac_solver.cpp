@@ -868,19 +868,22 @@ void CellPoissonFACOps::computeCompositeResidualOnLevel(SAMRAIVectorReal& r,
   const auto& T = d_h.tables(ln);
   const sfac::PoissonDev ps = poissonDev(ln);
   const int ewing = (d_ewing && ln > d_ln_min) ? 1 : 0;
-  // S1 ghost fill, S2 flux, S3 coarsened fine flux, S4 residual
+  // S1 ghost fill, S3 coarsened fine flux onto the faces it replaces, then
+  // S2 (fluxes, formed on the fly) and S4 (residual, saved boundary flux)
+  // in one pass
   ghostFill(*u.data, ln, ln > d_ln_min, error_equation_indicator, false);
-  sfac::launch_flux(T.cells.launch(), T.geom, d_flux->interleaved(ln), u.data->table(ln),
-                    u.data->ghost(), ps, ewing, d_stream);
+  sfac::LevelGeom G = T.geom;
   if (ln < d_ln_max) {
     const auto& Tf = d_h.tables(ln + 1);
     sfac::launch_flux_coarsen(Tf.flux_to_coarser.launch(), T.geom, Tf.geom,
                               d_flux->interleaved(ln), d_oflux_tab[ln + 1], d_stream);
+  } else {
+    G.fmask = nullptr;  // the finer levels of the hierarchy are not part of this solve
   }
-  sfac::launch_residual(T.cells.launch(), T.geom, r.data->table(ln), r.data->ghost(),
-                        f.data->table(ln), f.data->ghost(), u.data->table(ln), u.data->ghost(),
-                        d_flux->interleaved(ln), ps, ln > d_ln_min ? d_oflux_tab[ln] : nullptr,
-                        d_stream);
+  sfac::launch_residual_fused(T.cells.launch(), G, r.data->table(ln), r.data->ghost(),
+                              f.data->table(ln), f.data->ghost(), u.data->table(ln),
+                              u.data->ghost(), d_flux->interleaved(ln), ps, ewing,
+                              ln > d_ln_min ? d_oflux_tab[ln] : nullptr, d_stream);
 }
 
 void CellPoissonFACOps::restrictSolution(const SAMRAIVectorReal& s, SAMRAIVectorReal& d,

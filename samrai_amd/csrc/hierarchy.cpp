@@ -429,6 +429,37 @@ void PatchHierarchy::buildLevel(int ln) {
     T.restrict_to_coarser = uploadList(rs);
     T.prolong_from_coarser = uploadList(pr);
     T.flux_to_coarser = uploadList(fc);
+    // the coarser level's faces these coarsened fluxes replace: per coarse
+    // cell, bit 2a its lower face and bit 2a+1 its upper face along a
+    LevelTables& TC = d_tables[ln - 1];
+    const int npc = (int)C->boxes.size();
+    std::vector<long> off(npc + 1, 0);
+    for (int p = 0; p < npc; ++p) off[p + 1] = off[p] + boxsize(C->boxes[p], dim);
+    std::vector<unsigned char> mask((size_t)std::max(off[npc], 1L), 0);
+    for (const BoxItem& it : fc) {
+      const int a = it.tag & 3;
+      const Box& PB = C->boxes[it.p];
+      const long nx = PB.hi[0] - PB.lo[0] + 1, ny = PB.hi[1] - PB.lo[1] + 1;
+      const int k0 = dim == 3 ? it.lo[2] : 0, k1 = dim == 3 ? it.hi[2] : 0;
+      for (int k = k0; k <= k1; ++k)
+        for (int j = it.lo[1]; j <= it.hi[1]; ++j)
+          for (int i = it.lo[0]; i <= it.hi[0]; ++i) {
+            int c[3] = {i, j, k};
+            const int fcx = c[a];
+            for (int side = 0; side < 2; ++side) {
+              // side 0: the cell above the face (its lower face); 1: below
+              int cc[3] = {c[0], c[1], c[2]};
+              cc[a] = side == 0 ? fcx : fcx - 1;
+              if (cc[a] < PB.lo[a] || cc[a] > PB.hi[a]) continue;
+              const long o = off[it.p] + (cc[0] - PB.lo[0]) +
+                             nx * ((cc[1] - PB.lo[1]) + ny * (dim == 3 ? cc[2] - PB.lo[2] : 0));
+              mask[o] |= (unsigned char)(1u << (2 * a + (side == 0 ? 0 : 1)));
+            }
+          }
+    }
+    off.pop_back();
+    TC.geom.fmask = upload(mask);
+    TC.geom.fmask_off = upload(off);
   }
 
   T.geom.box = T.d_box;

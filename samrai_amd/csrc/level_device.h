@@ -57,6 +57,12 @@ struct LevelGeom {
   // Ewing-corrected flux of every coarse-fine face, same indexing as cf;
   // written by the pre-pass of each colour, read by the colour kernel.
   double* cf_flux;
+  // faces whose flux the next finer level's coarsened boundary flux
+  // replaces (nullptr on the finest level): one byte per interior cell of
+  // patch p at fmask[fmask_off[p] + cell], bit 2*axis the cell's lower face,
+  // bit 2*axis+1 its upper face
+  const unsigned char* fmask;
+  const long* fmask_off;
 };
 
 // Poisson specification (PoissonSpecifications): D constant or side data,

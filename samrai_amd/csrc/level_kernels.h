@@ -58,22 +58,19 @@ void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, in
 void launch_ewing_faces(const GhostLaunch& w, const LevelGeom& L, const double* const* u, int ug,
                         const PoissonDev& ps, const int* stop, hipStream_t s);
 
-// Materialised flux (dim side arrays per patch, ghost 0, pointer table
-// interleaved as [p * dim + axis]), with the Ewing fix.
-void launch_flux(const GhostLaunch& cells, const LevelGeom& L, double* const* flux,
-                 const double* const* u, int ug, const PoissonDev& ps, int ewing,
-                 hipStream_t s);
-
 // Coarse faces under the finer level's patch boundaries take the
 // area-weighted mean of the saved fine flux (outerside) values.
 void launch_flux_coarsen(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
                          double* const* cflux, const double* const* foflux, hipStream_t s);
 
-// r = f - div F - C u on the interior; optionally save the boundary flux.
-void launch_residual(const GhostLaunch& cells, const LevelGeom& L, double* const* r, int rg,
-                     const double* const* f, int fg, const double* const* u, int ug,
-                     const double* const* flux, const PoissonDev& ps,
-                     double* const* oflux, hipStream_t s);
+// Composite residual in one pass: every face flux formed on the fly (with
+// the Ewing fix when ewing), except faces marked in L.fmask, whose flux is
+// read from flux (written there by launch_flux_coarsen): r = f - div F - C u
+// on the interior, and the patch boundary fluxes into oflux (if not null).
+void launch_residual_fused(const GhostLaunch& cells, const LevelGeom& L, double* const* r,
+                           int rg, const double* const* f, int fg, const double* const* u,
+                           int ug, const double* const* flux, const PoissonDev& ps, int ewing,
+                           double* const* oflux, hipStream_t s);
 
 // Cell weighted average fine -> coarse (items: p coarse, q fine, coarse box).
 void launch_restrict(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,

@@ -403,50 +403,6 @@ k_rbgs(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* e,
 
 // ------------------------------------------------------ flux / residual
 
-template <int DIM, bool DVAR>
-__global__ void __launch_bounds__(kThreads)
-k_flux(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* flux,
-       const double* const* uu, int ug, PoissonDev ps, int ewing) {
-  const int ci = xcd_chunk();
-  const Chunk ch = chunks[ci];
-  if (ch.item < 0) return;
-  const BoxItem it = items[ch.item];
-  const int p = it.p;
-  const Box b = L.box[p];
-  const View vu = vview(b, ug, DIM);
-  // every per-axis array below is indexed by unrolled loop counters only, so
-  // it lives in registers (a runtime index would put it in scratch memory)
-  View vd[3], vf[3];
-  const double* Dp[3] = {nullptr, nullptr, nullptr};
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    vf[a] = vview(b, 0, DIM, a);
-    if (DVAR) {
-      vd[a] = vview(b, ps.dghost, DIM, a);
-      Dp[a] = ps.D[a][p];
-    }
-  }
-  const double* u = uu[p];
-  // interleaved side-data table: [p * DIM + axis]
-  double* F[3] = {flux[p * DIM + 0], flux[p * DIM + 1], DIM == 3 ? flux[p * DIM + 2] : nullptr};
-  double di[3] = {1.0 / L.dx[0], 1.0 / L.dx[1], DIM == 3 ? 1.0 / L.dx[2] : 0.0};
-  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
-    int c[3];
-    if (!decode_box(it, (long)ch.start + t, c)) break;
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      F[a][vf[a].at(c[0], c[1], c[2])] = face_flux<DIM, DVAR>(
-          L, p, b, u, vu, Dp, vd, ps.dconst, a, c[0], c[1], c[2], di[a], ewing);
-      if (c[a] == b.hi[a]) {
-        int h[3] = {c[0], c[1], c[2]};
-        h[a] += 1;
-        F[a][vf[a].at(h[0], h[1], h[2])] = face_flux<DIM, DVAR>(
-            L, p, b, u, vu, Dp, vd, ps.dconst, a, h[0], h[1], h[2], di[a], ewing);
-      }
-    }
-  }
-}
-
 // Snapshot of the Ewing-corrected flux on every coarse-fine face before a
 // colour pass.  The reference computes all fluxes of a patch before the
 // colour update (CellPoissonFACOps.C:1392-1428); the corrected flux reads the
@@ -505,11 +461,15 @@ __device__ inline long oslab(const Box& b, int a, int dim, const int* c) {
   return idx;
 }
 
-template <int DIM, int CMODE>
+// Composite residual of a level in one pass (the fluxes of k_flux formed on
+// the fly, the coarsened fine fluxes read where L.fmask marks them, then
+// the residual of k_residual): the same per-face values and the same
+// residual expression, without writing and re-reading the side arrays.
+template <int DIM, bool DVAR, int CMODE>
 __global__ void __launch_bounds__(kThreads)
-k_residual(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* rr, int rg,
-           const double* const* ff, int fg, const double* const* uu, int ug,
-           const double* const* flux, PoissonDev ps, double* const* oflux) {
+k_residual_fused(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* rr,
+                 int rg, const double* const* ff, int fg, const double* const* uu, int ug,
+                 const double* const* flux, PoissonDev ps, int ewing, double* const* oflux) {
   const int ci = xcd_chunk();
   const Chunk ch = chunks[ci];
   if (ch.item < 0) return;
@@ -518,39 +478,60 @@ k_residual(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const
   const Box b = L.box[p];
   const View vu = vview(b, ug, DIM), vr = vview(b, rg, DIM), vfg = vview(b, fg, DIM);
   const View vc = vview(b, ps.cghost, DIM);
-  View vf[3];
-  for (int a = 0; a < DIM; ++a) vf[a] = vview(b, 0, DIM, a);
+  View vd[3], vf[3];
+  const double* Dp[3] = {nullptr, nullptr, nullptr};
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    vf[a] = vview(b, 0, DIM, a);
+    if (DVAR) {
+      vd[a] = vview(b, ps.dghost, DIM, a);
+      Dp[a] = ps.D[a][p];
+    }
+  }
+  const double* u = uu[p];
   const double* F[3] = {flux[p * DIM + 0], flux[p * DIM + 1],
                         DIM == 3 ? flux[p * DIM + 2] : nullptr};
-  const double dxi = 1.0 / L.dx[0], dyi = 1.0 / L.dx[1];
-  const double dzi = DIM == 3 ? 1.0 / L.dx[2] : 0.0;
+  const unsigned char* mask = L.fmask ? L.fmask + L.fmask_off[p] : nullptr;
+  const double di[3] = {1.0 / L.dx[0], 1.0 / L.dx[1], DIM == 3 ? 1.0 / L.dx[2] : 0.0};
+  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
     int c[3];
     if (!decode_box(it, (long)ch.start + t, c)) break;
-    int i = c[0], j = c[1], k = c[2];
-    double scal = CMODE == 2 ? ps.C[p][vc.at(i, j, k)] : (CMODE == 1 ? ps.cconst : 0.0);
+    const int i = c[0], j = c[1], k = c[2];
+    const int m = mask ? mask[(i - b.lo[0]) + nx * ((j - b.lo[1]) +
+                                                     ny * (DIM == 3 ? k - b.lo[2] : 0))]
+                       : 0;
+    double fl[3] = {0.0, 0.0, 0.0}, fh[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) {
+      int h[3] = {i, j, k};
+      h[a] += 1;
+      fl[a] = (m & (1 << (2 * a)))
+                  ? F[a][vf[a].at(i, j, k)]
+                  : face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, a, i, j, k, di[a],
+                                         ewing);
+      fh[a] = (m & (2 << (2 * a)))
+                  ? F[a][vf[a].at(h[0], h[1], h[2])]
+                  : face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, a, h[0], h[1],
+                                         h[2], di[a], ewing);
+    }
+    const double dxi = di[0], dyi = di[1], dzi = di[2];
+    const double scal = CMODE == 2 ? ps.C[p][vc.at(i, j, k)] : (CMODE == 1 ? ps.cconst : 0.0);
     double res;
     if (DIM == 3)
       res = ff[p][vfg.at(i, j, k)] -
-            (dxi * (F[0][vf[0].at(i + 1, j, k)] - F[0][vf[0].at(i, j, k)]) +
-             dyi * (F[1][vf[1].at(i, j + 1, k)] - F[1][vf[1].at(i, j, k)]) +
-             dzi * (F[2][vf[2].at(i, j, k + 1)] - F[2][vf[2].at(i, j, k)])) -
-            scal * uu[p][vu.at(i, j, k)];
+            (dxi * (fh[0] - fl[0]) + dyi * (fh[1] - fl[1]) + dzi * (fh[2] - fl[2])) -
+            scal * u[vu.at(i, j, k)];
     else
-      res = ff[p][vfg.at(i, j, k)] -
-            (dxi * (F[0][vf[0].at(i + 1, j, k)] - F[0][vf[0].at(i, j, k)]) +
-             dyi * (F[1][vf[1].at(i, j + 1, k)] - F[1][vf[1].at(i, j, k)])) -
-            scal * uu[p][vu.at(i, j, k)];
+      res = ff[p][vfg.at(i, j, k)] - (dxi * (fh[0] - fl[0]) + dyi * (fh[1] - fl[1])) -
+            scal * u[vu.at(i, j, k)];
     rr[p][vr.at(i, j, k)] = res;
     if (oflux) {
       // OutersideData::copy(SideData): the patch's boundary faces
+#pragma unroll
       for (int a = 0; a < DIM; ++a) {
-        if (c[a] == b.lo[a]) oflux[p * 6 + 2 * a][oslab(b, a, DIM, c)] = F[a][vf[a].at(i, j, k)];
-        if (c[a] == b.hi[a]) {
-          int h[3] = {i, j, k};
-          h[a] += 1;
-          oflux[p * 6 + 2 * a + 1][oslab(b, a, DIM, c)] = F[a][vf[a].at(h[0], h[1], h[2])];
-        }
+        if (c[a] == b.lo[a]) oflux[p * 6 + 2 * a][oslab(b, a, DIM, c)] = fl[a];
+        if (c[a] == b.hi[a]) oflux[p * 6 + 2 * a + 1][oslab(b, a, DIM, c)] = fh[a];
       }
     }
   }
@@ -803,29 +784,6 @@ void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, in
   check_launch();
 }
 
-void launch_flux(const GhostLaunch& w, const LevelGeom& L, double* const* flux,
-                 const double* const* u, int ug, const PoissonDev& ps, int ewing,
-                 hipStream_t s) {
-  if (w.nchunks == 0) return;
-  bool dvar = ps.D[0] != nullptr;
-  if (L.dim == 3) {
-    if (dvar)
-      hipLaunchKernelGGL((k_flux<3, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
-                         w.chunks, L, flux, u, ug, ps, ewing);
-    else
-      hipLaunchKernelGGL((k_flux<3, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
-                         w.chunks, L, flux, u, ug, ps, ewing);
-  } else {
-    if (dvar)
-      hipLaunchKernelGGL((k_flux<2, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
-                         w.chunks, L, flux, u, ug, ps, ewing);
-    else
-      hipLaunchKernelGGL((k_flux<2, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
-                         w.chunks, L, flux, u, ug, ps, ewing);
-  }
-  check_launch();
-}
-
 void launch_ewing_faces(const GhostLaunch& w, const LevelGeom& L, const double* const* u, int ug,
                         const PoissonDev& ps, const int* stop, hipStream_t s) {
   if (w.nchunks == 0) return;
@@ -860,29 +818,36 @@ void launch_flux_coarsen(const GhostLaunch& w, const LevelGeom& coarse, const Le
   check_launch();
 }
 
-template <int DIM>
-static void residual_dispatch(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
-                              const double* const* f, int fg, const double* const* u, int ug,
-                              const double* const* flux, const PoissonDev& ps,
-                              double* const* oflux, hipStream_t s) {
-  if (ps.cmode == 0)
-    hipLaunchKernelGGL((k_residual<DIM, 0>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
-                       w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
+template <int DIM, bool DVAR>
+static void residual_fused_c(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
+                             const double* const* f, int fg, const double* const* u, int ug,
+                             const double* const* flux, const PoissonDev& ps, int ewing,
+                             double* const* oflux, hipStream_t s) {
+  const dim3 g(xcd_grid(w.nchunks)), bl(kThreads);
+  if (ps.cmode == 2)
+    hipLaunchKernelGGL((k_residual_fused<DIM, DVAR, 2>), g, bl, 0, s, w.items, w.chunks, L, r, rg,
+                       f, fg, u, ug, flux, ps, ewing, oflux);
   else if (ps.cmode == 1)
-    hipLaunchKernelGGL((k_residual<DIM, 1>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
-                       w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
+    hipLaunchKernelGGL((k_residual_fused<DIM, DVAR, 1>), g, bl, 0, s, w.items, w.chunks, L, r, rg,
+                       f, fg, u, ug, flux, ps, ewing, oflux);
   else
-    hipLaunchKernelGGL((k_residual<DIM, 2>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items,
-                       w.chunks, L, r, rg, f, fg, u, ug, flux, ps, oflux);
+    hipLaunchKernelGGL((k_residual_fused<DIM, DVAR, 0>), g, bl, 0, s, w.items, w.chunks, L, r, rg,
+                       f, fg, u, ug, flux, ps, ewing, oflux);
 }
 
-void launch_residual(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
-                     const double* const* f, int fg, const double* const* u, int ug,
-                     const double* const* flux, const PoissonDev& ps, double* const* oflux,
-                     hipStream_t s) {
+void launch_residual_fused(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
+                           const double* const* f, int fg, const double* const* u, int ug,
+                           const double* const* flux, const PoissonDev& ps, int ewing,
+                           double* const* oflux, hipStream_t s) {
   if (w.nchunks == 0) return;
-  if (L.dim == 3) residual_dispatch<3>(w, L, r, rg, f, fg, u, ug, flux, ps, oflux, s);
-  else residual_dispatch<2>(w, L, r, rg, f, fg, u, ug, flux, ps, oflux, s);
+  const bool dvar = ps.D[0] != nullptr;
+  if (L.dim == 3) {
+    if (dvar) residual_fused_c<3, true>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+    else residual_fused_c<3, false>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+  } else {
+    if (dvar) residual_fused_c<2, true>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+    else residual_fused_c<2, false>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+  }
   check_launch();
 }
 
