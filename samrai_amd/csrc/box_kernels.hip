@@ -170,15 +170,21 @@ __device__ inline double block_max(double v, double* wm) {
 // work-group 0 records it.  true: skip this sweep.
 __device__ inline bool stop_sweep(const BoxCheck& K, int b) {
   if (!K.stop) return false;
-  bool stop = *K.stop != 0;
-  if (!stop && K.slots && K.sweep > 0) {
+  bool stop;
+  if (K.slots && K.sweep > 0) {
+    // both words are read unconditionally, so that their loads are in
+    // flight together (one memory round trip before the decision)
+    const int flag = *K.stop;
     const double m = __longlong_as_double((long long)K.slots[(K.sweep + 2) % 3]);
-    stop = !(m > K.tol);
-    if (stop && b == 0 && threadIdx.x == 0) {
+    const bool now = !(m > K.tol);
+    stop = (flag != 0) | now;
+    if (flag == 0 && now && b == 0 && threadIdx.x == 0) {
       *K.stop = 1;
       *K.sweeps = K.sweep;
       if (K.maxres_out) *K.maxres_out = m;
     }
+  } else {
+    stop = *K.stop != 0;
   }
   // the slot the next sweep raises (its last reader was the previous sweep)
   if (!stop && K.slots && b == 0 && threadIdx.x == 0) K.slots[(K.sweep + 1) % 3] = 0ull;
@@ -197,14 +203,15 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
   // XCD-aware tile order: work-groups are dealt round-robin to the 8 XCDs,
   // so consecutive tiles (one slab of the level) share an XCD's L2.
   const int b = blockIdx.x;
+
+  // 1. early exit (first, so that its loads overlap the tile set-up)
+  if (stop_sweep(K, b)) return;
+
   const int t = (b & 7) * per_xcd + (b >> 3);
   if (t >= ntiles) return;
   const int tx = t % ntx, ty = (t / ntx) % nty, tz = t / (ntx * nty);
   const int x0 = tx * TX, y0 = ty * TY, z0 = DIM == 3 ? tz * TZ : 0;
   const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
-
-  // 1. early exit
-  if (stop_sweep(K, b)) return;
 
   // colour parity: red <=> (ilev + jlev + klev + shift) even; in LDS
   // coordinates the sum is base + lx + ly + lz
