@@ -1,0 +1,95 @@
+/*
+ * samrai_learned.h -- C ABI of the learned block smoother in libsamrai_fac.so.
+ *
+ * None of these entry points has a counterpart in the reference: SAMRAI's
+ * CellPoissonFACOps smooths only with red-black Gauss-Seidel
+ * (solv/CellPoissonFACOps.C:1338-1514, called through
+ * FACOperatorStrategy::smoothError, solv/FACOperatorStrategy.h:82-364).
+ * The learned smoother is an extra correction a smoothError implementation
+ * may apply per 16^3 block of a level before (or instead of) red-black
+ * sweeps; the red-black smoother remains the fallback and the FP64 residual
+ * remains the convergence test.
+ *
+ * Blocks: 16x16x16 cells, x fastest, blocks contiguous (block b, cell
+ * (x,y,z) at b*4096 + (z*16 + y)*16 + x).  Face coefficients of a block:
+ * dxf[b][z][y][0..16], dyf[b][z][0..16][x], dzf[b][0..16][y][x].
+ * BF16 tensors are raw uint16 bit patterns, channels-last with 16 channels
+ * per cell.  All pointers are device pointers; every call is
+ * stream-ordered, returns 0 on success and < 0 on error, with the message
+ * in sfac_last_error() (samrai_fac.h).
+ */
+#ifndef SAMRAI_LEARNED_H
+#define SAMRAI_LEARNED_H
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 3x3x3 convolution, 16 -> 16 channels, zero padding at block faces, BF16
+ * operands on the matrix cores with FP32 accumulation.  w[t][n][k] with
+ * t = 9(dz+1) + 3(dy+1) + (dx+1), n the output and k the input channel.
+ * mode 0: out = relu(conv(x) + bias) (BF16);
+ * mode 1: pred[cell] = conv(x)[cell][0] + bias[0] (FP32);
+ * mode 2: out = conv(x) * (mask > 0) (BF16; backward of a ReLU layer when w
+ *         is the tap-flipped transposed weight tensor). */
+int sfac_conv3x3x3_bf16(const uint16_t* x, const uint16_t* w, const float* bias,
+                        const uint16_t* mask, uint16_t* out, float* pred, int nblocks, int mode,
+                        hipStream_t stream);
+
+/* Weight gradient of one convolution layer:
+ * grad[t][co][ci] = sum_cells gz[cell][co] * x[cell + off(t)][ci] (t < 27),
+ * grad[27][co][0] = sum_cells gz[cell][co] (bias), other entries of tap 27
+ * are sums against a zero column.  partial: device scratch of
+ * sfac_conv3x3x3_wgrad_partial_size(nblocks) floats. */
+int sfac_conv3x3x3_wgrad_bf16(const uint16_t* x, const uint16_t* gz, float* grad, float* partial,
+                              int nblocks, hipStream_t stream);
+size_t sfac_conv3x3x3_wgrad_partial_size(int nblocks);
+
+/* Block problems div(D grad e) = r, homogeneous Dirichlet on the block faces
+ * (ghost = -interior), FP64.  sfac_block_sor: red-black SOR sweeps (omega 1
+ * is red-black Gauss-Seidel), starting from e (from_input != 0) or zero.
+ * sfac_block_residual: rout = r - div(D grad e). */
+int sfac_block_sor(const double* r, const double* dxf, const double* dyf, const double* dzf,
+                   double h, int sweeps, double omega, int from_input, double* e, int nblocks,
+                   hipStream_t stream);
+int sfac_block_residual(const double* r, const double* dxf, const double* dyf, const double* dzf,
+                        double h, const double* e, double* rout, int nblocks, hipStream_t stream);
+
+/* The network: 2 -> 16 -> 16 -> 1 channels, 3x3x3 convolutions, ReLU after
+ * the hidden layers.  Input features per cell: r h^2 / s and log D, with s
+ * the RMS of r h^2 over the block; output e / s.  FP32 master parameters
+ * and Adam state on the device, BF16 copies for the matrix cores.
+ * Parameters of layer l (3 layers, 27*16*16 + 16 floats each): w[t][co][ci]
+ * then bias[co]; gradients per layer: 28*16*16 floats as in
+ * sfac_conv3x3x3_wgrad_bf16. */
+int sfac_learned_create(int max_blocks, unsigned long long seed, void** net);
+int sfac_learned_destroy(void* net);
+int sfac_learned_features(void* net, const double* r, const double* logd, double h, int nblocks,
+                          hipStream_t stream);
+int sfac_learned_forward(void* net, int nblocks, hipStream_t stream);
+/* e = omega * s * prediction (FP64), after features + forward. */
+int sfac_learned_correction(void* net, double omega, double* e, int nblocks, hipStream_t stream);
+/* Loss mean((prediction - target / s)^2) and the gradients of every layer,
+ * after features + forward on the same blocks. */
+int sfac_learned_backward(void* net, const double* target, int nblocks, hipStream_t stream);
+int sfac_learned_adam(void* net, float lr, hipStream_t stream);
+/* features + forward + backward + Adam. */
+int sfac_learned_train_step(void* net, const double* r, const double* logd, const double* target,
+                            double h, int nblocks, float lr, hipStream_t stream);
+/* which 0: parameters (3 * (27*16*16 + 16) floats); 1: gradients (3 * 28*16*16). Synchronous. */
+int sfac_learned_get_params(void* net, float* host, int which);
+int sfac_learned_set_params(void* net, const float* host);
+int sfac_learned_loss(void* net, float* loss);
+/* Device buffers of the last features/forward: features (BF16, 16 channels per
+ * cell), prediction (FP32 per cell), per-block scale s (FP64). */
+int sfac_learned_buffers(void* net, void** features, float** pred, double** scale);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

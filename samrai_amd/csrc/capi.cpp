@@ -41,6 +41,14 @@ int guarded(F&& f) {
   return -1;
 }
 
+}  // namespace
+
+namespace samrai_amd {
+// Used by entry points defined in other translation units (learned smoother).
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace samrai_amd
+
+namespace {
 void need(const void* p, const char* what) {
   if (!p) fail(std::string("null ") + what);
 }

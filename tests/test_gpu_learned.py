@@ -1,0 +1,210 @@
+"""The learned block smoother on the device against the numpy oracle:
+matrix-core convolutions (forward, backward data, weight gradient), the
+network's loss and gradients, the FP64 block solver, and training on pairs
+the FAC solver generates, with what the trained network gains."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import learned as L
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _bits(torch, a):
+    return torch.from_numpy(L.bf16_bits(a).view(np.int16)).cuda()
+
+
+def _f32(t):
+    return L.from_bits(t.cpu().numpy().view(np.uint16))
+
+
+def _rand_layers(rng, scale=0.3):
+    layers = []
+    for l in range(3):
+        w = np.zeros((L.NTAP, L.CH, L.CH), np.float32)
+        w[:, :L.COUT[l], :L.CIN[l]] = rng.uniform(-scale, scale, (L.NTAP, L.COUT[l], L.CIN[l]))
+        b = np.zeros(L.CH, np.float32)
+        b[:L.COUT[l]] = rng.uniform(-0.1, 0.1, L.COUT[l])
+        layers.append(np.concatenate([w.ravel(), b]).astype(np.float32))
+    return layers
+
+
+def _close_bf16(got, exp):
+    """Device outputs are FP32 sums rounded to BF16: within one BF16 ulp of
+    the float64 sum rounded the same way."""
+    tol = 2.0 ** -7 * np.abs(exp) + 1e-6 * np.abs(exp).max()
+    bad = np.abs(got - exp) > tol
+    assert not bad.any(), (int(bad.sum()), float(np.abs(got - exp).max()))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_conv_modes_match_oracle(torch_gpu, mode):
+    torch = torch_gpu
+    from samrai_amd import learned as SL
+    _say("conv mode %d, 3 blocks" % mode)
+    rng = np.random.default_rng(10 + mode)
+    x = L.bf16(rng.standard_normal((3, 16, 16, 16, 16)).astype(np.float32))
+    w = L.bf16(rng.uniform(-0.3, 0.3, (27, 16, 16)).astype(np.float32))
+    bias = rng.uniform(-0.5, 0.5, 16).astype(np.float32)
+    mask = L.bf16(np.maximum(rng.standard_normal(x.shape), 0).astype(np.float32))
+    out = SL.conv3x3x3(torch, _bits(torch, x), _bits(torch, w),
+                       torch.from_numpy(bias).cuda(), _bits(torch, mask), mode)
+    torch.cuda.synchronize()
+    z = L.conv(x, w)
+    if mode == 0:
+        _close_bf16(_f32(out).reshape(z.shape), L.bf16(np.maximum(z + bias, 0).astype(np.float32)))
+    elif mode == 1:
+        np.testing.assert_allclose(out.cpu().numpy(), z[..., 0] + bias[0], rtol=1e-5,
+                                   atol=1e-5 * np.abs(z[..., 0]).max())
+    else:
+        # backward data: the tap-flipped transposed weights give the adjoint
+        wb = np.ascontiguousarray(w[::-1].transpose(0, 2, 1))
+        out = SL.conv3x3x3(torch, _bits(torch, x), _bits(torch, wb), None, _bits(torch, mask), 2)
+        torch.cuda.synchronize()
+        exp = L.conv_bwd(x, w) * (mask > 0)
+        _close_bf16(_f32(out).reshape(exp.shape), L.bf16(exp.astype(np.float32)))
+
+
+def test_wgrad_matches_oracle(torch_gpu):
+    torch = torch_gpu
+    from samrai_amd import learned as SL
+    _say("weight gradient, 3 blocks")
+    rng = np.random.default_rng(20)
+    x = L.bf16(rng.standard_normal((3, 16, 16, 16, 16)).astype(np.float32))
+    g = L.bf16(rng.standard_normal((3, 16, 16, 16, 16)).astype(np.float32))
+    got = SL.conv3x3x3_wgrad(torch, _bits(torch, x), _bits(torch, g)).cpu().numpy()
+    exp = L.wgrad(x, g)
+    # FP32 sums over 12288 cells of O(1) products
+    np.testing.assert_allclose(got[:27], exp[:27], rtol=0, atol=2e-6 * np.sqrt(12288) * 4)
+    np.testing.assert_allclose(got[27, :, 0], exp[27, :, 0], rtol=0, atol=1e-3)
+    assert np.all(got[27, :, 1:] == 0)
+
+
+def _problem(torch, rng, nb):
+    r = rng.standard_normal((nb, 16, 16, 16))
+    logd = rng.uniform(-1.5, 1.5, (nb, 16, 16, 16))
+    target = rng.standard_normal((nb, 16, 16, 16)) * 1e-4
+    return (torch.from_numpy(r).cuda(), torch.from_numpy(logd).cuda(),
+            torch.from_numpy(target).cuda())
+
+
+def test_network_loss_and_gradients_match_oracle(torch_gpu):
+    torch = torch_gpu
+    from samrai_amd import learned as SL
+    _say("network loss and gradients, 2 blocks")
+    rng = np.random.default_rng(30)
+    net = SL.LearnedSmoother(torch, 4, seed=1)
+    layers = _rand_layers(rng)
+    net.set_params(layers)
+    np.testing.assert_array_equal(np.concatenate(net.params()), np.concatenate(layers))
+    r, logd, target = _problem(torch, rng, 2)
+    h = 1.0 / 64
+    net.forward(r, logd, h)
+    net.backward(target)
+    torch.cuda.synchronize()
+    feat_ptr, _, scale_ptr = net.buffers()
+    # features as the device made them (bits), and its block scales
+    feat = _wrap(torch, feat_ptr, 2 * 4096 * 16, torch.int16).view(2, 16, 16, 16, 16)
+    scale = _wrap(torch, scale_ptr, 2, torch.float64).cpu().numpy()
+    x = _f32(feat)
+    xo, so = L.features(r.cpu().numpy(), logd.cpu().numpy(), h)
+    np.testing.assert_allclose(scale, so, rtol=1e-13)
+    _close_bf16(x, L.bf16(xo))
+    loss_o, grads_o = L.loss_and_grads(layers, x, target.cpu().numpy(), scale, emulate=True)
+    loss = net.loss()
+    assert abs(loss - loss_o) <= 1e-4 * loss_o, (loss, loss_o)
+    grads = net.grads()
+    for l in range(3):
+        ref = grads_o[l]
+        err = np.linalg.norm(grads[l] - ref) / np.linalg.norm(ref)
+        _say("layer %d: relative gradient error %.2e" % (l, err))
+        assert err < 1e-2, (l, err)
+    net.close()
+
+
+def _wrap(torch, ptr, n, dtype):
+    """A device tensor viewing n elements at ptr (copied out immediately)."""
+    import ctypes
+    esize = torch.tensor([], dtype=dtype).element_size()
+    out = torch.empty(n, dtype=dtype, device="cuda")
+    from samrai_amd import fac
+    hip = ctypes.CDLL(fac.LIB_PATH)  # hipMemcpy through the HIP runtime the library uses
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    assert hip.hipMemcpy(ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(ptr), n * esize,
+                         3) == 0  # hipMemcpyDeviceToDevice
+    return out
+
+
+def test_block_sor_and_residual(torch_gpu):
+    torch = torch_gpu
+    from samrai_amd import learned as SL
+    _say("block SOR / residual, 4 blocks")
+    rng = np.random.default_rng(40)
+    h = 1.0 / 64
+    nb = 4
+    dxf = np.exp(rng.standard_normal((nb, 16, 16, 17)))
+    dyf = np.exp(rng.standard_normal((nb, 16, 17, 16)))
+    dzf = np.exp(rng.standard_normal((nb, 17, 16, 16)))
+    r = rng.standard_normal((nb, 16, 16, 16))
+    t = [torch.from_numpy(a).cuda() for a in (r, dxf, dyf, dzf)]
+    for sweeps, omega in ((1, 1.0), (7, 1.6)):
+        e = SL.block_sor(torch, *t, h, sweeps, omega).cpu().numpy()
+        eo = L.block_sor(r, dxf, dyf, dzf, h, sweeps, omega)
+        np.testing.assert_allclose(e, eo, rtol=1e-12, atol=1e-12 * np.abs(eo).max())
+    res = SL.block_residual(torch, t[0], torch.from_numpy(eo).cuda(), *t[1:], h).cpu().numpy()
+    reso = L.block_residual(r, eo, dxf, dyf, dzf, h)
+    np.testing.assert_allclose(res, reso, rtol=0, atol=1e-10 * np.abs(r).max())
+    # closed form: the discrete eigenfunction is reproduced
+    v, lam = L.block_eigenfunction(2, 5, 1, nblocks=1)
+    one = np.ones((1, 16, 16, 17))
+    f = [torch.from_numpy(a).cuda() for a in
+         (lam / h ** 2 * v, one, one.transpose(0, 1, 3, 2).copy(), one.transpose(0, 3, 1, 2).copy())]
+    e = SL.block_sor(torch, *f, h, 400, 1.8).cpu().numpy()
+    assert np.abs(e - v).max() < 1e-10
+
+
+def test_training_on_fac_pairs_gains_over_no_network(torch_gpu):
+    torch = torch_gpu
+    from samrai_amd import learned as SL
+    _say("FAC-generated pairs: seeds 0-3 train, seed 9 test")
+    train = [SL.fac_training_pairs(torch, s, cycles=(1, 2)) for s in range(4)]
+    test = SL.fac_training_pairs(torch, 9, cycles=(1, 2))
+    cat = {k: torch.cat([p[k] for p in train]).contiguous() for k in ("r", "logd", "e")}
+    h = train[0]["h"]
+    _say("pairs: %d train blocks, %d test blocks" % (cat["r"].shape[0], test["r"].shape[0]))
+    net = SL.LearnedSmoother(torch, 128, seed=3)
+
+    def quality(n):
+        e = n.correction(test["r"], test["logd"], h)
+        rr = SL.block_residual(torch, test["r"], e, test["dxf"], test["dyf"], test["dzf"], h)
+        err = (torch.linalg.vector_norm(e - test["e"]) / torch.linalg.vector_norm(test["e"])).item()
+        res = (torch.linalg.vector_norm(rr) / torch.linalg.vector_norm(test["r"])).item()
+        return err, res, e
+
+    err0, res0, _ = quality(net)
+    losses = net.train(cat["r"], cat["logd"], cat["e"], h, steps=600, batch=32, lr=2e-3,
+                       log_every=100)
+    _say("losses: %s" % losses)
+    err1, res1, e_net = quality(net)
+    # red-black Gauss-Seidel with the same data, from zero and after the network
+    gs1 = SL.block_sor(torch, test["r"], test["dxf"], test["dyf"], test["dzf"], h, 1, 1.0)
+    err_gs = (torch.linalg.vector_norm(gs1 - test["e"]) / torch.linalg.vector_norm(test["e"])).item()
+    both = SL.block_sor(torch, test["r"], test["dxf"], test["dyf"], test["dzf"], h, 1, 1.0,
+                        e0=e_net)
+    err_both = (torch.linalg.vector_norm(both - test["e"]) /
+                torch.linalg.vector_norm(test["e"])).item()
+    _say("unseen seed: block-error ratio untrained %.3f trained %.3f (1 RBGS sweep %.3f, "
+         "network + 1 sweep %.3f); residual ratio untrained %.3f trained %.3f"
+         % (err0, err1, err_gs, err_both, res0, res1))
+    assert losses[-1][1] < 0.5 * losses[0][1]
+    assert err1 < 0.5 * err0
+    assert err1 < err_gs        # the trained network beats one red-black sweep on the block error
+    assert err_both < err_gs    # and still helps when the sweep follows it
+    net.close()
