@@ -169,6 +169,104 @@ def time_c2(fac, decks, torch, params, bits=64):
     return deck, out, (h, u, f, D, s)
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# multiply-adds per cell of the network as specified (2 -> 16 -> 16 -> 1
+# channels, 27 taps); the kernels pad every layer to 16 x 16 channels
+LEARNED_MACS_PER_CELL = 27 * (2 * 16 + 16 * 16 + 16 * 1)
+LEARNED_PADDED_MACS_PER_CELL = 27 * 16 * 16 * 3
+
+
+def learned_section(torch, seeds, steps, batch, log=print):
+    """C4 (learned smoother): pairs generated by the FAC solver on seeded
+    variable-coefficient problems (one-level 64^3 hierarchies of 16^3 patches,
+    residuals after 1, 2 and 3 FAC cycles, targets the FP64 block solutions),
+    training on the GPU by the library's own forward/backward kernels, the
+    matrix-core rate of the forward convolutions, and the gain on unseen seeds."""
+    from samrai_amd import learned as SL
+    t0 = time.perf_counter()
+    train = [SL.fac_training_pairs(torch, s, cycles=(1, 2, 3)) for s in range(seeds)]
+    test = [SL.fac_training_pairs(torch, 1000 + s, cycles=(1, 2, 3)) for s in range(2)]
+    cat = {k: torch.cat([p[k] for p in train]).contiguous() for k in ("r", "logd", "e")}
+    tst = {k: torch.cat([p[k] for p in test]).contiguous()
+           for k in ("r", "logd", "e", "dxf", "dyf", "dzf")}
+    h = train[0]["h"]
+    torch.cuda.synchronize()
+    gen_s = time.perf_counter() - t0
+    log("bench: learned: %d training blocks from %d seeds, %d test blocks (%.1f s)" %
+        (cat["r"].shape[0], seeds, tst["r"].shape[0], gen_s))
+    nmax = max(batch, tst["r"].shape[0])
+    net = SL.LearnedSmoother(torch, nmax, seed=0)
+
+    def quality():
+        e = net.correction(tst["r"], tst["logd"], h)
+        ne = torch.linalg.vector_norm(tst["e"])
+        both = SL.block_sor(torch, tst["r"], tst["dxf"], tst["dyf"], tst["dzf"], h, 1, 1.0, e0=e)
+        return (torch.linalg.vector_norm(e - tst["e"]) / ne).item(), \
+            (torch.linalg.vector_norm(both - tst["e"]) / ne).item()
+
+    err0, _ = quality()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    losses = net.train(cat["r"], cat["logd"], cat["e"], h, steps=steps, batch=batch, lr=2e-3,
+                       log_every=max(1, steps // 10))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t1
+    err1, err_both = quality()
+    ne = torch.linalg.vector_norm(tst["e"])
+    gs = [(torch.linalg.vector_norm(SL.block_sor(torch, tst["r"], tst["dxf"], tst["dyf"],
+                                                 tst["dzf"], h, k, 1.0) - tst["e"]) / ne).item()
+          for k in (1, 2, 4)]
+    # matrix-core rate of the forward pass (three conv launches) on the test blocks
+    rb, lb = tst["r"], tst["logd"]
+    nb = rb.shape[0]
+    net.forward(rb, lb, h)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(reps):
+        net.forward(rb, lb, h)
+    ev1.record()
+    torch.cuda.synchronize()
+    fwd_ms = ev0.elapsed_time(ev1) / reps
+    cells = nb * 4096
+    alg_tflops = 2.0 * LEARNED_MACS_PER_CELL * cells / (fwd_ms * 1e-3) / 1e12
+    padded_tflops = 2.0 * LEARNED_PADDED_MACS_PER_CELL * cells / (fwd_ms * 1e-3) / 1e12
+    # one red-black sweep of the same blocks (FP64), for the cost comparison
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(reps):
+        SL.block_sor(torch, rb, tst["dxf"], tst["dyf"], tst["dzf"], h, 1, 1.0)
+    ev1.record()
+    torch.cuda.synchronize()
+    gs_ms = ev0.elapsed_time(ev1) / reps
+    out = dict(
+        workload="pairs from FAC iterates (1-3 cycles, 2 coarse RBGS sweeps per cycle) on "
+                 "one-level 64^3 hierarchies of 16^3 patches, D = exp(GRF corr 0.1 sigma 1), "
+                 "8 Gaussian sources; target = FP64 solution of the block error equation "
+                 "(zero Dirichlet on the block faces, 400 SOR sweeps)",
+        network="3x3x3 conv 2->16->16->1, ReLU, BF16 weights/activations, FP32 accumulation "
+                "(v_mfma_f32_16x16x16_bf16), Adam on FP32 master weights",
+        train_blocks=int(cat["r"].shape[0]), train_seeds=seeds, test_blocks=int(tst["r"].shape[0]),
+        pair_generation_s=round(gen_s, 2), train_steps=steps, batch_blocks=batch,
+        train_s=round(train_s, 3),
+        train_cells_per_s=round(steps * batch * 4096 / train_s / 1e6, 1),
+        train_cells_per_s_unit="Mcell/s (forward + backward + Adam)",
+        loss_first_last=[losses[0][1], losses[-1][1]],
+        test_block_error_ratio=dict(untrained=round(err0, 4), trained=round(err1, 4),
+                                    trained_then_1_rbgs=round(err_both, 4),
+                                    rbgs_1=round(gs[0], 4), rbgs_2=round(gs[1], 4),
+                                    rbgs_4=round(gs[2], 4)),
+        forward_ms=round(fwd_ms, 4), forward_blocks=nb, rbgs_sweep_ms=round(gs_ms, 4),
+        roofline=dict(bound="mfma", achieved=round(alg_tflops, 2), peak=MFMA_BF16_PEAK_TFLOPS,
+                      unit="TFLOP/s", frac=round(alg_tflops / MFMA_BF16_PEAK_TFLOPS, 5),
+                      padded_tflops=round(padded_tflops, 2),
+                      kernel="k_conv (3 launches, forward)",
+                      flops_per_cell=2 * LEARNED_MACS_PER_CELL))
+    net.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,7 +277,13 @@ def main():
     ap.add_argument("--smoother-sweeps", type=int, default=200)
     ap.add_argument("--no-c2", action="store_true",
                     help="skip the variable-coefficient 4-level deck (C2) section")
-    ap.add_argument("--mode", choices=["solve", "smoother", "profile"], default="solve",
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-smoother (C4) section")
+    ap.add_argument("--learned-seeds", type=int, default=12)
+    ap.add_argument("--learned-steps", type=int, default=1500)
+    ap.add_argument("--learned-batch", type=int, default=64)
+    ap.add_argument("--mode", choices=["solve", "smoother", "profile", "learned"],
+                    default="solve",
                     help="smoother: only a few smoother launches and a 256 MiB copy of known "
                          "size (for hardware-counter passes); profile: only the solves of "
                          "--deck with --coarse (for kernel traces); neither prints the "
@@ -199,6 +303,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    if args.mode == "learned":
+        log("bench: learned %s" % json.dumps(learned_section(
+            torch, args.learned_seeds, args.learned_steps, args.learned_batch, log)))
+        return
     if args.mode == "profile":
         P = MG_PARAMS if args.coarse == "mg" else PARAMS
         if args.deck == "c2":
@@ -374,6 +486,12 @@ def main():
                   redblack_coarse=rb2, mg_coarse=mg2, mg_coarse_mixed_fp32=mg2m)
         print("bench: C2 %s" % json.dumps(c2), file=sys.stderr, flush=True)
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_section(torch, args.learned_seeds, args.learned_steps,
+                                  args.learned_batch, log)
+        log("bench: learned %s" % json.dumps(learned))
+
     cpu = None
     if not args.no_cpu_baseline:
         try:
@@ -402,7 +520,7 @@ def main():
                finest_smoother_gcells_per_s=round(finest_gcells, 2),
                fused_coarsest=fused, fused_tile=tile, fused_tile_autotune_us_per_sweep=tuning,
                roofline=roofline, cpu_baseline=cpu, with_mg_coarse_solver=mg,
-               mixed_precision=mixed_c1, c2=c2)
+               mixed_precision=mixed_c1, c2=c2, learned_smoother=learned)
     print(json.dumps(out), flush=True)
 
 

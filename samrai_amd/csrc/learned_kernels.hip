@@ -44,7 +44,7 @@ constexpr int WSZ = NTAP * CH * CH;    // conv weights per layer
 constexpr int PSZ = WSZ + CH;          // + bias
 constexpr int GSZ = (NTAP + 1) * CH * CH;  // gradient slab: 27 taps + bias row
 constexpr int NLAYER = 3;
-constexpr int WGRAD_ROWS = 32;         // rows accumulated per wave in the weight gradient
+constexpr int WGRAD_ROWS = 16;         // rows accumulated per wave in the weight gradient
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -75,7 +75,10 @@ __device__ inline bf16x4 load4(const uint16_t* p) {
 
 enum { MODE_HIDDEN = 0, MODE_FINAL = 1, MODE_BWD = 2 };
 
-// One wave per output row (16 cells along x); four rows per work-group.
+// One work-group per z-plane of a block (16 rows of 16 cells), four waves
+// of four rows each.  The plane's input neighbourhood (planes z-1..z+1,
+// rows -1..16, cells -1..16; zeros outside the block) is staged once in LDS
+// with 16-byte loads, and each wave keeps its 27 weight taps in registers.
 // Lane l: cell x = l & 15 of the A operand, channel group g = l >> 4.
 // A[i][k]  (lane l: i = l & 15, k = 4g..4g+3)  = input at (x + dx, y + dy, z + dz), channel k
 // B[k][n]  (lane l: n = l & 15, k = 4g..4g+3)  = w[t][n][k]
@@ -84,47 +87,68 @@ enum { MODE_HIDDEN = 0, MODE_FINAL = 1, MODE_BWD = 2 };
 // MODE_FINAL : pred[cell] = D[cell][0] + bias[0] (FP32).
 // MODE_BWD   : out = D * (mask[cell][n] > 0) as BF16 (gradient of the
 //              previous layer's pre-activation; mask = its ReLU output).
+constexpr int CONV_ROWS = 4;
+constexpr int HB = NB + 2;  // halo-padded side
+
 template <int MODE>
 __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x,
                                               const uint16_t* __restrict__ w,
                                               const float* __restrict__ bias,
                                               const uint16_t* __restrict__ mask,
                                               uint16_t* __restrict__ out,
-                                              float* __restrict__ pred, int nrows) {
+                                              float* __restrict__ pred, int nplanes) {
+  __shared__ uint4 tile[3 * HB * HB * 2];  // 31104 B: [pz][py][px][2 halves of 8 channels]
+  const int plane = blockIdx.x;
+  if (plane >= nplanes) return;  // whole work-group
+  const int b = plane / NB;
+  const int z = plane % NB;
+  const uint16_t* xb = x + (size_t)b * NCELL * CH;
+  for (int s = threadIdx.x; s < 3 * HB * HB * 2; s += 256) {
+    const int half = s & 1;
+    const int c = s >> 1;
+    const int px = c % HB, py = (c / HB) % HB, pz = c / (HB * HB);
+    const int xx = px - 1, yy = py - 1, zz = z + pz - 1;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if ((unsigned)xx < (unsigned)NB && (unsigned)yy < (unsigned)NB && (unsigned)zz < (unsigned)NB)
+      v = *reinterpret_cast<const uint4*>(xb + ((size_t)((zz * NB + yy) * NB + xx)) * CH +
+                                          8 * half);
+    tile[s] = v;
+  }
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= nrows) return;  // whole wave
-  const int b = row / NROW;
-  const int y = row % NB;
-  const int z = (row / NB) % NB;
+  const int wave = threadIdx.x >> 6;
   const int cx = lane & 15;
   const int g = lane >> 4;
-  const uint16_t* xb = x + (size_t)b * NCELL * CH;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  bf16x4 wr[NTAP];
 #pragma unroll
-  for (int t = 0; t < NTAP; ++t) {
-    const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-    const int xx = cx + dx, yy = y + dy, zz = z + dz;
-    bf16x4 a = {0, 0, 0, 0};
-    if ((unsigned)xx < (unsigned)NB && (unsigned)yy < (unsigned)NB && (unsigned)zz < (unsigned)NB)
-      a = load4(xb + ((size_t)((zz * NB + yy) * NB + xx)) * CH + 4 * g);
-    const bf16x4 bw = load4(w + (t * CH + cx) * CH + 4 * g);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
-  }
-  const size_t cell0 = (size_t)row * NB;  // first cell of the row (block-major)
+  for (int t = 0; t < NTAP; ++t) wr[t] = load4(w + (t * CH + cx) * CH + 4 * g);
+  const float bv = MODE == MODE_HIDDEN ? bias[cx] : (MODE == MODE_FINAL ? bias[0] : 0.f);
+  __syncthreads();
+  const uint16_t* lds = reinterpret_cast<const uint16_t*>(tile);
+  for (int r = 0; r < CONV_ROWS; ++r) {
+    const int y = wave * CONV_ROWS + r;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const size_t cell = cell0 + 4 * g + j;
-    float v = acc[j];
-    if (MODE == MODE_HIDDEN) {
-      v += bias[cx];
-      v = v > 0.f ? v : 0.f;
-      out[cell * CH + cx] = f2bf(v);
-    } else if (MODE == MODE_FINAL) {
-      if (cx == 0) pred[cell] = v + bias[0];
-    } else {
-      const bool on = bf2f(mask[cell * CH + cx]) > 0.f;
-      out[cell * CH + cx] = f2bf(on ? v : 0.f);
+    for (int t = 0; t < NTAP; ++t) {
+      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      const bf16x4 a = load4(lds + (((dz + 1) * HB + (y + dy + 1)) * HB + (cx + dx + 1)) * CH +
+                             4 * g);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wr[t], acc, 0, 0, 0);
+    }
+    const size_t cell0 = (size_t)(plane * NB + y) * NB;  // first cell of the row
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const size_t cell = cell0 + 4 * g + j;
+      float v = acc[j];
+      if (MODE == MODE_HIDDEN) {
+        v += bv;
+        v = v > 0.f ? v : 0.f;
+        out[cell * CH + cx] = f2bf(v);
+      } else if (MODE == MODE_FINAL) {
+        if (cx == 0) pred[cell] = v + bv;
+      } else {
+        const bool on = bf2f(mask[cell * CH + cx]) > 0.f;
+        out[cell * CH + cx] = f2bf(on ? v : 0.f);
+      }
     }
   }
 }
@@ -165,20 +189,26 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ x,
     bf16x4 a;
 #pragma unroll
     for (int j = 0; j < 4; ++j) a[j] = (short)gz[(cell0 + 4 * g + j) * CH + cx];
+    // per (dz, dy): the six cells x = 4g-1 .. 4g+4 of channel ci serve all three dx taps
 #pragma unroll
-    for (int t = 0; t < NTAP; ++t) {
-      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+    for (int q = 0; q < 9; ++q) {
+      const int dz = q / 3 - 1, dy = q % 3 - 1;
       const int yy = y + dy, zz = z + dz;
-      bf16x4 bx = {0, 0, 0, 0};
+      short v[6] = {0, 0, 0, 0, 0, 0};
       if ((unsigned)yy < (unsigned)NB && (unsigned)zz < (unsigned)NB) {
         const uint16_t* xr = xb + (size_t)((zz * NB + yy) * NB) * CH + cx;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int xx = 4 * g + j + dx;
-          if ((unsigned)xx < (unsigned)NB) bx[j] = (short)xr[xx * CH];
+        for (int k = 0; k < 6; ++k) {
+          const int xx = 4 * g - 1 + k;
+          if ((unsigned)xx < (unsigned)NB) v[k] = (short)xr[xx * CH];
         }
       }
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx, acc[t], 0, 0, 0);
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const bf16x4 bx = {v[1 + dx], v[2 + dx], v[3 + dx], v[4 + dx]};
+        const int t = 3 * q + dx + 1;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bx, acc[t], 0, 0, 0);
+      }
     }
     acc[NTAP] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, ones, acc[NTAP], 0, 0, 0);
   }
@@ -189,13 +219,26 @@ __global__ __launch_bounds__(256) void k_wgrad(const uint16_t* __restrict__ x,
     for (int j = 0; j < 4; ++j) p[(t * CH + 4 * g + j) * CH + cx] = acc[t][j];
 }
 
-__global__ void k_wgrad_reduce(const float* __restrict__ partial, int nwaves,
-                               float* __restrict__ grad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= GSZ) return;
+// Sum of the per-wave slabs: a work-group per 16 outputs, 16 threads per
+// output each summing every 16th slab in order, then the 16 partial sums in
+// a fixed order (deterministic).
+__global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ partial,
+                                                      int nwaves, float* __restrict__ grad) {
+  __shared__ float red[16][17];
+  const int o = threadIdx.x & 15;
+  const int part = threadIdx.x >> 4;
+  const int i = blockIdx.x * 16 + o;
   float s = 0.f;
-  for (int w = 0; w < nwaves; ++w) s += partial[(size_t)w * GSZ + i];
-  grad[i] = s;
+  if (i < GSZ)
+    for (int w = part; w < nwaves; w += 16) s += partial[(size_t)w * GSZ + i];
+  red[part][o] = s;
+  __syncthreads();
+  if (part == 0 && i < GSZ) {
+    float t = 0.f;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) t += red[k][o];
+    grad[i] = t;
+  }
 }
 
 // Features of one block per work-group (256 threads, 16 cells each):
@@ -501,14 +544,18 @@ int wgrad_waves(int nrows) { return (nrows + WGRAD_ROWS - 1) / WGRAD_ROWS; }
 
 void launch_conv(int mode, const uint16_t* x, const uint16_t* w, const float* bias,
                  const uint16_t* mask, uint16_t* out, float* pred, int nblocks, hipStream_t s) {
-  const int nrows = nblocks * NROW;
-  const dim3 grid((nrows + 3) / 4), block(256);
+  static_assert(4 * CONV_ROWS == NB, "a work-group's four waves cover one z-plane");
+  const int nplanes = nblocks * NB;
+  const dim3 grid(nplanes), block(256);
   if (mode == MODE_HIDDEN)
-    hipLaunchKernelGGL(k_conv<MODE_HIDDEN>, grid, block, 0, s, x, w, bias, mask, out, pred, nrows);
+    hipLaunchKernelGGL(k_conv<MODE_HIDDEN>, grid, block, 0, s, x, w, bias, mask, out, pred,
+                       nplanes);
   else if (mode == MODE_FINAL)
-    hipLaunchKernelGGL(k_conv<MODE_FINAL>, grid, block, 0, s, x, w, bias, mask, out, pred, nrows);
+    hipLaunchKernelGGL(k_conv<MODE_FINAL>, grid, block, 0, s, x, w, bias, mask, out, pred,
+                       nplanes);
   else
-    hipLaunchKernelGGL(k_conv<MODE_BWD>, grid, block, 0, s, x, w, bias, mask, out, pred, nrows);
+    hipLaunchKernelGGL(k_conv<MODE_BWD>, grid, block, 0, s, x, w, bias, mask, out, pred,
+                       nplanes);
 }
 
 void launch_wgrad(const uint16_t* x, const uint16_t* gz, float* partial, float* grad, int nblocks,
@@ -516,7 +563,7 @@ void launch_wgrad(const uint16_t* x, const uint16_t* gz, float* partial, float* 
   const int nrows = nblocks * NROW;
   const int nw = wgrad_waves(nrows);
   hipLaunchKernelGGL(k_wgrad, dim3((nw + 3) / 4), dim3(256), 0, s, x, gz, partial, nrows, nw);
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3((GSZ + 255) / 256), dim3(256), 0, s, partial, nw, grad);
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3((GSZ + 15) / 16), dim3(256), 0, s, partial, nw, grad);
 }
 
 void forward(Net* n, int nblocks, hipStream_t s) {

@@ -125,7 +125,7 @@ def test_network_loss_and_gradients_match_oracle(torch_gpu):
         ref = grads_o[l]
         err = np.linalg.norm(grads[l] - ref) / np.linalg.norm(ref)
         _say("layer %d: relative gradient error %.2e" % (l, err))
-        assert err < 1e-2, (l, err)
+        assert err < 1e-4, (l, err)
     net.close()
 
 
