@@ -240,7 +240,34 @@ def learned_section(torch, seeds, steps, batch, log=print):
     ev1.record()
     torch.cuda.synchronize()
     gs_ms = ev0.elapsed_time(ev1) / reps
+    # smoothing a level of 16^3 patches (unseen seed, 64^3 cells): red-black
+    # (2 sweeps) per iteration, with and without the learned correction first;
+    # the FP64 RMS residual judges both
+    P = SL.BlockLevelProblem(torch, 2000)
+    sv = P.solver()
+    level = {}
+    for learned in (False, True):
+        P.u.fill(0.0)
+        hist = []
+        t_it = 0.0
+        for it in range(8):
+            sv.synchronize()
+            t2 = time.perf_counter()
+            if learned:
+                sv.residual(P.r, P.u, P.f, 0, False)
+                SL.learned_correct(sv, net, P.u, P.r, P.D, 0)
+            sv.smooth(P.u, P.f, 0, 2)
+            sv.synchronize()
+            t_it += time.perf_counter() - t2
+            sv.residual(P.r, P.u, P.f, 0, False)
+            hist.append(sv.rms_norm(P.r))
+        level["learned_then_rbgs2" if learned else "rbgs2"] = dict(
+            rms_residual=[float("%.4e" % v) for v in hist], ms_per_iteration=round(t_it / 8 * 1e3, 3))
+    sv.close()
+    P.close()
     out = dict(
+        level_smoothing=dict(workload="one-level 64^3 problem of 16^3 patches (unseen seed), "
+                                      "8 iterations from u = 0", **level),
         workload="pairs from FAC iterates (1-3 cycles, 2 coarse RBGS sweeps per cycle) on "
                  "one-level 64^3 hierarchies of 16^3 patches, D = exp(GRF corr 0.1 sigma 1), "
                  "8 Gaussian sources; target = FP64 solution of the block error equation "

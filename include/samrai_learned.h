@@ -25,6 +25,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "samrai_fac.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -87,6 +89,17 @@ int sfac_learned_loss(void* net, float* loss);
 /* Device buffers of the last features/forward: features (BF16, 16 channels per
  * cell), prediction (FP32 per cell), per-block scale s (FP64). */
 int sfac_learned_buffers(void* net, void** features, float** pred, double** scale);
+
+/* The learned correction on level ln of a solver's hierarchy whose patches
+ * on that level are all 16x16x16 (cubic cells): u += omega * s * net(r h^2 / s,
+ * log D) on every patch interior.  r: cell data with ghost 0 (for example
+ * from sfac_ops_residual); D: the side data given to the solver (ghost 0),
+ * or NULL for log D = 0; u: cell data of any ghost width (ghosts are not
+ * touched: fill them before the next smoothing).  Runs on the solver's
+ * stream; the network must hold at least as many blocks as the level has
+ * patches.  A smoothError implementation calls it before red-black sweeps. */
+int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac_patch_data* r,
+                             sfac_patch_data* D, int ln, double omega);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,9 @@ int guarded(F&& f) {
 namespace samrai_amd {
 // Used by entry points defined in other translation units (learned smoother).
 void set_last_error(const std::string& msg) { g_last_error = msg; }
+int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
+                          double* const* const* dtab, int nblocks, double h, double omega,
+                          hipStream_t s);
 }  // namespace samrai_amd
 
 namespace {
@@ -509,6 +512,39 @@ int sfac_ops_residual(sfac_solver* s, sfac_patch_data* r, sfac_patch_data* u,
     solv::SAMRAIVectorReal rv = vec(s, r), uv = vec(s, u), fv = vec(s, f);
     s->s->ops().computeCompositeResidualOnLevel(rv, uv, fv, ln, homogeneous != 0);
   });
+}
+
+int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac_patch_data* r,
+                             sfac_patch_data* D, int ln, double omega) {
+  int rc = guarded([&] {
+    need_init(s);
+    need(net, "learned smoother");
+    need(u, "u");
+    need(r, "r");
+    const hier::PatchHierarchy& H = *s->h->h;
+    if (H.dim() != 3) fail("learned correction: 3-D hierarchies only");
+    if (ln < 0 || ln >= H.numberOfLevels()) fail("learned correction: level out of range");
+    const hier::PatchLevel& L = H.level(ln);
+    for (const Box& b : L.boxes)
+      for (int d = 0; d < 3; ++d)
+        if (b.hi[d] - b.lo[d] + 1 != 16)
+          fail("learned correction: every patch of the level must be 16x16x16");
+    if (L.dx[0] != L.dx[1] || L.dx[0] != L.dx[2])
+      fail("learned correction: cells must be cubes");
+    if (u->d->centring() != pdat::PatchDataSet::kCell || r->d->centring() != pdat::PatchDataSet::kCell)
+      fail("learned correction: u and r must be cell data");
+    if (r->d->ghost() != 0) fail("learned correction: r must have ghost width 0");
+    if (D && (D->d->centring() != pdat::PatchDataSet::kSide || D->d->ghost() != 0))
+      fail("learned correction: D must be side data with ghost width 0");
+  });
+  if (rc != 0) return rc;
+  const hier::PatchLevel& L = s->h->h->level(ln);
+  double* const* dt[3] = {nullptr, nullptr, nullptr};
+  if (D)
+    for (int a = 0; a < 3; ++a) dt[a] = D->d->table(ln, a);
+  return samrai_amd::learned_correct_level(net, u->d->table(ln), u->d->ghost(), r->d->table(ln),
+                                           D ? dt : nullptr, (int)L.boxes.size(), L.dx[0], omega,
+                                           s->s->ops().stream());
 }
 
 int sfac_ops_restrict_solution(sfac_solver* s, sfac_patch_data* u, int dest_ln) {

@@ -482,6 +482,42 @@ __global__ __launch_bounds__(256) void k_block_residual(const double* __restrict
   }
 }
 
+// Level <-> blocks (every patch of the level is one 16^3 block).  Patch
+// arrays are ArrayData layout (x fastest over the ghost box); cell data with
+// ghost 0 and side data with ghost 0 have the block layouts directly.
+__global__ void k_gather_blocks(double* const* __restrict__ tab, int nblocks,
+                                double* __restrict__ out) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)nblocks * NCELL) return;
+  out[i] = tab[i / NCELL][i % NCELL];
+}
+
+__global__ void k_logd_blocks(double* const* __restrict__ tx, double* const* __restrict__ ty,
+                              double* const* __restrict__ tz, int nblocks,
+                              double* __restrict__ logd) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)nblocks * NCELL) return;
+  const int b = (int)(i / NCELL), c = (int)(i % NCELL);
+  const int x = c % NB, y = (c / NB) % NB, z = c / (NB * NB);
+  const double* X = tx[b];
+  const double* Y = ty[b];
+  const double* Z = tz[b];
+  const double s = log(X[(z * NB + y) * (NB + 1) + x]) + log(X[(z * NB + y) * (NB + 1) + x + 1]) +
+                   log(Y[(z * (NB + 1) + y) * NB + x]) + log(Y[(z * (NB + 1) + y + 1) * NB + x]) +
+                   log(Z[(z * NB + y) * NB + x]) + log(Z[((z + 1) * NB + y) * NB + x]);
+  logd[i] = s / 6.0;
+}
+
+__global__ void k_add_blocks(double* const* __restrict__ tab, int ghost, int nblocks,
+                             const double* __restrict__ e) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= (long)nblocks * NCELL) return;
+  const int b = (int)(i / NCELL), c = (int)(i % NCELL);
+  const int x = c % NB, y = (c / NB) % NB, z = c / (NB * NB);
+  const int n = NB + 2 * ghost;
+  tab[b][((z + ghost) * n + (y + ghost)) * n + (x + ghost)] += e[i];
+}
+
 // ---------------------------------------------------------------------------
 struct Net {
   int max_blocks = 0;
@@ -499,6 +535,7 @@ struct Net {
   float* partial = nullptr;
   float* loss_partial = nullptr;
   float* loss = nullptr;
+  double* blk[3] = {};         // level gathers: residual, log D, correction
   int step = 0;
   int cin[NLAYER] = {2, CH, CH};
   int cout[NLAYER] = {CH, CH, 1};
@@ -642,6 +679,7 @@ void free_net(Net* n) {
   (void)hipFree(n->partial);
   (void)hipFree(n->loss_partial);
   (void)hipFree(n->loss);
+  for (int k = 0; k < 3; ++k) (void)hipFree(n->blk[k]);
   delete n;
 }
 
@@ -652,6 +690,41 @@ Net* net_of(void* h, int nblocks) {
   return n;
 }
 
+}  // namespace
+
+namespace samrai_amd {
+// The learned correction of one level whose patches are all 16^3 blocks:
+// u += omega * s * net(r h^2 / s, log D), r cell data with ghost 0, D side
+// data with ghost 0 (dtab null: log D = 0), u cell data with ghost `ughost`.
+// Called by sfac_ops_learned_correct (capi.cpp) after its checks.
+int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
+                          double* const* const* dtab, int nblocks, double h, double omega,
+                          hipStream_t s) {
+  return guarded([&] {
+    Net* n = net_of(net, nblocks);
+    need(utab && rtab, "null patch tables");
+    need(h > 0.0, "h must be positive");
+    if (nblocks == 0) return;
+    const int ncells = nblocks * NCELL;
+    const dim3 grid((ncells + 255) / 256), block(256);
+    hipLaunchKernelGGL(k_gather_blocks, grid, block, 0, s, rtab, nblocks, n->blk[0]);
+    const double* logd = nullptr;
+    if (dtab) {
+      hipLaunchKernelGGL(k_logd_blocks, grid, block, 0, s, dtab[0], dtab[1], dtab[2], nblocks,
+                         n->blk[1]);
+      logd = n->blk[1];
+    }
+    hipLaunchKernelGGL(k_features, dim3(nblocks), dim3(256), 0, s, n->blk[0], logd, h * h,
+                       n->act[0], n->scale);
+    forward(n, nblocks, s);
+    hipLaunchKernelGGL(k_correction, grid, block, 0, s, n->pred, n->scale, omega, ncells,
+                       n->blk[2]);
+    hipLaunchKernelGGL(k_add_blocks, grid, block, 0, s, utab, ughost, nblocks, n->blk[2]);
+  });
+}
+}  // namespace samrai_amd
+
+namespace {
 }  // namespace
 
 extern "C" {
@@ -739,6 +812,7 @@ int sfac_learned_create(int max_blocks, unsigned long long seed, void** out) {
       n->partial = dalloc<float>((size_t)wgrad_waves(max_blocks * NROW) * GSZ);
       n->loss_partial = dalloc<float>((ncells + 255) / 256);
       n->loss = dalloc<float>(1);
+      for (int k = 0; k < 3; ++k) n->blk[k] = dalloc<double>(ncells);
       init_params(n, seed, 0);
       ck(hipStreamSynchronize(0));
     } catch (...) {

@@ -57,6 +57,7 @@ def lib():
     L.sfac_learned_set_params.argtypes = [_vp, _vp]
     L.sfac_learned_loss.argtypes = [_vp, _vp]
     L.sfac_learned_buffers.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]
+    L.sfac_ops_learned_correct.argtypes = [_vp, _vp, _vp, _vp, _vp, i, d]
     _bound = True
     return L
 
@@ -218,6 +219,78 @@ class LearnedSmoother:
             if log_every and (k % log_every == 0 or k == steps - 1):
                 losses.append((k, self.loss()))
         return losses
+
+
+def learned_correct(solver, net, u, r, D, ln, omega=1.0):
+    """u += omega * s * net(r h^2 / s, log D) on every 16^3 patch of level ln
+    (sfac_ops_learned_correct); r: cell data with ghost 0, D: side data or None."""
+    fac._check(lib().sfac_ops_learned_correct(solver.s, net.h, u.d, r.d,
+                                              None if D is None else D.d, int(ln), float(omega)))
+
+
+class BlockLevelProblem:
+    """A seeded one-level n^3 hierarchy tiled by 16^3 patches on the unit cube:
+    D = exp(g) on faces (g a Gaussian random field, correlation length 0.1,
+    unit variance), f = 8 Gaussian sources of width 0.02, homogeneous
+    Dirichlet.  Holds u (ghost 1), f and a residual r (ghost 0), D, and the
+    block face arrays and cell log D of every patch (patch p = block p)."""
+
+    def __init__(self, torch, seed, n=64):
+        from . import decks
+        assert n % NB == 0
+        m = n // NB
+        rng = np.random.default_rng(5000 + seed)
+        deck = dict(name="block-level-s%d" % seed, dim=3, ratio=2, dlo=(0, 0, 0),
+                    dhi=(n - 1,) * 3, centres=rng.uniform(0.1, 0.9, (8, 3)),
+                    amps=rng.uniform(-1.0, 1.0, 8), levels=[])
+        self.boxes = [((a * NB, b * NB, c * NB),
+                       (a * NB + NB - 1, b * NB + NB - 1, c * NB + NB - 1))
+                      for c in range(m) for b in range(m) for a in range(m)]
+        deck["levels"] = [self.boxes]
+        self.torch = torch
+        self.h = 1.0 / n
+        self.hier = fac.PatchHierarchy(3, deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
+        self.hier.add_level(2, self.boxes)
+        self.hier.finalize()
+        self.u = fac.PatchData(self.hier, 1)
+        self.f = fac.PatchData(self.hier, 0)
+        self.r = fac.PatchData(self.hier, 0)
+        self.D = fac.PatchData(self.hier, 0, "side")
+        modes = decks.grf_modes(seed)
+        kw = dict(dtype=torch.float64, device="cuda")
+        faces = [[], [], []]
+        for p, box in enumerate(self.boxes):
+            fv = decks.gaussian_sources(deck, decks.cell_points(deck, 0, box, xp=torch, **kw),
+                                        xp=torch)
+            self.f.put_device(0, p, fv.permute(2, 1, 0).contiguous())
+            for a in range(3):
+                g = decks.grf_eval(decks.face_points(deck, 0, box, a, xp=torch, **kw), modes,
+                                   xp=torch)
+                dv = torch.exp(g).permute(2, 1, 0).contiguous()
+                self.D.put_device(0, p, dv, axis=a)
+                faces[a].append(dv)
+        self.dxf, self.dyf, self.dzf = (torch.stack(fl).contiguous() for fl in faces)
+        self.logd = (torch.log(self.dxf[..., :-1]) + torch.log(self.dxf[..., 1:]) +
+                     torch.log(self.dyf[:, :, :-1]) + torch.log(self.dyf[:, :, 1:]) +
+                     torch.log(self.dzf[:, :-1]) + torch.log(self.dzf[:, 1:])) / 6.0
+
+    def solver(self, **params):
+        s = fac.CellPoissonFACSolver(self.hier, **params)
+        s.set_d_patch_data(self.D)
+        s.set_dirichlet(0.0)
+        s.initialize_solver_state(self.u, self.f)
+        return s
+
+    def residual_blocks(self):
+        """The residual patch data as a [patches, 16, 16, 16] device tensor."""
+        torch = self.torch
+        return torch.stack([torch.as_tensor(self.r.get(0, p), device="cuda").view(NB, NB, NB)
+                            for p in range(len(self.boxes))]).contiguous()
+
+    def close(self):
+        for x in (self.u, self.f, self.r, self.D):
+            x.close()
+        self.hier.close()
 
 
 def fac_training_pairs(torch, seed, cycles=(1, 2, 3), n=64, coarse_sweeps=2, sor_sweeps=400,

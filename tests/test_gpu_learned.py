@@ -208,3 +208,55 @@ def test_training_on_fac_pairs_gains_over_no_network(torch_gpu):
     assert err1 < err_gs        # the trained network beats one red-black sweep on the block error
     assert err_both < err_gs    # and still helps when the sweep follows it
     net.close()
+
+
+def _interior_blocks(torch, P, x):
+    """Interior cells of every patch of cell data x (ghost 1) as [p, 16, 16, 16]."""
+    return torch.stack([torch.as_tensor(x.get(0, p), device="cuda").view(18, 18, 18)[1:-1, 1:-1, 1:-1]
+                        for p in range(len(P.boxes))]).contiguous()
+
+
+def test_level_learned_correction_and_smoothing(torch_gpu):
+    torch = torch_gpu
+    from samrai_amd import learned as SL
+    _say("learned correction of a level of 16^3 patches (sfac_ops_learned_correct)")
+    train = [SL.fac_training_pairs(torch, s, cycles=(1, 2)) for s in range(4)]
+    cat = {k: torch.cat([p[k] for p in train]).contiguous() for k in ("r", "logd", "e")}
+    net = SL.LearnedSmoother(torch, 128, seed=3)
+    net.train(cat["r"], cat["logd"], cat["e"], train[0]["h"], steps=600, batch=32, lr=2e-3)
+    P = SL.BlockLevelProblem(torch, 11)
+    s = P.solver()
+    # the entry point applies exactly the block path: gather r, network, add into u
+    P.u.fill(0.0)
+    s.residual(P.r, P.u, P.f, 0, False)
+    rb = P.residual_blocks()
+    e_ref = net.correction(rb, None, P.h)
+    SL.learned_correct(s, net, P.u, P.r, None, 0)
+    s.synchronize()
+    assert torch.equal(_interior_blocks(torch, P, P.u), e_ref)
+    P.u.fill(0.0)
+    e_ref = net.correction(rb, P.logd, P.h)
+    SL.learned_correct(s, net, P.u, P.r, P.D, 0)
+    s.synchronize()
+    got = _interior_blocks(torch, P, P.u)
+    assert (torch.linalg.vector_norm(got - e_ref) / torch.linalg.vector_norm(e_ref)).item() < 1e-3
+    # smoothing the level: red-black (2 sweeps) per iteration, with and without
+    # the learned correction first; the FP64 residual judges both
+    hist = {}
+    for learned in (False, True):
+        P.u.fill(0.0)
+        h = []
+        for it in range(6):
+            if learned:
+                s.residual(P.r, P.u, P.f, 0, False)
+                SL.learned_correct(s, net, P.u, P.r, P.D, 0)
+            s.smooth(P.u, P.f, 0, 2)
+            s.residual(P.r, P.u, P.f, 0, False)
+            h.append(s.rms_norm(P.r))
+        hist[learned] = h
+    _say("RMS residual, red-black only: %s" % ["%.3e" % v for v in hist[False]])
+    _say("RMS residual, learned + red-black: %s" % ["%.3e" % v for v in hist[True]])
+    assert hist[True][-1] < hist[False][-1]
+    s.close()
+    P.close()
+    net.close()
