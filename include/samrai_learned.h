@@ -101,6 +101,19 @@ int sfac_learned_buffers(void* net, void** features, float** pred, double** scal
 int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac_patch_data* r,
                              sfac_patch_data* D, int ln, double omega);
 
+/* Attach the network to a solver (net NULL detaches): CellPoissonFACOps::
+ * smoothError then applies omega times the learned correction of the error
+ * equation's residual before its red-black sweeps, on every level above the
+ * coarsest whose patches are all 16x16x16 with cubic cells.  Each
+ * solveSystem starts with it enabled and falls back to red-black alone for
+ * the rest of the solve after a cycle whose residual reduction factor
+ * exceeds 0.5; the FP64 residual remains the convergence test.  The network
+ * must hold at least as many blocks as such a level has patches. */
+int sfac_solver_set_learned_smoother(sfac_solver* s, void* net, double omega);
+/* 1 if the learned correction is attached and enabled (it is switched off
+ * by the fallback), 0 if not, -1 for a null solver. */
+int sfac_solver_learned_active(sfac_solver* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -514,6 +514,19 @@ int sfac_ops_residual(sfac_solver* s, sfac_patch_data* r, sfac_patch_data* u,
   });
 }
 
+int sfac_solver_set_learned_smoother(sfac_solver* s, void* net, double omega) {
+  return guarded([&] {
+    need(s, "solver");
+    if (!(omega > 0.0)) fail("learned smoother: omega must be positive");
+    s->s->ops().setLearnedSmoother(net, omega);
+  });
+}
+
+int sfac_solver_learned_active(sfac_solver* s) {
+  if (!s) return -1;
+  return s->s->ops().learnedActive() ? 1 : 0;
+}
+
 int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac_patch_data* r,
                              sfac_patch_data* D, int ln, double omega) {
   int rc = guarded([&] {

@@ -215,6 +215,8 @@ bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
   if ((int)d_factors.size() < d_max_cycles) d_factors.resize(d_max_cycles);
   d_history.clear();
   d_iterations = 0;
+  // the learned smoother (if attached) starts each solve enabled
+  d_gpu->setLearnedEnabled(true);
   const hipStream_t stream = d_gpu->stream();
   do {
     d_factors[d_iterations] = d_residual_norm;
@@ -254,6 +256,10 @@ bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
     ++d_iterations;
     d_net_factor = d_residual_norm / (initial + 1e-20);
     d_avg_factor = std::pow(d_net_factor, 1.0 / d_iterations);
+    // fallback: red-black alone once a cycle with the learned correction
+    // reduces the residual by less than a factor of two
+    if (d_gpu->learnedActive() && d_factors[d_iterations - 1] > 0.5)
+      d_gpu->setLearnedEnabled(false);
     d_ops.postprocessOneCycle(d_iterations - 1, u, d_r);
   } while (d_residual_norm > eff && d_iterations < d_max_cycles);
   hip_check(hipStreamSynchronize(d_gpu->stream()), "hipStreamSynchronize");

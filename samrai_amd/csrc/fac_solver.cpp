@@ -17,6 +17,11 @@
 #include <cstring>
 
 namespace samrai_amd {
+// learned_kernels.hip: the learned correction of a level of 16^3 patches
+int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
+                          double* const* const* dtab, int nblocks, double h, double omega,
+                          hipStream_t s);
+
 namespace pdat {
 
 PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost)
@@ -843,8 +848,41 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
   return num_sweeps;
 }
 
+void CellPoissonFACOps::setLearnedSmoother(void* net, double omega) {
+  d_learned = net;
+  d_learned_omega = omega;
+  d_learned_on = true;
+  if (net && !d_learned_r) d_learned_r.reset(new PatchDataSet(d_h, PatchDataSet::kCell, 0));
+  ++d_generation;
+}
+
+bool CellPoissonFACOps::learnedLevel(int ln) const {
+  if (d_h.dim() != 3 || ln <= d_ln_min) return false;
+  const hier::PatchLevel& L = d_h.level(ln);
+  if (L.dx[0] != L.dx[1] || L.dx[0] != L.dx[2]) return false;
+  for (const Box& b : L.boxes)
+    for (int d = 0; d < 3; ++d)
+      if (b.hi[d] - b.lo[d] + 1 != 16) return false;
+  return true;
+}
+
 void CellPoissonFACOps::smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal& r, int ln,
                                     int num_sweeps) {
+  if (learnedActive() && learnedLevel(ln)) {
+    // residual of the error equation on this level, then the learned correction
+    SAMRAIVectorReal tr = e;
+    tr.data = d_learned_r.get();
+    computeCompositeResidualOnLevel(tr, e, r, ln, true);
+    PatchDataSet* D = d_spec.getD();
+    if (D && D->ghost() != 0) fail("learned smoother: D must have ghost width 0");
+    double* const* dt[3] = {nullptr, nullptr, nullptr};
+    if (D)
+      for (int a = 0; a < 3; ++a) dt[a] = D->table(ln, a);
+    if (samrai_amd::learned_correct_level(d_learned, e.data->table(ln), e.data->ghost(), d_learned_r->table(ln),
+                              D ? dt : nullptr, (int)d_h.level(ln).boxes.size(),
+                              d_h.level(ln).dx[0], d_learned_omega, d_stream) != 0)
+      fail("learned smoother: correction failed");
+  }
   smoothErrorByRedBlack(*e.data, *r.data, ln, num_sweeps, -1.0);
 }
 

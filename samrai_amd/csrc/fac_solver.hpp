@@ -203,6 +203,19 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   // (the residuals the outer loop tests stay FP64), and the multigrid
   // tolerance is floored at kCoarseFP32Tol.
   void setCoarsePrecision(int bits);
+  // Learned block correction (learned_kernels.hip) before the red-black
+  // sweeps of smoothError on every level above the coarsest whose patches
+  // are all 16^3 with cubic cells: the error equation's residual on the level
+  // goes through the network and omega * its correction is added to e.  net
+  // null switches it off.  setLearnedEnabled is the per-solve fallback switch.
+  void setLearnedSmoother(void* net, double omega);
+  bool learnedActive() const { return d_learned != nullptr && d_learned_on; }
+  void setLearnedEnabled(bool on) {
+    if (on != d_learned_on) {
+      d_learned_on = on;
+      ++d_generation;
+    }
+  }
   int coarsePrecision() const { return d_coarse_fp32 ? 32 : 64; }
   static constexpr double kCoarseFP32Tol = 1e-6;
 
@@ -270,6 +283,11 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   void solveBoxMG(pdat::PatchDataSet& e, const pdat::PatchDataSet& r);
   bool d_use_graphs = true;
   unsigned long d_generation = 0;
+  void* d_learned = nullptr;
+  double d_learned_omega = 1.0;
+  bool d_learned_on = true;
+  std::unique_ptr<pdat::PatchDataSet> d_learned_r;
+  bool learnedLevel(int ln) const;
   int d_box_variant = -1;
   void setupBoxPath();
   void releaseBoxPath();

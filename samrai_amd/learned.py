@@ -58,6 +58,8 @@ def lib():
     L.sfac_learned_loss.argtypes = [_vp, _vp]
     L.sfac_learned_buffers.argtypes = [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]
     L.sfac_ops_learned_correct.argtypes = [_vp, _vp, _vp, _vp, _vp, i, d]
+    L.sfac_solver_set_learned_smoother.argtypes = [_vp, _vp, d]
+    L.sfac_solver_learned_active.argtypes = [_vp]
     _bound = True
     return L
 
@@ -228,29 +230,47 @@ def learned_correct(solver, net, u, r, D, ln, omega=1.0):
                                               None if D is None else D.d, int(ln), float(omega)))
 
 
+def set_learned_smoother(solver, net, omega=1.0):
+    """Attach the network to a solver's smoothError (None detaches)."""
+    fac._check(lib().sfac_solver_set_learned_smoother(solver.s, None if net is None else net.h,
+                                                      float(omega)))
+
+
+def learned_active(solver):
+    return lib().sfac_solver_learned_active(solver.s) == 1
+
+
 class BlockLevelProblem:
-    """A seeded one-level n^3 hierarchy tiled by 16^3 patches on the unit cube:
+    """A seeded hierarchy tiled by 16^3 patches on the unit cube (one n^3 level,
+    or with two_level an (n/2)^3 level under a full n^3 level):
     D = exp(g) on faces (g a Gaussian random field, correlation length 0.1,
     unit variance), f = 8 Gaussian sources of width 0.02, homogeneous
     Dirichlet.  Holds u (ghost 1), f and a residual r (ghost 0), D, and the
     block face arrays and cell log D of every patch (patch p = block p)."""
 
-    def __init__(self, torch, seed, n=64):
+    def __init__(self, torch, seed, n=64, two_level=False):
         from . import decks
-        assert n % NB == 0
-        m = n // NB
+        assert n % NB == 0 and (not two_level or n % (2 * NB) == 0)
         rng = np.random.default_rng(5000 + seed)
+        n0 = n // 2 if two_level else n
         deck = dict(name="block-level-s%d" % seed, dim=3, ratio=2, dlo=(0, 0, 0),
-                    dhi=(n - 1,) * 3, centres=rng.uniform(0.1, 0.9, (8, 3)),
+                    dhi=(n0 - 1,) * 3, centres=rng.uniform(0.1, 0.9, (8, 3)),
                     amps=rng.uniform(-1.0, 1.0, 8), levels=[])
-        self.boxes = [((a * NB, b * NB, c * NB),
-                       (a * NB + NB - 1, b * NB + NB - 1, c * NB + NB - 1))
-                      for c in range(m) for b in range(m) for a in range(m)]
-        deck["levels"] = [self.boxes]
+
+        def tiling(nn):
+            m = nn // NB
+            return [((a * NB, b * NB, c * NB),
+                     (a * NB + NB - 1, b * NB + NB - 1, c * NB + NB - 1))
+                    for c in range(m) for b in range(m) for a in range(m)]
+
+        deck["levels"] = [tiling(n0)] + ([tiling(n)] if two_level else [])
+        self.fine = len(deck["levels"]) - 1
+        self.boxes = deck["levels"][self.fine]
         self.torch = torch
         self.h = 1.0 / n
         self.hier = fac.PatchHierarchy(3, deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
-        self.hier.add_level(2, self.boxes)
+        for boxes in deck["levels"]:
+            self.hier.add_level(2, boxes)
         self.hier.finalize()
         self.u = fac.PatchData(self.hier, 1)
         self.f = fac.PatchData(self.hier, 0)
@@ -259,16 +279,18 @@ class BlockLevelProblem:
         modes = decks.grf_modes(seed)
         kw = dict(dtype=torch.float64, device="cuda")
         faces = [[], [], []]
-        for p, box in enumerate(self.boxes):
-            fv = decks.gaussian_sources(deck, decks.cell_points(deck, 0, box, xp=torch, **kw),
-                                        xp=torch)
-            self.f.put_device(0, p, fv.permute(2, 1, 0).contiguous())
-            for a in range(3):
-                g = decks.grf_eval(decks.face_points(deck, 0, box, a, xp=torch, **kw), modes,
-                                   xp=torch)
-                dv = torch.exp(g).permute(2, 1, 0).contiguous()
-                self.D.put_device(0, p, dv, axis=a)
-                faces[a].append(dv)
+        for ln, boxes in enumerate(deck["levels"]):
+            for p, box in enumerate(boxes):
+                fv = decks.gaussian_sources(deck, decks.cell_points(deck, ln, box, xp=torch, **kw),
+                                            xp=torch)
+                self.f.put_device(ln, p, fv.permute(2, 1, 0).contiguous())
+                for a in range(3):
+                    g = decks.grf_eval(decks.face_points(deck, ln, box, a, xp=torch, **kw), modes,
+                                       xp=torch)
+                    dv = torch.exp(g).permute(2, 1, 0).contiguous()
+                    self.D.put_device(ln, p, dv, axis=a)
+                    if ln == self.fine:
+                        faces[a].append(dv)
         self.dxf, self.dyf, self.dzf = (torch.stack(fl).contiguous() for fl in faces)
         self.logd = (torch.log(self.dxf[..., :-1]) + torch.log(self.dxf[..., 1:]) +
                      torch.log(self.dyf[:, :, :-1]) + torch.log(self.dyf[:, :, 1:]) +
@@ -284,7 +306,7 @@ class BlockLevelProblem:
     def residual_blocks(self):
         """The residual patch data as a [patches, 16, 16, 16] device tensor."""
         torch = self.torch
-        return torch.stack([torch.as_tensor(self.r.get(0, p), device="cuda").view(NB, NB, NB)
+        return torch.stack([torch.as_tensor(self.r.get(self.fine, p), device="cuda").view(NB, NB, NB)
                             for p in range(len(self.boxes))]).contiguous()
 
     def close(self):

@@ -260,3 +260,39 @@ def test_level_learned_correction_and_smoothing(torch_gpu):
     s.close()
     P.close()
     net.close()
+
+
+def test_fac_solve_with_learned_smoother(torch_gpu):
+    torch = torch_gpu
+    from samrai_amd import learned as SL
+    _say("FAC solve (2 levels, 32^3 + 64^3 in 16^3 patches) with and without the learned smoother")
+    train = [SL.fac_training_pairs(torch, s, cycles=(1, 2)) for s in range(4)]
+    cat = {k: torch.cat([p[k] for p in train]).contiguous() for k in ("r", "logd", "e")}
+    net = SL.LearnedSmoother(torch, 128, seed=3)
+    net.train(cat["r"], cat["logd"], cat["e"], train[0]["h"], steps=600, batch=32, lr=2e-3)
+    P = SL.BlockLevelProblem(torch, 12, n=64, two_level=True)
+    s = P.solver(max_cycles=60, residual_tol=1e-300, relative_residual_tol=1e-10,
+                 num_pre_sweeps=2, num_post_sweeps=2, coarse_solver_choice="mg",
+                 coarse_solver_tolerance=1e-8, coarse_solver_max_iterations=10,
+                 prolongation_method="CONSERVATIVE_LINEAR_REFINE")
+    out = {}
+    for learned in (False, True):
+        SL.set_learned_smoother(s, net if learned else None)
+        P.u.fill(0.0)
+        ok = s.solve_system(P.u, P.f)
+        s.synchronize()
+        u = torch.stack([torch.as_tensor(P.u.get(1, p)) for p in range(len(P.boxes))])
+        out[learned] = (ok, s.history(), u, SL.learned_active(s))
+        _say("learned=%s: converged %s in %d cycles, factors %s, still active %s"
+             % (learned, ok, len(out[learned][1]),
+                ["%.3f" % (b / a) for a, b in zip(out[learned][1][:-1], out[learned][1][1:])][:8],
+                out[learned][3]))
+    assert out[False][0] and out[True][0]
+    u0, u1 = out[False][2], out[True][2]
+    # both meet the FP64 relative residual 1e-10: the solutions agree to that level
+    assert (u1 - u0).abs().max().item() <= 1e-7 * u0.abs().max().item()
+    assert not out[False][3]
+    SL.set_learned_smoother(s, None)
+    s.close()
+    P.close()
+    net.close()
