@@ -194,7 +194,7 @@ def learned_section(torch, seeds, steps, batch, log=print):
     gen_s = time.perf_counter() - t0
     log("bench: learned: %d training blocks from %d seeds, %d test blocks (%.1f s)" %
         (cat["r"].shape[0], seeds, tst["r"].shape[0], gen_s))
-    nmax = max(batch, tst["r"].shape[0])
+    nmax = max(batch, tst["r"].shape[0], 512)
     net = SL.LearnedSmoother(torch, nmax, seed=0)
 
     def quality():
@@ -265,7 +265,38 @@ def learned_section(torch, seeds, steps, batch, log=print):
             rms_residual=[float("%.4e" % v) for v in hist], ms_per_iteration=round(t_it / 8 * 1e3, 3))
     sv.close()
     P.close()
+    # whole FAC solves to 1e-10 on a two-level problem (64^3 + 128^3 in 16^3
+    # patches, unseen seed): plain red-black smoothing against the learned
+    # correction before it (omega 1 and 0.5), multigrid coarsest-level solver
+    P = SL.BlockLevelProblem(torch, 3000, n=128, two_level=True)
+    sv = P.solver(**dict(MG_PARAMS, max_cycles=100))
+    fac_solves = {}
+    u_ref = None
+    for name, om in (("rbgs", None), ("learned_omega_1", 1.0), ("learned_omega_0.5", 0.5)):
+        SL.set_learned_smoother(sv, None if om is None else net, om or 1.0)
+        P.u.fill(0.0)
+        sv.solve_system(P.u, P.f)  # graph capture
+        P.u.fill(0.0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        ok = sv.solve_system(P.u, P.f)
+        sv.synchronize()
+        dt = time.perf_counter() - t2
+        uf = torch.stack([torch.as_tensor(P.u.get(1, p)) for p in range(len(P.boxes))])
+        if u_ref is None:
+            u_ref = uf
+        fac_solves[name] = dict(s_per_solve=round(dt, 4), cycles=sv.number_of_iterations(),
+                                converged=bool(ok), final_residual=float(sv.history()[-1]),
+                                learned_still_active=SL.learned_active(sv),
+                                max_diff_vs_rbgs=float((uf - u_ref).abs().max()))
+    SL.set_learned_smoother(sv, None)
+    sv.close()
+    P.close()
+    log("bench: learned FAC solves %s" % json.dumps(fac_solves))
     out = dict(
+        fac_solves=dict(workload="two levels, 64^3 + 128^3 in 16^3 patches (576 patches), "
+                                 "D = exp(GRF), 8 Gaussian sources, unseen seed; RBGS 2+2, mg "
+                                 "coarse, rel tol 1e-10", **fac_solves),
         level_smoothing=dict(workload="one-level 64^3 problem of 16^3 patches (unseen seed), "
                                       "8 iterations from u = 0", **level),
         workload="pairs from FAC iterates (1-3 cycles, 2 coarse RBGS sweeps per cycle) on "
