@@ -12,8 +12,9 @@
 // kept in FP32 (Adam state too) and fed to the matrix cores in BF16; every
 // convolution accumulates in FP32.  Channel counts are padded to 16 so that
 // every layer is the same implicit GEMM: per output row of 16 cells and per
-// tap, one v_mfma_f32_16x16x16_bf16 with M = 16 cells, N = 16 output
-// channels, K = 16 input channels.
+// pair of taps, one v_mfma_f32_16x16x32_bf16 with M = 16 cells, N = 16
+// output channels, K = 2 taps x 16 input channels (the weight gradient uses
+// v_mfma_f32_16x16x16_bf16 with K = 16 cells).
 //
 // Activations are channels-last BF16: act[block][z][y][x][16] (32 B per
 // cell).  Convolution weights are stored per tap: wf[t][co][ci] (forward)
@@ -47,6 +48,8 @@ constexpr int NLAYER = 3;
 constexpr int WGRAD_ROWS = 16;         // rows accumulated per wave in the weight gradient
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int NPAIR = (NTAP + 1) / 2;  // tap pairs per K = 32 MFMA
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __host__ __device__ inline uint16_t f2bf(float f) {
@@ -118,9 +121,18 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x,
   const int wave = threadIdx.x >> 6;
   const int cx = lane & 15;
   const int g = lane >> 4;
-  bf16x4 wr[NTAP];
+  // K = 32 per MFMA: two taps of 16 channels.  Lane group g holds channels
+  // 8 (g & 1) .. +7 of tap 2q + (g >> 1); the 28th tap is zero weights.
+  const int half = g & 1;
+  const int tsel = g >> 1;
+  bf16x8 wr[NPAIR];
 #pragma unroll
-  for (int t = 0; t < NTAP; ++t) wr[t] = load4(w + (t * CH + cx) * CH + 4 * g);
+  for (int q = 0; q < NPAIR; ++q) {
+    const int t = 2 * q + tsel;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (t < NTAP) v = *reinterpret_cast<const uint4*>(w + (t * CH + cx) * CH + 8 * half);
+    wr[q] = __builtin_bit_cast(bf16x8, v);
+  }
   const float bv = MODE == MODE_HIDDEN ? bias[cx] : (MODE == MODE_FINAL ? bias[0] : 0.f);
   __syncthreads();
   const uint16_t* lds = reinterpret_cast<const uint16_t*>(tile);
@@ -128,11 +140,13 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ x,
     const int y = wave * CONV_ROWS + r;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < NTAP; ++t) {
+    for (int q = 0; q < NPAIR; ++q) {
+      const int t = min(2 * q + tsel, NTAP - 1);  // the zero-weight 28th tap reads tap 27's cells
       const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const bf16x4 a = load4(lds + (((dz + 1) * HB + (y + dy + 1)) * HB + (cx + dx + 1)) * CH +
-                             4 * g);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wr[t], acc, 0, 0, 0);
+      const uint4 v = *reinterpret_cast<const uint4*>(
+          lds + (((dz + 1) * HB + (y + dy + 1)) * HB + (cx + dx + 1)) * CH + 8 * half);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, v), wr[q], acc, 0,
+                                                    0, 0);
     }
     const size_t cell0 = (size_t)(plane * NB + y) * NB;  // first cell of the row
 #pragma unroll
