@@ -272,8 +272,11 @@ def learned_section(torch, seeds, steps, batch, log=print):
     sv = P.solver(**dict(MG_PARAMS, max_cycles=100))
     fac_solves = {}
     u_ref = None
-    for name, om in (("rbgs", None), ("learned_omega_1", 1.0), ("learned_omega_0.5", 0.5)):
+    for name, om, phases in (("rbgs", None, 3), ("learned_omega_1", 1.0, 3),
+                             ("learned_omega_0.5", 0.5, 3), ("learned_pre_only", 1.0, 1),
+                             ("learned_post_only", 1.0, 2)):
         SL.set_learned_smoother(sv, None if om is None else net, om or 1.0)
+        SL.set_learned_phases(sv, phases)
         P.u.fill(0.0)
         sv.solve_system(P.u, P.f)  # graph capture
         P.u.fill(0.0)

@@ -110,6 +110,9 @@ int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac
  * exceeds 0.5; the FP64 residual remains the convergence test.  The network
  * must hold at least as many blocks as such a level has patches. */
 int sfac_solver_set_learned_smoother(sfac_solver* s, void* net, double omega);
+/* Which smoothing calls of the recursive V-cycle apply it: bit 0 the
+ * pre-smoothing, bit 1 the post-smoothing (default 3, both). */
+int sfac_solver_set_learned_phases(sfac_solver* s, int mask);
 /* 1 if the learned correction is attached and enabled (it is switched off
  * by the fallback), 0 if not, -1 for a null solver. */
 int sfac_solver_learned_active(sfac_solver* s);

@@ -522,6 +522,14 @@ int sfac_solver_set_learned_smoother(sfac_solver* s, void* net, double omega) {
   });
 }
 
+int sfac_solver_set_learned_phases(sfac_solver* s, int mask) {
+  return guarded([&] {
+    need(s, "solver");
+    if (mask < 0 || mask > 3) fail("learned smoother: phase mask must be 0..3");
+    s->s->ops().setLearnedPhases(mask);
+  });
+}
+
 int sfac_solver_learned_active(sfac_solver* s) {
   if (!s) return -1;
   return s->s->ops().learnedActive() ? 1 : 0;

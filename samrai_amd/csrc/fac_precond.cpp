@@ -110,12 +110,15 @@ void FACPreconditioner::facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal
     d_ops.solveCoarsestLevel(e, r, ln);
     return;
   }
+  if (d_gpu) d_gpu->setSmoothingPhase(0);
   d_ops.smoothError(e, r, ln, d_pre);
   d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);
   d_ops.restrictResidual(d_tr, r, ln - 1);
   facCycle_Recursive(e, r, u, lmax, lmin, ln - 1);
   d_ops.prolongErrorAndCorrect(e, e, ln);
+  if (d_gpu) d_gpu->setSmoothingPhase(1);
   d_ops.smoothError(e, r, ln, d_post);
+  if (d_gpu) d_gpu->setSmoothingPhase(0);
 }
 
 // FACPreconditioner.C:530-668.  Smoothing acts on a temporary error

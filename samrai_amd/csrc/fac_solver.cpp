@@ -868,7 +868,7 @@ bool CellPoissonFACOps::learnedLevel(int ln) const {
 
 void CellPoissonFACOps::smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal& r, int ln,
                                     int num_sweeps) {
-  if (learnedActive() && learnedLevel(ln)) {
+  if (learnedActive() && learnedLevel(ln) && ((d_learned_phases >> d_phase) & 1)) {
     // residual of the error equation on this level, then the learned correction
     SAMRAIVectorReal tr = e;
     tr.data = d_learned_r.get();

@@ -210,6 +210,14 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   // null switches it off.  setLearnedEnabled is the per-solve fallback switch.
   void setLearnedSmoother(void* net, double omega);
   bool learnedActive() const { return d_learned != nullptr && d_learned_on; }
+  // Which smoothing calls of the recursive cycle apply it: bit 0 the
+  // pre-smoothing, bit 1 the post-smoothing (default both); the cycle marks
+  // each call with setSmoothingPhase (0 pre, 1 post).
+  void setLearnedPhases(int mask) {
+    d_learned_phases = mask;
+    ++d_generation;
+  }
+  void setSmoothingPhase(int phase) { d_phase = phase; }
   void setLearnedEnabled(bool on) {
     if (on != d_learned_on) {
       d_learned_on = on;
@@ -286,6 +294,8 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   void* d_learned = nullptr;
   double d_learned_omega = 1.0;
   bool d_learned_on = true;
+  int d_learned_phases = 3;
+  int d_phase = 0;
   std::unique_ptr<pdat::PatchDataSet> d_learned_r;
   bool learnedLevel(int ln) const;
   int d_box_variant = -1;

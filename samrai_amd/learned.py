@@ -60,6 +60,7 @@ def lib():
     L.sfac_ops_learned_correct.argtypes = [_vp, _vp, _vp, _vp, _vp, i, d]
     L.sfac_solver_set_learned_smoother.argtypes = [_vp, _vp, d]
     L.sfac_solver_learned_active.argtypes = [_vp]
+    L.sfac_solver_set_learned_phases.argtypes = [_vp, i]
     _bound = True
     return L
 
@@ -234,6 +235,11 @@ def set_learned_smoother(solver, net, omega=1.0):
     """Attach the network to a solver's smoothError (None detaches)."""
     fac._check(lib().sfac_solver_set_learned_smoother(solver.s, None if net is None else net.h,
                                                       float(omega)))
+
+
+def set_learned_phases(solver, mask):
+    """Bit 0: apply in pre-smoothing, bit 1: in post-smoothing (default 3)."""
+    fac._check(lib().sfac_solver_set_learned_phases(solver.s, int(mask)))
 
 
 def learned_active(solver):
