@@ -340,7 +340,7 @@ def main():
                     help="skip the variable-coefficient 4-level deck (C2) section")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-smoother (C4) section")
-    ap.add_argument("--learned-seeds", type=int, default=12)
+    ap.add_argument("--learned-seeds", type=int, default=50)
     ap.add_argument("--learned-steps", type=int, default=1500)
     ap.add_argument("--learned-batch", type=int, default=64)
     ap.add_argument("--mode", choices=["solve", "smoother", "profile", "learned"],
