@@ -287,11 +287,6 @@ int sfac_fused_tile_shape(int dim, int variant, int* shape3);
  * coarse_mg_stats: V-cycles and relative residual of the last coarse solve. */
 int sfac_solver_coarse_mg_stats(sfac_solver* s, int* iterations, double* relres);
 
-/* FACPreconditioner::setAlgorithmChoice (FACPreconditioner.h:316,
- * FACPreconditioner.C:843-866): "default" (facCycle_Recursive),
- * "mccormick-s4.3" (facCycle_McCormick, .C:530-668) or "pernice" (facCycle,
- * .C:678-763), case-insensitive; any other name is an error. */
-int sfac_solver_set_algorithm_choice(sfac_solver* s, const char* choice);
 
 /* Berger-Rigoutsos clustering for building levels (mesh::BergerRigoutsos,
  * BergerRigoutsosNode.C:725-760 recombination, :1251-1325 minimal box,

@@ -47,8 +47,8 @@ namespace samrai_amd {
 // Used by entry points defined in other translation units (learned smoother).
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
-                          double* const* const* dtab, int nblocks, double h, double omega,
-                          hipStream_t s);
+                          double* const* const* dtab, double logd_const, int nblocks, double h,
+                          double omega, hipStream_t s);
 }  // namespace samrai_amd
 
 namespace {
@@ -564,8 +564,8 @@ int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac
   if (D)
     for (int a = 0; a < 3; ++a) dt[a] = D->d->table(ln, a);
   return samrai_amd::learned_correct_level(net, u->d->table(ln), u->d->ghost(), r->d->table(ln),
-                                           D ? dt : nullptr, (int)L.boxes.size(), L.dx[0], omega,
-                                           s->s->ops().stream());
+                                           D ? dt : nullptr, 0.0, (int)L.boxes.size(), L.dx[0],
+                                           omega, s->s->ops().stream());
 }
 
 int sfac_ops_restrict_solution(sfac_solver* s, sfac_patch_data* u, int dest_ln) {

@@ -22,8 +22,31 @@ FACPreconditioner::FACPreconditioner(FACOperatorStrategy& ops, CellPoissonFACOps
 FACPreconditioner::~FACPreconditioner() { deallocateSolverState(); }
 
 void FACPreconditioner::releaseCycleGraph() {
-  if (d_cycle_exec) (void)hipGraphExecDestroy(d_cycle_exec);
-  d_cycle_exec = nullptr;
+  for (int k = 0; k < 2; ++k) {
+    if (d_cycle_exec[k]) (void)hipGraphExecDestroy(d_cycle_exec[k]);
+    d_cycle_exec[k] = nullptr;
+  }
+}
+
+// Copy of u (levels lmin..lmax) taken before a cycle with the learned
+// correction, so that a cycle it spoils can be undone.
+void FACPreconditioner::saveSolution(const SAMRAIVectorReal& u) {
+  const hier::PatchHierarchy& h = u.data->hierarchy();
+  if (!d_u_save || d_u_save->ghost() != u.data->ghost())
+    d_u_save.reset(new PatchDataSet(h, PatchDataSet::kCell, u.data->ghost()));
+  for (int ln = d_lmin; ln <= d_lmax; ++ln)
+    hip_check(hipMemcpyAsync(d_u_save->slab(ln), u.data->slab(ln),
+                             sizeof(double) * u.data->slabSize(ln), hipMemcpyDeviceToDevice,
+                             d_gpu->stream()),
+              "hipMemcpyAsync(save u)");
+}
+
+void FACPreconditioner::restoreSolution(SAMRAIVectorReal& u) {
+  for (int ln = d_lmin; ln <= d_lmax; ++ln)
+    hip_check(hipMemcpyAsync(u.data->slab(ln), d_u_save->slab(ln),
+                             sizeof(double) * u.data->slabSize(ln), hipMemcpyDeviceToDevice,
+                             d_gpu->stream()),
+              "hipMemcpyAsync(restore u)");
 }
 
 void FACPreconditioner::setMaxCycles(int n) {
@@ -90,6 +113,7 @@ void FACPreconditioner::deallocateSolverState() {
   d_r_data.reset();
   d_tr_data.reset();
   d_te_data.reset();
+  d_u_save.reset();
   d_initialized = false;
 }
 
@@ -134,6 +158,7 @@ void FACPreconditioner::facCycle_McCormick(SAMRAIVectorReal& e, SAMRAIVectorReal
   }
   d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);  // 2a
   d_gpu->setToScalarInterior(*d_te.data, 0.0, ln);
+  d_gpu->setSmoothingPhase(0);
   d_ops.smoothError(d_te, d_tr, ln, d_pre);                     // 2b
   d_gpu->add(*e.data, *d_te.data, ln, ln);                      // 2c
   d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);  // 3a
@@ -143,7 +168,9 @@ void FACPreconditioner::facCycle_McCormick(SAMRAIVectorReal& e, SAMRAIVectorReal
   d_ops.prolongErrorAndCorrect(e, e, ln);                       // 3d
   d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);  // 4a
   d_gpu->setToScalarInterior(*d_te.data, 0.0, ln);
+  d_gpu->setSmoothingPhase(1);
   d_ops.smoothError(d_te, d_tr, ln, d_post);                    // 4b
+  d_gpu->setSmoothingPhase(0);
   d_gpu->add(*e.data, *d_te.data, ln, ln);                      // 4c
 }
 
@@ -151,11 +178,13 @@ void FACPreconditioner::facCycle_McCormick(SAMRAIVectorReal& e, SAMRAIVectorReal
 // an ascending loop.
 void FACPreconditioner::facCycle(SAMRAIVectorReal& e, SAMRAIVectorReal& r, SAMRAIVectorReal&,
                                  int lmax, int lmin) {
+  d_gpu->setSmoothingPhase(0);
   for (int ln = lmax; ln > lmin; --ln) {
     d_ops.smoothError(e, r, ln, d_pre);
     d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);
     d_ops.restrictResidual(d_tr, r, ln - 1);
   }
+  d_gpu->setSmoothingPhase(1);
   for (int ln = lmin; ln <= lmax; ++ln) {
     if (ln == lmin) {
       d_ops.solveCoarsestLevel(e, r, ln);
@@ -164,6 +193,7 @@ void FACPreconditioner::facCycle(SAMRAIVectorReal& e, SAMRAIVectorReal& r, SAMRA
       d_ops.smoothError(e, r, ln, d_post);
     }
   }
+  d_gpu->setSmoothingPhase(0);
 }
 
 void FACPreconditioner::runCycle(SAMRAIVectorReal& u) {
@@ -218,15 +248,22 @@ bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
   if ((int)d_factors.size() < d_max_cycles) d_factors.resize(d_max_cycles);
   d_history.clear();
   d_iterations = 0;
+  d_rollbacks = 0;
   // the learned smoother (if attached) starts each solve enabled
   d_gpu->setLearnedEnabled(true);
   const hipStream_t stream = d_gpu->stream();
   do {
     d_factors[d_iterations] = d_residual_norm;
+    const bool learned = d_gpu->learnedActive();
+    if (learned) saveSolution(u);
+    // one captured cycle per learned-smoother state, so that the fallback
+    // does not force a new capture
+    const int gi = learned ? 1 : 0;
     if (d_gpu->graphsUsable()) {
-      if (!d_cycle_exec || d_cycle_gen != d_gpu->generation() || d_cycle_u != u.data ||
-          d_cycle_f != f.data) {
-        releaseCycleGraph();
+      if (!d_cycle_exec[gi] || d_cycle_gen[gi] != d_gpu->generation() ||
+          d_cycle_u[gi] != u.data || d_cycle_f[gi] != f.data) {
+        if (d_cycle_exec[gi]) (void)hipGraphExecDestroy(d_cycle_exec[gi]);
+        d_cycle_exec[gi] = nullptr;
         hipGraph_t g = nullptr;
         hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
                   "hipStreamBeginCapture");
@@ -238,31 +275,37 @@ bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
           throw;
         }
         hip_check(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
-        hipError_t err = hipGraphInstantiate(&d_cycle_exec, g, nullptr, nullptr, 0);
+        hipError_t err = hipGraphInstantiate(&d_cycle_exec[gi], g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
-        if (err != hipSuccess) d_cycle_exec = nullptr;
+        if (err != hipSuccess) d_cycle_exec[gi] = nullptr;
         hip_check(err, "hipGraphInstantiate");
-        d_cycle_gen = d_gpu->generation();
-        d_cycle_u = u.data;
-        d_cycle_f = f.data;
+        d_cycle_gen[gi] = d_gpu->generation();
+        d_cycle_u[gi] = u.data;
+        d_cycle_f[gi] = f.data;
       }
-      hip_check(hipGraphLaunch(d_cycle_exec, stream), "hipGraphLaunch");
+      hip_check(hipGraphLaunch(d_cycle_exec[gi], stream), "hipGraphLaunch");
     } else {
       enqueueCycle(u, f);
     }
-    d_residual_norm = d_gpu->rmsNormFinish();
+    const double prev = d_residual_norm;
+    const double norm = d_gpu->rmsNormFinish();
+    const double factor = prev != 0 ? norm / prev : 0.0;
+    // fallback: a cycle with the learned correction that reduces the residual
+    // by less than a factor of two, or gives a non-finite norm (NaN compares
+    // false), is undone and the solve goes on with red-black alone
+    if (learned && !(factor <= 0.5 && std::isfinite(norm))) {
+      d_gpu->setLearnedEnabled(false);
+      restoreSolution(u);
+      d_residual_norm = computeFullCompositeResidual(d_r, u, f);
+      ++d_rollbacks;
+      continue;
+    }
+    d_residual_norm = norm;
     d_history.push_back(d_residual_norm);
-    if (d_factors[d_iterations] != 0)
-      d_factors[d_iterations] = d_residual_norm / d_factors[d_iterations];
-    else
-      d_factors[d_iterations] = 0;
+    d_factors[d_iterations] = factor;
     ++d_iterations;
     d_net_factor = d_residual_norm / (initial + 1e-20);
     d_avg_factor = std::pow(d_net_factor, 1.0 / d_iterations);
-    // fallback: red-black alone once a cycle with the learned correction
-    // reduces the residual by less than a factor of two
-    if (d_gpu->learnedActive() && d_factors[d_iterations - 1] > 0.5)
-      d_gpu->setLearnedEnabled(false);
     d_ops.postprocessOneCycle(d_iterations - 1, u, d_r);
   } while (d_residual_norm > eff && d_iterations < d_max_cycles);
   hip_check(hipStreamSynchronize(d_gpu->stream()), "hipStreamSynchronize");

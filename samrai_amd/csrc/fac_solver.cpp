@@ -19,8 +19,9 @@
 namespace samrai_amd {
 // learned_kernels.hip: the learned correction of a level of 16^3 patches
 int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
-                          double* const* const* dtab, int nblocks, double h, double omega,
-                          hipStream_t s);
+                          double* const* const* dtab, double logd_const, int nblocks, double h,
+                          double omega, hipStream_t s);
+int learned_attach(void* net, int delta);
 
 namespace pdat {
 
@@ -97,6 +98,7 @@ CellPoissonFACOps::CellPoissonFACOps(hier::PatchHierarchy& h, hipStream_t stream
 }
 
 CellPoissonFACOps::~CellPoissonFACOps() {
+  if (d_learned) (void)learned_attach(d_learned, -1);
   deallocateOperatorState();
   for (hipEvent_t e : d_event_pool) (void)hipEventDestroy(e);
   (void)hipFree(d_stop);
@@ -209,6 +211,7 @@ void CellPoissonFACOps::initializeOperatorState(const SAMRAIVectorReal& u,
   d_partial_n = maxchunks;
   hip_check(hipMalloc(&d_partial, sizeof(double) * d_partial_n), "hipMalloc(partial)");
   setupBoxPath();
+  if (d_learned) checkLearnedSpecification();
 }
 
 void CellPoissonFACOps::deallocateOperatorState() {
@@ -849,15 +852,71 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
 }
 
 void CellPoissonFACOps::setLearnedSmoother(void* net, double omega) {
+  if (net && !(omega > 0.0 && std::isfinite(omega)))
+    fail("learned smoother: omega must be positive and finite");
+  if (net && net != d_learned && learned_attach(net, 1) != 0)
+    fail("learned smoother: cannot attach the network");
+  if (d_learned && d_learned != net) (void)learned_attach(d_learned, -1);
   d_learned = net;
   d_learned_omega = omega;
   d_learned_on = true;
   if (net && !d_learned_r) d_learned_r.reset(new PatchDataSet(d_h, PatchDataSet::kCell, 0));
+  if (net && d_ln_min >= 0) checkLearnedSpecification();
   ++d_generation;
+}
+
+// The network maps (r h^2 / s, log D) to e / s for div(D grad e) = r: it
+// knows nothing of a C term and needs D > 0.  Variable D is checked on the
+// device (every face value positive and finite) when the smoother is attached
+// to an initialized solver and at initialization.
+void CellPoissonFACOps::checkLearnedSpecification() {
+  if (d_spec.cMode() != 0)
+    fail("learned smoother: needs C = 0 (the network models div(D grad e) = r only)");
+  if (d_spec.dIsConstant()) {
+    const double d = d_spec.getDConstant();
+    if (!(d > 0.0 && std::isfinite(d)))
+      fail("learned smoother: needs a positive finite D constant");
+    return;
+  }
+  const PatchDataSet& D = *d_spec.getD();
+  if (D.ghost() != 0) fail("learned smoother: D must have ghost width 0");
+  // every patch array of every level (not the alignment padding between them)
+  std::vector<const double*> ptr;
+  std::vector<long> len;
+  for (int ln = d_ln_min; ln <= d_ln_max; ++ln)
+    for (int p = 0; p < (int)d_h.level(ln).boxes.size(); ++p)
+      for (int a = 0; a < d_h.dim(); ++a) {
+        ptr.push_back(D.pointer(ln, p, a));
+        len.push_back(D.size(ln, p, a));
+      }
+  if (ptr.empty()) return;
+  void* buf = nullptr;
+  const size_t nb = ptr.size() * (sizeof(double*) + sizeof(long));
+  hip_check(hipMalloc(&buf, nb), "hipMalloc(check)");
+  int bad = 0;
+  try {
+    const double** dptr = static_cast<const double**>(buf);
+    long* dlen = reinterpret_cast<long*>(dptr + ptr.size());
+    hip_check(hipMemcpy(dptr, ptr.data(), ptr.size() * sizeof(double*), hipMemcpyHostToDevice),
+              "hipMemcpy");
+    hip_check(hipMemcpy(dlen, len.data(), len.size() * sizeof(long), hipMemcpyHostToDevice),
+              "hipMemcpy");
+    sfac::launch_fill_int(d_stop, 1, 0, d_stream);
+    sfac::launch_flag_nonpositive(dptr, dlen, (int)ptr.size(), d_stop, d_stream);
+    hip_check(hipMemcpyAsync(&bad, d_stop, sizeof(int), hipMemcpyDeviceToHost, d_stream),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
+  } catch (...) {
+    (void)hipFree(buf);
+    throw;
+  }
+  (void)hipFree(buf);
+  if (bad) fail("learned smoother: D has a non-positive or non-finite value (log D undefined)");
 }
 
 bool CellPoissonFACOps::learnedLevel(int ln) const {
   if (d_h.dim() != 3 || ln <= d_ln_min) return false;
+  if (d_spec.cMode() != 0) return false;
   const hier::PatchLevel& L = d_h.level(ln);
   if (L.dx[0] != L.dx[1] || L.dx[0] != L.dx[2]) return false;
   for (const Box& b : L.boxes)
@@ -878,9 +937,12 @@ void CellPoissonFACOps::smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal&
     double* const* dt[3] = {nullptr, nullptr, nullptr};
     if (D)
       for (int a = 0; a < 3; ++a) dt[a] = D->table(ln, a);
-    if (samrai_amd::learned_correct_level(d_learned, e.data->table(ln), e.data->ghost(), d_learned_r->table(ln),
-                              D ? dt : nullptr, (int)d_h.level(ln).boxes.size(),
-                              d_h.level(ln).dx[0], d_learned_omega, d_stream) != 0)
+    // constant D: log D enters as a uniform channel
+    const double logd_const = D ? 0.0 : std::log(d_spec.getDConstant());
+    if (samrai_amd::learned_correct_level(d_learned, e.data->table(ln), e.data->ghost(),
+                                          d_learned_r->table(ln), D ? dt : nullptr, logd_const,
+                                          (int)d_h.level(ln).boxes.size(), d_h.level(ln).dx[0],
+                                          d_learned_omega, d_stream) != 0)
       fail("learned smoother: correction failed");
   }
   smoothErrorByRedBlack(*e.data, *r.data, ln, num_sweeps, -1.0);

@@ -218,12 +218,9 @@ class CellPoissonFACOps : public FACOperatorStrategy {
     ++d_generation;
   }
   void setSmoothingPhase(int phase) { d_phase = phase; }
-  void setLearnedEnabled(bool on) {
-    if (on != d_learned_on) {
-      d_learned_on = on;
-      ++d_generation;
-    }
-  }
+  // The on/off state is not part of generation(): the preconditioner keeps
+  // one captured cycle per state (learnedActive()).
+  void setLearnedEnabled(bool on) { d_learned_on = on; }
   int coarsePrecision() const { return d_coarse_fp32 ? 32 : 64; }
   static constexpr double kCoarseFP32Tol = 1e-6;
 
@@ -298,6 +295,7 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   int d_phase = 0;
   std::unique_ptr<pdat::PatchDataSet> d_learned_r;
   bool learnedLevel(int ln) const;
+  void checkLearnedSpecification();
   int d_box_variant = -1;
   void setupBoxPath();
   void releaseBoxPath();
@@ -350,6 +348,8 @@ class FACPreconditioner {
   double getAvgConvergenceFactor() const { return d_avg_factor; }
   const std::vector<double>& residualHistory() const { return d_history; }
   double lastSolveSeconds() const { return d_seconds; }
+  // cycles of the last solve that the learned-smoother fallback undid
+  int rollbacks() const { return d_rollbacks; }
   // one V-cycle on the error equation with the current residual (tests)
   void runOneCycle(SAMRAIVectorReal& u);
   SAMRAIVectorReal& errorVector() { return d_e; }
@@ -371,10 +371,15 @@ class FACPreconditioner {
   // one cycle of solveSystem up to the residual norm's device reductions
   void enqueueCycle(SAMRAIVectorReal& u, SAMRAIVectorReal& f);
   void releaseCycleGraph();
-  hipGraphExec_t d_cycle_exec = nullptr;
-  unsigned long d_cycle_gen = 0;
-  const void* d_cycle_u = nullptr;
-  const void* d_cycle_f = nullptr;
+  // captured cycles, [0] without and [1] with the learned correction
+  hipGraphExec_t d_cycle_exec[2] = {nullptr, nullptr};
+  unsigned long d_cycle_gen[2] = {0, 0};
+  const void* d_cycle_u[2] = {nullptr, nullptr};
+  const void* d_cycle_f[2] = {nullptr, nullptr};
+  std::unique_ptr<pdat::PatchDataSet> d_u_save;
+  int d_rollbacks = 0;
+  void saveSolution(const SAMRAIVectorReal& u);
+  void restoreSolution(SAMRAIVectorReal& u);
   FACOperatorStrategy& d_ops;
   CellPoissonFACOps* d_gpu;
   int d_max_cycles = 10;

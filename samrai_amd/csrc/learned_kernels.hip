@@ -258,7 +258,8 @@ __global__ __launch_bounds__(256) void k_wgrad_reduce(const float* __restrict__ 
 // Features of one block per work-group (256 threads, 16 cells each):
 // s = RMS over the block of r h^2 (1 if zero); X[cell] = (r h^2 / s, log D, 0...).
 __global__ __launch_bounds__(256) void k_features(const double* __restrict__ r,
-                                                  const double* __restrict__ logd, double h2,
+                                                  const double* __restrict__ logd,
+                                                  double logd_const, double h2,
                                                   uint16_t* __restrict__ x,
                                                   double* __restrict__ scale) {
   __shared__ double red[256];
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ r,
 #pragma unroll
     for (int k = 0; k < CH; ++k) v[k] = 0;
     v[0] = f2bf((float)(rb[c] * h2 / s));
-    v[1] = logd ? f2bf((float)logd[(size_t)b * NCELL + c]) : (uint16_t)0;
+    v[1] = f2bf((float)(logd ? logd[(size_t)b * NCELL + c] : logd_const));
     uint4* dst = reinterpret_cast<uint4*>(xb + (size_t)c * CH);
     dst[0] = make_uint4(v[0] | (v[1] << 16), 0u, 0u, 0u);
     dst[1] = make_uint4(0u, 0u, 0u, 0u);
@@ -551,6 +552,7 @@ struct Net {
   float* loss = nullptr;
   double* blk[3] = {};         // level gathers: residual, log D, correction
   int step = 0;
+  int attached = 0;            // solvers using the network (learned_attach)
   int cin[NLAYER] = {2, CH, CH};
   int cout[NLAYER] = {CH, CH, 1};
 };
@@ -707,13 +709,26 @@ Net* net_of(void* h, int nblocks) {
 }  // namespace
 
 namespace samrai_amd {
+// A solver attaching (delta 1) or detaching (-1) the network: a network
+// with a solver attached cannot be destroyed (the solver's captured cycle
+// graphs hold its buffers).
+int learned_attach(void* net, int delta) {
+  return guarded([&] {
+    need(net != nullptr, "null learned smoother");
+    Net* n = static_cast<Net*>(net);
+    need(n->attached + delta >= 0, "learned smoother: detach without attach");
+    n->attached += delta;
+  });
+}
+
 // The learned correction of one level whose patches are all 16^3 blocks:
 // u += omega * s * net(r h^2 / s, log D), r cell data with ghost 0, D side
-// data with ghost 0 (dtab null: log D = 0), u cell data with ghost `ughost`.
+// data with ghost 0 (dtab null: log D = logd_const everywhere), u cell data
+// with ghost `ughost`.
 // Called by sfac_ops_learned_correct (capi.cpp) after its checks.
 int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
-                          double* const* const* dtab, int nblocks, double h, double omega,
-                          hipStream_t s) {
+                          double* const* const* dtab, double logd_const, int nblocks, double h,
+                          double omega, hipStream_t s) {
   return guarded([&] {
     Net* n = net_of(net, nblocks);
     need(utab && rtab, "null patch tables");
@@ -728,7 +743,7 @@ int learned_correct_level(void* net, double* const* utab, int ughost, double* co
                          n->blk[1]);
       logd = n->blk[1];
     }
-    hipLaunchKernelGGL(k_features, dim3(nblocks), dim3(256), 0, s, n->blk[0], logd, h * h,
+    hipLaunchKernelGGL(k_features, dim3(nblocks), dim3(256), 0, s, n->blk[0], logd, logd_const, h * h,
                        n->act[0], n->scale);
     forward(n, nblocks, s);
     hipLaunchKernelGGL(k_correction, grid, block, 0, s, n->pred, n->scale, omega, ncells,
@@ -739,6 +754,10 @@ int learned_correct_level(void* net, double* const* utab, int ughost, double* co
 }  // namespace samrai_amd
 
 namespace {
+__global__ void k_zero_f32(float* __restrict__ p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0.f;
+}
 }  // namespace
 
 extern "C" {
@@ -764,7 +783,8 @@ int sfac_conv3x3x3_wgrad_bf16(const uint16_t* x, const uint16_t* gz, float* grad
     need(nblocks >= 0, "negative block count");
     need(x && gz && grad && partial, "null pointer");
     if (nblocks == 0) {
-      ck(hipMemsetAsync(grad, 0, GSZ * sizeof(float), stream));
+      // a fill kernel, not a memset node (callers may capture this call)
+      hipLaunchKernelGGL(k_zero_f32, dim3((GSZ + 255) / 256), dim3(256), 0, stream, grad, GSZ);
       return;
     }
     launch_wgrad(x, gz, partial, grad, nblocks, stream);
@@ -840,6 +860,9 @@ int sfac_learned_create(int max_blocks, unsigned long long seed, void** out) {
 int sfac_learned_destroy(void* net) {
   return guarded([&] {
     if (net) {
+      need(static_cast<Net*>(net)->attached == 0,
+           "learned smoother: the network is attached to a solver (detach it with "
+           "sfac_solver_set_learned_smoother(s, NULL, ...) before destroying it)");
       ck(hipDeviceSynchronize());
       free_net(static_cast<Net*>(net));
     }
@@ -853,7 +876,7 @@ int sfac_learned_features(void* net, const double* r, const double* logd, double
     need(r != nullptr, "null residual");
     need(h > 0.0, "h must be positive");
     if (nblocks == 0) return;
-    hipLaunchKernelGGL(k_features, dim3(nblocks), dim3(256), 0, stream, r, logd, h * h,
+    hipLaunchKernelGGL(k_features, dim3(nblocks), dim3(256), 0, stream, r, logd, 0.0, h * h,
                        n->act[0], n->scale);
   });
 }
@@ -906,7 +929,7 @@ int sfac_learned_train_step(void* net, const double* r, const double* logd, cons
     Net* n = net_of(net, nblocks);
     need(r && target, "null residual or target");
     need(h > 0.0 && nblocks > 0, "bad h or block count");
-    hipLaunchKernelGGL(k_features, dim3(nblocks), dim3(256), 0, stream, r, logd, h * h,
+    hipLaunchKernelGGL(k_features, dim3(nblocks), dim3(256), 0, stream, r, logd, 0.0, h * h,
                        n->act[0], n->scale);
     forward(n, nblocks, stream);
     backward(n, target, nblocks, stream);

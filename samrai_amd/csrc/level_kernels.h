@@ -98,6 +98,10 @@ void launch_axpy(const GhostLaunch& cells, const LevelGeom& L, double* const* y,
 // Fill n doubles with v.
 void launch_fill(double* p, long n, double v, hipStream_t s);
 void launch_fill_int(int* p, int n, int v, hipStream_t s);
+// *flag = 1 if an entry of segment i (device pointer ptr[i], length len[i]) is
+// not positive and finite; flag is not cleared here.
+void launch_flag_nonpositive(const double* const* ptr, const long* len, int nseg, int* flag,
+                             hipStream_t s);
 // Set the cells of each item's box in patch item.p to v (dst ghost width g).
 void launch_set_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int g,
                       double v, hipStream_t s);

@@ -915,6 +915,27 @@ void launch_fill(double* p, long n, double v, hipStream_t s) {
   check_launch();
 }
 
+// *flag = 1 if a value of some segment is not positive and finite (NaN
+// included).  One work-group per segment.
+__global__ void __launch_bounds__(kThreads)
+k_flag_nonpositive(const double* const* ptr, const long* len, int* flag) {
+  const double* x = ptr[blockIdx.x];
+  const long n = len[blockIdx.x];
+  int bad = 0;
+  for (long i = threadIdx.x; i < n; i += kThreads) {
+    const double v = x[i];
+    if (!(v > 0.0 && v <= 1.7976931348623157e308)) bad = 1;
+  }
+  if (bad) *flag = 1;
+}
+
+void launch_flag_nonpositive(const double* const* ptr, const long* len, int nseg, int* flag,
+                             hipStream_t s) {
+  if (nseg <= 0) return;
+  hipLaunchKernelGGL(k_flag_nonpositive, dim3(nseg), dim3(kThreads), 0, s, ptr, len, flag);
+  check_launch();
+}
+
 void launch_fill_int(int* p, int n, int v, hipStream_t s) {
   hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, p, n, v);
   check_launch();

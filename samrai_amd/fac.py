@@ -413,6 +413,7 @@ class CellPoissonFACSolver:
         if self.s:
             lib().sfac_solver_destroy(self.s)
             self.s = None
+        self._learned_net = None  # the solver no longer holds the network
 
     def __del__(self):
         try:

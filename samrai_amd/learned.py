@@ -138,8 +138,9 @@ class LearnedSmoother:
         self.h = h
 
     def close(self):
+        """Free the network (an error while a solver still holds it)."""
         if self.h:
-            lib().sfac_learned_destroy(self.h)
+            fac._check(lib().sfac_learned_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -232,9 +233,13 @@ def learned_correct(solver, net, u, r, D, ln, omega=1.0):
 
 
 def set_learned_smoother(solver, net, omega=1.0):
-    """Attach the network to a solver's smoothError (None detaches)."""
+    """Attach the network to a solver's smoothError (None detaches).  The
+    solver keeps a reference to the network while attached: the library
+    refuses to destroy a network a solver holds (its captured cycle graphs
+    use the network's buffers)."""
     fac._check(lib().sfac_solver_set_learned_smoother(solver.s, None if net is None else net.h,
                                                       float(omega)))
+    solver._learned_net = net
 
 
 def set_learned_phases(solver, mask):

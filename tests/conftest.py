@@ -16,8 +16,8 @@ def pytest_configure(config):
     # The oracle is plain C: build it here if it is missing (seconds).
     so = os.path.join(ROOT, "oracle", "liboracle_fac.so")
     if not os.path.exists(so):
-        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "liboracle_fac.so"],
-                       check=False, capture_output=True)
+        subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=False,
+                       capture_output=True)
 
 
 @pytest.fixture(scope="session")
