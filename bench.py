@@ -2,19 +2,23 @@
 """FAC Poisson benchmark on one MI355X.
 
 A step is one CellPoissonFACSolver::solveSystem from u = 0 to a relative
-residual of 1e-10 (RMS residual <= 1e-10 * RMS f, FP64) on the 3-D deck:
-unit cube, 128^3 base in 64^3 patches, three levels at ratio 2 refined where
-|grad u| exceeds the 70th percentile, boxes of 16^3..64^3; D = 1,
-u = sin(pi x) sin(pi y) sin(pi z), Dirichlet 0; red-black GS 2+2 sweeps,
-coarsest level red-black (500 sweeps, tolerance 1e-12), Ewing coarse-fine
-fluxes, conservative linear prolongation.
+residual of 1e-10 (RMS residual <= 1e-10 * RMS f, FP64) on the
+variable-coefficient deck: unit cube, 128^3 base, four levels at ratio 2
+refined around eight Gaussian sources, patch sides drawn from U{8..48}
+(2147 patches, 54 M cells), D = exp(g) with g a Gaussian random field
+(correlation length 0.1, sigma 1), f = the eight Gaussians (width 0.02),
+Dirichlet 0; red-black GS 2+2 sweeps, Ewing coarse-fine fluxes,
+conservative linear prolongation, coarsest level by the library's geometric
+multigrid (the red-black coarse solve, 500 sweeps, does not reach 1e-10 on
+this deck and is reported beside it).
 
 Prints one JSON line: time to solution (value, s), smoother throughput, the
-roofline of the smoother kernel with the largest share of the time, measured
-with HIP events on the solver's stream over one more (untimed) solve, and the
-CPU baseline (the C oracle, a
-bounded sample of FAC cycles on the same deck, projected to the cycle count
-the GPU needed).
+roofline of the kernel with the largest share of the time (HIP events on the
+solver's stream over one more, untimed, solve), the CPU baseline (the C
+oracle on the host cores, all cores and one thread, a bounded sample of FAC
+cycles projected to the GPU's cycle count) and, as extra keys, the 3-level
+constant-coefficient deck, the reference's default coarse tolerance, mixed
+precision and the learned smoother.
 """
 import argparse
 import json
@@ -29,31 +33,42 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# BASELINE's solver: RBGS 2+2, relative residual 1e-10, Ewing, conservative
+# linear prolongation.  Coarsest level: the library's multigrid (hypre's
+# stopping test, relative 1e-8, at most 10 V-cycles) or red-black sweeps
+# (up to 500 per coarse solve, pointwise tolerance 1e-12; the reference's
+# default 1e-8 stalls the outer iteration, see the coarse_tolerance_1e-8 key).
 PARAMS = dict(max_cycles=200, residual_tol=1e-300, relative_residual_tol=1e-10,
               num_pre_sweeps=2, num_post_sweeps=2, coarse_solver_choice="redblack",
               coarse_solver_tolerance=1e-12, coarse_solver_max_iterations=500,
               cf_discretization="Ewing", prolongation_method="CONSERVATIVE_LINEAR_REFINE")
-
-
-def build_problem(fac, decks, deck, params=None):
-    h = fac.PatchHierarchy(deck["dim"], deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
-    for boxes in deck["levels"]:
-        h.add_level(deck["ratio"], boxes)
-    h.finalize()
-    u = fac.PatchData(h, 1)
-    f = fac.PatchData(h, 0)
-    for ln, boxes in enumerate(deck["levels"]):
-        for p, box in enumerate(boxes):
-            f.put(ln, p, decks.sine_rhs(deck, ln, box))
-    s = fac.CellPoissonFACSolver(h, **(params or PARAMS))
-    s.set_d_constant(1.0)
-    s.set_dirichlet(0.0)
-    s.initialize_solver_state(u, f)
-    return h, u, f, s
-
-
 MG_PARAMS = dict(PARAMS, coarse_solver_choice="mg", coarse_solver_tolerance=1e-8,
                  coarse_solver_max_iterations=10)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def host_description():
+    """nproc and the lscpu fields that identify the host CPU."""
+    out = dict(nproc=os.cpu_count())
+    try:
+        out["affinity"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket",
+                     "Socket(s)", "NUMA node(s)", "CPU max MHz"):
+                out[k] = v.strip()
+    except Exception as e:  # lscpu missing: keep nproc
+        out["lscpu"] = "unavailable (%s)" % e
+    return out
 
 
 def max_error(decks, deck, u):
@@ -66,71 +81,55 @@ def max_error(decks, deck, u):
     return err
 
 
-def cpu_baseline(decks, deck, cycles_gpu, ncycles, threads, mg=False):
+def oracle_for(deck, params, f, D, threads):
+    """The C oracle on the same hierarchy, keys and data (f, D copied from
+    the GPU patch data)."""
     from oracle.oracle import OracleFAC
     o = OracleFAC(3, deck["levels"], deck["ratio"], deck["dlo"], deck["dhi"], [0.0] * 3,
                   [1.0] * 3)
-    o.set_poisson(False, 1.0, 0, 0.0)
+    o.set_poisson(D is not None, 1.0, 0, 0.0)
     o.set_bc([1.0] * 6, [0.0] * 6, [0.0] * 6)
-    P = MG_PARAMS if mg else PARAMS
-    o.set_params(max_cycles=ncycles, residual_tol=1e-300, relative_residual_tol=1e-10,
-                 pre=2, post=2, coarse_max_iterations=P["coarse_solver_max_iterations"],
-                 coarse_tol=P["coarse_solver_tolerance"], ewing=True, nthreads=threads)
-    o.set_coarse_mg(mg)
+    o.set_params(max_cycles=params["max_cycles"], residual_tol=1e-300,
+                 relative_residual_tol=1e-10, pre=2, post=2,
+                 coarse_max_iterations=params["coarse_solver_max_iterations"],
+                 coarse_tol=params["coarse_solver_tolerance"], ewing=True, nthreads=threads)
+    o.set_coarse_mg(params["coarse_solver_choice"] == "mg")
+    o.set_prolongation(params["prolongation_method"])
     for ln, boxes in enumerate(deck["levels"]):
-        for p, box in enumerate(boxes):
-            o.put("f", ln, p, decks.sine_rhs(deck, ln, box))
+        for p in range(len(boxes)):
+            o.put("f", ln, p, f.get(ln, p))
+            if D is not None:
+                for a in range(3):
+                    o.put("D", ln, p, D.get(ln, p, axis=a), axis=a)
+    return o
+
+
+def cpu_sample(deck, params, f, D, cycles_gpu, ncycles, threads):
+    """ncycles FAC cycles of the oracle on `threads` host threads, projected
+    to the GPU's cycle count (the initial composite residual is counted once,
+    as in a whole solve)."""
+    o = oracle_for(deck, dict(params, max_cycles=ncycles), f, D, threads)
     conv, it, res, hist = o.solve()
     secs = o.last_solve_seconds()
     o.close()
     per_cycle = secs / max(it, 1)
-    return dict(value=per_cycle * cycles_gpu, unit="s", cores=threads, kind="port",
-                sample="%d FAC cycles of the same deck with the C oracle on %d host threads "
-                       "(%.2f s, %.3f s/cycle), projected to the %d cycles the GPU solve took"
-                       % (it, threads, secs, per_cycle, cycles_gpu))
+    return dict(value=round(per_cycle * cycles_gpu, 3), unit="s", cores=threads, kind="port",
+                sample="%d FAC cycles of the same deck and keys with the C oracle on %d host "
+                       "thread(s) (%.2f s, %.3f s/cycle incl. the initial residual), projected "
+                       "to the %d cycles the GPU solve took" % (it, threads, secs, per_cycle,
+                                                                cycles_gpu))
 
 
-def setup_c2(fac, decks, torch, params, seed=0):
-    """C2: 4 levels, ~2100 patches with sides from U{8..48}, D = exp(GRF)
-    (correlation length 0.1, sigma 1) on the faces, f = eight Gaussian
-    sources of width 0.02; generated on the GPU and copied device to device."""
-    deck = decks.deck_3d_c2(seed)
-    h = fac.PatchHierarchy(3, deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
-    for boxes in deck["levels"]:
-        h.add_level(deck["ratio"], boxes)
-    h.finalize()
-    u = fac.PatchData(h, 1)
-    f = fac.PatchData(h, 0)
-    D = fac.PatchData(h, 0, "side")
-    modes = decks.grf_modes(seed)
-    kw = dict(dtype=torch.float64, device="cuda")
-    for ln, boxes in enumerate(deck["levels"]):
-        for p, box in enumerate(boxes):
-            fv = decks.gaussian_sources(deck, decks.cell_points(deck, ln, box, xp=torch, **kw),
-                                        xp=torch)
-            f.put_device(ln, p, fv.permute(2, 1, 0).contiguous())
-            for a in range(3):
-                g = decks.grf_eval(decks.face_points(deck, ln, box, a, xp=torch, **kw), modes,
-                                   xp=torch)
-                D.put_device(ln, p, torch.exp(g).permute(2, 1, 0).contiguous(), axis=a)
-    s = fac.CellPoissonFACSolver(h, **params)
-    s.set_d_patch_data(D)
-    s.set_dirichlet(0.0)
-    s.initialize_solver_state(u, f)
-    return deck, h, u, f, D, s
-
-
-def measured_traffic(tile):
-    """L2-miss bytes per launch of the fused sweep with this tile shape on the
-    C1 coarsest level, from the committed counter summary (None if absent)."""
-    path = os.path.join(ROOT, "profiles", "k_box_rbgs_c1_hbm_traffic.txt")
-    if tile is None or not os.path.exists(path):
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (lines 'traffic <name> <bytes>'), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_per_launch.txt")
+    if not os.path.exists(path):
         return None
-    key = "x".join(map(str, tile))
     for line in open(path):
         f = line.split()
-        if len(f) == 4 and f[0] == key:
-            return int(f[3])
+        if len(f) == 3 and f[0] == "traffic" and f[1] == name:
+            return int(f[2])
     return None
 
 
@@ -151,22 +150,67 @@ def level_errors(deck, ua, ub):
     return out
 
 
-def time_c2(fac, decks, torch, params, bits=64):
-    deck, h, u, f, D, s = setup_c2(fac, decks, torch, params)
-    s.set_coarse_precision(bits)
+def timed_solves(torch, s, u, f, steps):
+    """Mean seconds of `steps` solves from u = 0 (after one warm-up solve
+    that captures the cycle graph)."""
     u.fill(0.0)
-    s.solve_system(u, f)  # warm-up (graph capture, autotuning)
+    s.solve_system(u, f)
+    times = []
+    ok = False
+    for _ in range(steps):
+        u.fill(0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ok = s.solve_system(u, f)
+        s.synchronize()
+        times.append(time.perf_counter() - t0)
+    return dict(s_per_solve=round(sum(times) / len(times), 4), cycles=s.number_of_iterations(),
+                converged=bool(ok), final_residual=float(s.history()[-1]))
+
+
+def kernel_roofline(s, u, f, deck, fused_box):
+    """One more solve with HIP events around every smoother launch on the
+    solver's stream; per level: launches, time and algorithmic bytes; the
+    level whose smoother takes the most time gives the roofline."""
+    s.enable_kernel_timing(True)
     u.fill(0.0)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ok = s.solve_system(u, f)
+    s.solve_system(u, f)
+    per_level = [s.kernel_timing(ln) for ln in range(len(deck["levels"]))]
+    s.enable_kernel_timing(False)
+    levels = []
+    for ln, (n, ms, b) in enumerate(per_level):
+        if n == 0:
+            continue
+        kern = ("k_box_rbgs (colour-fused sweep)" if ln == 0 and fused_box
+                else s.smoother_kernel_name(ln))
+        levels.append(dict(level=ln, kernel=kern, launches=n, total_ms=round(ms, 3),
+                           avg_launch_us=round(ms * 1e3 / max(n, 1), 3),
+                           bytes_per_launch=round(b / max(n, 1)),
+                           achieved_GBs=round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else 0.0))
+    dom = max(levels, key=lambda d: d["total_ms"])
+    achieved = dom["achieved_GBs"]
+    traffic = measured_traffic(dom["kernel"].split()[0] + "_L%d" % dom["level"])
+    return dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                traffic_source=("FETCH_SIZE + WRITE_SIZE per launch, scaled by a copy of known "
+                                "size (profiles/hbm_traffic_per_launch.txt)" if traffic else None),
+                kernel="%s, level %d" % (dom["kernel"], dom["level"]),
+                launches=dom["launches"], avg_launch_us=dom["avg_launch_us"],
+                bytes_per_launch=dom["bytes_per_launch"], per_level=levels)
+
+
+def smoother_rate(s, deck, ln, sweeps):
+    """Interior cells x full red+black sweeps per second (Gcell/s) of
+    smoothErrorByRedBlack on level ln (the solver's error and residual)."""
+    e = s.internal_vector("e")
+    r = s.internal_vector("r")
+    n = sum(int(np.prod([hh - ll + 1 for ll, hh in zip(*b)])) for b in deck["levels"][ln])
+    s.smooth(e, r, ln, 2)
     s.synchronize()
-    dt = time.perf_counter() - t0
-    out = dict(s_per_solve=round(dt, 4), cycles=s.number_of_iterations(), converged=bool(ok),
-               final_residual=float(s.history()[-1]))
-    if params.get("coarse_solver_choice") == "mg":
-        out["last_coarse_vcycles"] = s.coarse_mg_stats()[0]
-    return deck, out, (h, u, f, D, s)
+    t1 = time.perf_counter()
+    s.smooth(e, r, ln, sweeps)
+    s.synchronize()
+    return round(n * sweeps / (time.perf_counter() - t1) / 1e9, 2)
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
@@ -331,13 +375,15 @@ def learned_section(torch, seeds, steps, batch, log=print):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-cycles", type=int, default=2)
-    ap.add_argument("--smoother-sweeps", type=int, default=200)
-    ap.add_argument("--no-c2", action="store_true",
-                    help="skip the variable-coefficient 4-level deck (C2) section")
+    ap.add_argument("--cpu-cycles", type=int, default=2,
+                    help="oracle cycles of the all-cores CPU sample (the one-thread sample "
+                         "runs one cycle)")
+    ap.add_argument("--no-cpu-1thread", action="store_true")
+    ap.add_argument("--smoother-sweeps", type=int, default=40)
+    ap.add_argument("--no-c1", action="store_true", help="skip the 3-level D = 1 deck (C1)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-smoother (C4) section")
     ap.add_argument("--learned-seeds", type=int, default=50)
@@ -345,13 +391,13 @@ def main():
     ap.add_argument("--learned-batch", type=int, default=64)
     ap.add_argument("--mode", choices=["solve", "smoother", "profile", "learned"],
                     default="solve",
-                    help="smoother: only a few smoother launches and a 256 MiB copy of known "
-                         "size (for hardware-counter passes); profile: only the solves of "
+                    help="smoother: a 256 MiB copy of known size and a few smoother calls on "
+                         "--deck (for hardware-counter passes); profile: only the solves of "
                          "--deck with --coarse (for kernel traces); neither prints the "
                          "benchmark line")
-    ap.add_argument("--deck", choices=["c1", "c2"], default="c1",
-                    help="deck of --mode profile")
-    ap.add_argument("--coarse", choices=["redblack", "mg"], default="redblack",
+    ap.add_argument("--deck", choices=["c1", "c2"], default="c2",
+                    help="deck of --mode profile / smoother")
+    ap.add_argument("--coarse", choices=["redblack", "mg"], default="mg",
                     help="coarsest-level solver of --mode profile")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -365,36 +411,32 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
 
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
-
     if args.mode == "learned":
         log("bench: learned %s" % json.dumps(learned_section(
             torch, args.learned_seeds, args.learned_steps, args.learned_batch, log)))
         return
-    if args.mode == "profile":
-        P = MG_PARAMS if args.coarse == "mg" else PARAMS
-        if args.deck == "c2":
-            deck, h, u, f, D, s = setup_c2(fac, decks, torch, P)
+
+    def problem(deck_name, params):
+        if deck_name == "c2":
+            deck, h, u, f, D, s = decks.build_c2_problem(fac, torch, params)
         else:
             deck = decks.deck_3d_c1()
-            h, u, f, s = build_problem(fac, decks, deck, P)
+            h, u, f, s = decks.build_sine_problem(fac, deck, params)
+            D = None
+        return deck, h, u, f, D, s
+
+    if args.mode == "profile":
+        deck, h, u, f, D, s = problem(args.deck, MG_PARAMS if args.coarse == "mg" else PARAMS)
         for k in range(args.warmup + args.steps):
             u.fill(0.0)
             t0 = time.perf_counter()
             s.solve_system(u, f)
             s.synchronize()
-            print("bench: profile %s %s solve %d: %d cycles, %.4f s" %
-                  (args.deck, args.coarse, k, s.number_of_iterations(),
-                   time.perf_counter() - t0), file=sys.stderr, flush=True)
+            log("bench: profile %s %s solve %d: %d cycles, %.4f s" %
+                (args.deck, args.coarse, k, s.number_of_iterations(), time.perf_counter() - t0))
         return
-    deck = decks.deck_3d_c1()
-    ncells = decks.count_cells(deck)
-    npatch = sum(len(l) for l in deck["levels"])
-    print("bench: deck %s, %d patches, %d cells; building" % (deck["name"], npatch, ncells),
-          file=sys.stderr, flush=True)
-    h, u, f, s = build_problem(fac, decks, deck)
     if args.mode == "smoother":
+        deck, h, u, f, D, s = problem(args.deck, MG_PARAMS)
         # a plain copy of known size calibrates the byte counters
         x = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
@@ -403,20 +445,25 @@ def main():
         e = s.internal_vector("e")
         r = s.internal_vector("r")
         r.fill(1.0)
-        s.smooth(e, r, 0, 20)
-        s.smooth(e, r, len(deck["levels"]) - 1, 5)
+        nl = len(deck["levels"])
+        s.smooth(e, r, 0, 10)
+        for ln in range(1, nl):
+            s.smooth(e, r, ln, 2)
         s.synchronize()
-        print("bench: smoother mode done (copy 256 MiB, 20 coarsest sweeps, 5 finest sweeps)",
-              file=sys.stderr, flush=True)
+        log("bench: smoother mode done (copy 256 MiB, 10 coarsest sweeps, 2 sweeps per finer "
+            "level)")
         return
 
+    # ---------------- the headline: C2, plain FP64, multigrid coarse solve
+    deck, h, u, f, D, s = problem("c2", MG_PARAMS)
+    ncells = decks.count_cells(deck)
+    npatch = sum(len(l) for l in deck["levels"])
+    log("bench: deck %s, %d patches, %d cells" % (deck["name"], npatch, ncells))
     for w in range(args.warmup):
         u.fill(0.0)
         s.solve_system(u, f)
-        print("bench: warmup %d: %d cycles, %.3f s" % (w, s.number_of_iterations(),
-                                                        s.last_solve_seconds()),
-              file=sys.stderr, flush=True)
-
+        log("bench: warmup %d: %d cycles, %.3f s" % (w, s.number_of_iterations(),
+                                                     s.last_solve_seconds()))
     converged = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -426,126 +473,63 @@ def main():
     s.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
     cycles = s.number_of_iterations()
     hist = s.history()
-    fused = s.fused_coarsest_active()
-    tile = fac.fused_tile_shapes(3)[s.fused_tile_variant()] if fused else None
-    tuning = {"x".join(map(str, sh)): round(ms * 1e3, 2)
-              for sh, ms in zip(fac.fused_tile_shapes(3), s.fused_tile_timings())}
-    # one more solve with HIP events around every smoother launch (kept out
-    # of the timed steps: the events add host work between launches)
-    s.enable_kernel_timing(True)
-    u.fill(0.0)
-    s.solve_system(u, f)
-    per_level = [s.kernel_timing(ln) for ln in range(len(deck["levels"]))]
-    s.enable_kernel_timing(False)
-    ms_per_step = elapsed * 1e3 / args.steps
-    print("bench: %d steps, %.3f s/step, %d cycles, final residual %.3e" %
-          (args.steps, ms_per_step / 1e3, cycles, hist[-1]), file=sys.stderr, flush=True)
-    err = max_error(decks, deck, u)
+    log("bench: %d steps, %.4f s/step, %d cycles, final residual %.3e" %
+        (args.steps, ms_per_step / 1e3, cycles, hist[-1]))
+    roofline = kernel_roofline(s, u, f, deck, s.fused_coarsest_active())
+    log("bench: roofline %s" % json.dumps(roofline))
+    nl = len(deck["levels"])
+    gcells = {"level_%d" % ln: smoother_rate(s, deck, ln, args.smoother_sweeps)
+              for ln in range(nl)}
+    log("bench: smoother Gcell/s %s" % json.dumps(gcells))
+    u_ref = u  # the FP64 solution (mixed-precision errors are measured against it)
 
-    # smoother throughput (interior cells x full red+black sweeps / s) on
-    # the coarsest level (the dominant operation) and on the finest level
-    e = s.internal_vector("e")
-    r = s.internal_vector("r")
+    # red-black coarse solve beside it (does not reach 1e-10 on this deck)
+    _, _, ur, fr, Dr, sr = problem("c2", dict(PARAMS, max_cycles=60))
+    rb = timed_solves(torch, sr, ur, fr, 1)
+    rb["note"] = "max_cycles 60"
+    log("bench: C2 red-black coarse solve %s" % json.dumps(rb))
+    sr.close()
+    del ur, fr, Dr
 
-    def smoother_rate(ln):
-        n = sum(int(np.prod([hh - ll + 1 for ll, hh in zip(*b)])) for b in deck["levels"][ln])
-        s.synchronize()
-        t1 = time.perf_counter()
-        s.smooth(e, r, ln, args.smoother_sweeps)
-        s.synchronize()
-        return n * args.smoother_sweeps / (time.perf_counter() - t1) / 1e9
+    # mixed precision (coarsest-level solve in FP32 inside the FP64 iteration)
+    _, _, um, fm, Dm, sm = problem("c2", MG_PARAMS)
+    sm.set_coarse_precision(32)
+    mixed = timed_solves(torch, sm, um, fm, max(1, min(args.steps, 5)))
+    mixed["error_vs_fp64_per_level"] = level_errors(deck, um, u_ref)
+    mixed["fp64_s_per_solve"] = round(ms_per_step / 1e3, 4)
+    mixed["scope"] = ("coarsest-level solve (multigrid) in FP32 storage and arithmetic; FAC "
+                      "iteration, residuals, finer levels and u in FP64")
+    log("bench: C2 mixed precision %s" % json.dumps(mixed))
+    sm.close()
+    del um, fm, Dm
 
-    smoother_gcells = smoother_rate(0)
-    finest_gcells = smoother_rate(len(deck["levels"]) - 1)
-
-    # roofline of the smoother kernel that takes the most time: algorithmic
-    # bytes of each launch (24 B per cell per full sweep at constant D: u
-    # read, f read, u write) over its HIP-event duration
-    levels = []
-    for ln, (n, ms, b) in enumerate(per_level):
-        levels.append(dict(level=ln, kernel=("k_box_rbgs (colour-fused sweep)" if ln == 0 and fused
-                                             else "k_rbgs (one colour pass)"),
-                           launches=n, total_ms=round(ms, 3),
-                           avg_launch_us=round(ms * 1e3 / max(n, 1), 3),
-                           bytes_per_launch=round(b / max(n, 1)),
-                           achieved_GBs=round(b / (ms * 1e-3) / 1e9, 1) if ms > 0 else 0.0))
-    dom = max(levels, key=lambda d: d["total_ms"])
-    achieved = dom["achieved_GBs"]
-    traffic = measured_traffic(tile) if dom["level"] == 0 and fused else None
-    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                    traffic_source=("FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per launch "
-                                    "(profiles/k_box_rbgs_c1_hbm_traffic.txt)"
-                                    if traffic else None),
-                    kernel="%s, level %d" % (dom["kernel"], dom["level"]),
-                    launches=dom["launches"], avg_launch_us=dom["avg_launch_us"],
-                    bytes_per_launch=dom["bytes_per_launch"], per_level=levels)
-
-    # the same deck with the multigrid coarsest-level solver (the reference's
-    # default coarse solver is hypre; "mg" is this library's stand-in for it)
-    h2, u2, f2, s2 = build_problem(fac, decks, deck, MG_PARAMS)
-    u2.fill(0.0)
-    s2.solve_system(u2, f2)
-    mg_times = []
-    for k in range(args.steps):
-        u2.fill(0.0)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        ok2 = s2.solve_system(u2, f2)
-        s2.synchronize()
-        mg_times.append(time.perf_counter() - t2)
-    mg = dict(s_per_solve=round(sum(mg_times) / len(mg_times), 4), cycles=s2.number_of_iterations(),
-              converged=bool(ok2), final_residual=float(s2.history()[-1]),
-              max_error_vs_exact=max_error(decks, deck, u2),
-              coarse="mg V(2,2), rel tol 1e-8, <= 10 V-cycles",
-              last_coarse_vcycles=s2.coarse_mg_stats()[0])
-    print("bench: mg coarse solver: %.4f s/solve, %d cycles" % (mg["s_per_solve"], mg["cycles"]),
-          file=sys.stderr, flush=True)
-
-    # mixed precision: the coarsest-level solve of the error equation in FP32
-    # (storage and arithmetic) inside the FP64 FAC iteration; errors per
-    # level against the FP64 solution of the same solver (which equals the
-    # oracle's to 1e-12 in the parity tests)
-    def mixed(params, ref_u):
-        hm, um, fm, sm = build_problem(fac, decks, deck, params)
-        sm.set_coarse_precision(32)
-        um.fill(0.0)
-        sm.solve_system(um, fm)
-        times = []
-        for k in range(args.steps):
-            um.fill(0.0)
-            torch.cuda.synchronize()
-            t3 = time.perf_counter()
-            okm = sm.solve_system(um, fm)
-            sm.synchronize()
-            times.append(time.perf_counter() - t3)
-        out = dict(s_per_solve=round(sum(times) / len(times), 4), cycles=sm.number_of_iterations(),
-                   converged=bool(okm), final_residual=float(sm.history()[-1]),
-                   error_vs_fp64_per_level=level_errors(deck, um, ref_u))
-        return out
-    mixed_c1 = dict(coarse="coarsest-level solve in FP32 storage and arithmetic; FAC "
-                           "iteration, residuals and finer levels FP64",
-                    redblack_coarse=mixed(PARAMS, u), mg_coarse=mixed(MG_PARAMS, u2))
-    mixed_c1["redblack_coarse"]["fp64_s_per_solve"] = round(ms_per_step / 1e3, 4)
-    mixed_c1["mg_coarse"]["fp64_s_per_solve"] = mg["s_per_solve"]
-    print("bench: mixed precision C1 %s" % json.dumps(mixed_c1), file=sys.stderr, flush=True)
-
-    c2 = None
-    if not args.no_c2:
-        print("bench: C2 deck (variable D), red-black and multigrid coarse solvers",
-              file=sys.stderr, flush=True)
-        deck2, rb2, _ = time_c2(fac, decks, torch, PARAMS)
-        _, mg2, ref2 = time_c2(fac, decks, torch, MG_PARAMS)
-        _, mg2m, mix2 = time_c2(fac, decks, torch, MG_PARAMS, bits=32)
-        mg2m["error_vs_fp64_per_level"] = level_errors(deck2, mix2[1], ref2[1])
-        mg2m["fp64_s_per_solve"] = mg2["s_per_solve"]
-        c2 = dict(workload="c2-3d: 4 levels ratio 2, base 128^3, patch sides U{8..48} (seed 0), "
-                           "D = exp(GRF, corr 0.1, sigma 1), f = 8 Gaussian sources (width 0.02)",
-                  patches=sum(len(l) for l in deck2["levels"]), cells=decks.count_cells(deck2),
-                  redblack_coarse=rb2, mg_coarse=mg2, mg_coarse_mixed_fp32=mg2m)
-        print("bench: C2 %s" % json.dumps(c2), file=sys.stderr, flush=True)
+    c1 = None
+    if not args.no_c1:
+        deck1, _, u1, f1, _, s1 = problem("c1", PARAMS)
+        c1 = dict(workload="c1-3d: 128^3 base, 3 levels ratio 2, |grad u| > 70th pct, Berger-"
+                           "Rigoutsos boxes 16^3..64^3, D = 1, sine RHS",
+                  patches=sum(len(l) for l in deck1["levels"]),
+                  cells=decks.count_cells(deck1))
+        c1["redblack_coarse"] = timed_solves(torch, s1, u1, f1, max(1, min(args.steps, 5)))
+        c1["redblack_coarse"]["max_error_vs_exact"] = max_error(decks, deck1, u1)
+        c1["redblack_coarse"]["roofline"] = kernel_roofline(s1, u1, f1, deck1,
+                                                            s1.fused_coarsest_active())
+        s1.close()
+        _, _, u1m, f1m, _, s1m = problem("c1", MG_PARAMS)
+        c1["mg_coarse"] = timed_solves(torch, s1m, u1m, f1m, max(1, min(args.steps, 5)))
+        s1m.close()
+        # the reference's default coarse tolerance (CellPoissonFACOps.C:594):
+        # 1e-8 absolute max residual per coarse solve
+        _, _, u1d, f1d, _, s1d = problem("c1", dict(PARAMS, coarse_solver_tolerance=1e-8,
+                                                    max_cycles=100))
+        c1["coarse_tolerance_1e-8"] = timed_solves(torch, s1d, u1d, f1d, 1)
+        c1["coarse_tolerance_1e-8"]["history_every_10"] = [float("%.3e" % v)
+                                                           for v in s1d.history()[::10]]
+        s1d.close()
+        log("bench: C1 %s" % json.dumps(c1))
 
     learned = None
     if not args.no_learned:
@@ -559,29 +543,30 @@ def main():
             threads = min(16, len(os.sched_getaffinity(0)))
         except AttributeError:
             threads = min(16, os.cpu_count() or 1)
-        print("bench: CPU baseline (%d cycles, %d threads)" % (args.cpu_cycles, threads),
-              file=sys.stderr, flush=True)
-        cpu = cpu_baseline(decks, deck, cycles, args.cpu_cycles, threads)
-        mg["cpu_baseline"] = cpu_baseline(decks, deck, mg["cycles"], args.cpu_cycles, threads,
-                                          mg=True)
+        log("bench: CPU baseline (%d cycles, %d threads)" % (args.cpu_cycles, threads))
+        cpu = cpu_sample(deck, MG_PARAMS, f, D, cycles, args.cpu_cycles, threads)
+        if not args.no_cpu_1thread:
+            log("bench: CPU baseline (1 cycle, 1 thread)")
+            cpu["one_thread"] = cpu_sample(deck, MG_PARAMS, f, D, cycles, 1, 1)
+        cpu["host"] = host_description()
+        log("bench: CPU %s" % json.dumps(cpu))
 
     out = dict(metric="time to 1e-10 rel residual (s)", value=round(ms_per_step / 1e3, 4),
                unit="s", n_gpus=1, steps=args.steps, warmup=args.warmup,
                ms_per_step=round(ms_per_step, 3), higher_is_better=False, scaling="strong",
                vs_baseline=None, dtype="f64", data="synthetic",
-               config=dict(workload="%s: 3-D unit cube, 128^3 base, 3 levels ratio 2, "
-                                    "|grad u| > 70th pct, boxes 16^3..64^3, D=1, sine RHS"
-                                    % deck["name"],
-                           patches=npatch, cells=ncells, levels=len(deck["levels"]),
-                           solver="FAC V-cycle, RBGS 2+2, coarse redblack 500 sweeps tol "
-                                  "1e-12, Ewing, conservative linear refine, rel tol 1e-10"),
+               config=dict(workload="%s: 3-D unit cube, 128^3 base, 4 levels ratio 2 around 8 "
+                                    "Gaussian sources, patch sides U{8..48}, D = exp(GRF corr "
+                                    "0.1 sigma 1), f = 8 Gaussians width 0.02" % deck["name"],
+                           patches=npatch, cells=ncells, levels=nl,
+                           patches_note="2147 patches for seed 0 (BASELINE: about 2500); "
+                                        "the count follows from the seeded partition",
+                           solver="FAC V-cycle, RBGS 2+2, Ewing, conservative linear refine, "
+                                  "coarsest level multigrid V(2,2) rel tol 1e-8 <= 10 V-cycles, "
+                                  "rel tol 1e-10"),
                cycles=cycles, converged=all(converged), final_residual=float(hist[-1]),
-               max_error_vs_exact=err,
-               smoother_gcells_per_s=round(smoother_gcells, 2),
-               finest_smoother_gcells_per_s=round(finest_gcells, 2),
-               fused_coarsest=fused, fused_tile=tile, fused_tile_autotune_us_per_sweep=tuning,
-               roofline=roofline, cpu_baseline=cpu, with_mg_coarse_solver=mg,
-               mixed_precision=mixed_c1, c2=c2, learned_smoother=learned)
+               smoother_gcells_per_s=gcells, roofline=roofline, cpu_baseline=cpu,
+               redblack_coarse=rb, mixed_precision=mixed, c1=c1, learned_smoother=learned)
     print(json.dumps(out), flush=True)
 
 

@@ -304,3 +304,51 @@ def gaussian_sources(deck, points, width=0.02, xp=np):
               (points[..., 2] - float(c[2])) ** 2)
         f = f + float(a) * xp.exp(-r2 / (2 * width ** 2))
     return f
+
+
+def build_sine_problem(fac, deck, params, dconst=1.0):
+    """Hierarchy, u (ghost 1, zero), f = the sine RHS (ghost 0) and a solver
+    with D constant, Dirichlet 0 and the given input keys, initialized."""
+    h = fac.PatchHierarchy(deck["dim"], deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
+    for boxes in deck["levels"]:
+        h.add_level(deck["ratio"], boxes)
+    h.finalize()
+    u = fac.PatchData(h, 1)
+    f = fac.PatchData(h, 0)
+    for ln, boxes in enumerate(deck["levels"]):
+        for p, box in enumerate(boxes):
+            f.put(ln, p, sine_rhs(deck, ln, box))
+    s = fac.CellPoissonFACSolver(h, **params)
+    s.set_d_constant(dconst)
+    s.set_dirichlet(0.0)
+    s.initialize_solver_state(u, f)
+    return h, u, f, s
+
+
+def build_c2_problem(fac, torch, params, seed=0, deck=None):
+    """The variable-coefficient deck on the GPU: D = exp(GRF) on the faces
+    and f = eight Gaussian sources, evaluated with torch on the device and
+    copied device to device into the patch data; u = 0; Dirichlet 0.
+    Returns (deck, hierarchy, u, f, D, solver)."""
+    deck = deck or deck_3d_c2(seed)
+    h = fac.PatchHierarchy(3, deck["dlo"], deck["dhi"], [0.0] * 3, [1.0] * 3)
+    for boxes in deck["levels"]:
+        h.add_level(deck["ratio"], boxes)
+    h.finalize()
+    u = fac.PatchData(h, 1)
+    f = fac.PatchData(h, 0)
+    D = fac.PatchData(h, 0, "side")
+    modes = grf_modes(deck["seed"])
+    kw = dict(dtype=torch.float64, device="cuda")
+    for ln, boxes in enumerate(deck["levels"]):
+        for p, box in enumerate(boxes):
+            fv = gaussian_sources(deck, cell_points(deck, ln, box, xp=torch, **kw), xp=torch)
+            f.put_device(ln, p, fv.permute(2, 1, 0).contiguous())
+            for a in range(3):
+                g = grf_eval(face_points(deck, ln, box, a, xp=torch, **kw), modes, xp=torch)
+                D.put_device(ln, p, torch.exp(g).permute(2, 1, 0).contiguous(), axis=a)
+    s = fac.CellPoissonFACSolver(h, **params)
+    s.set_d_patch_data(D)
+    s.set_dirichlet(0.0)
+    s.initialize_solver_state(u, f)
+    return deck, h, u, f, D, s

@@ -376,6 +376,11 @@ class CellPoissonFACSolver:
         return n.value, ms.value, b.value
 
     # FACOperatorStrategy operations (after initialize_solver_state)
+    def smoother_kernel_name(self, ln):
+        """Name of the kernel that smoothErrorByRedBlack launches on level ln
+        (for reports)."""
+        return "k_rbgs (one colour pass)"
+
     def internal_vector(self, which):
         idx = {"e": 0, "r": 1, "tr": 2, "w": 3}[which]
         d = _vp()
