@@ -51,6 +51,21 @@ int sfac_conv3x3x3_wgrad_bf16(const uint16_t* x, const uint16_t* gz, float* grad
                               int nblocks, hipStream_t stream);
 size_t sfac_conv3x3x3_wgrad_partial_size(int nblocks);
 
+/* No counterpart in the reference: the weight gradient above for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * or training loops, harnesses that forbid hipMalloc).  It allocates nothing:
+ * the per-wave slabs live in a static device array of the library, one per
+ * GPU, 7 MiB, held until the process ends; its address is looked up for the
+ * current device on the first call there (the stream must belong to the
+ * current device).  nblocks <= SFAC_WGRAD_NOALLOC_MAX_BLOCKS, larger counts
+ * are rejected with an error.  Results are bit-identical to
+ * sfac_conv3x3x3_wgrad_bf16.  Every call on one GPU shares that array, so
+ * calls that overlap in time on one GPU (several streams or host threads)
+ * race; calls ordered on one stream do not. */
+#define SFAC_WGRAD_NOALLOC_MAX_BLOCKS 16
+int sfac_conv3x3x3_wgrad_bf16_noalloc(const uint16_t* x, const uint16_t* gz, float* grad,
+                                      int nblocks, hipStream_t stream);
+
 /* Block problems div(D grad e) = r, homogeneous Dirichlet on the block faces
  * (ghost = -interior), FP64.  sfac_block_sor: red-black SOR sweeps (omega 1
  * is red-black Gauss-Seidel), starting from e (from_input != 0) or zero.

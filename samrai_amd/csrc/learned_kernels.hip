@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -595,6 +596,36 @@ T* dalloc(size_t n) {
 
 int wgrad_waves(int nrows) { return (nrows + WGRAD_ROWS - 1) / WGRAD_ROWS; }
 
+// Per-wave gradient slabs of the allocation-free weight gradient
+// (sfac_conv3x3x3_wgrad_bf16_noalloc): one array per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (on the
+// host it names a stand-in, not device memory): it passes the address that
+// hipGetSymbolAddress returns for the current device.
+constexpr int NOALLOC_MAX_BLOCKS = SFAC_WGRAD_NOALLOC_MAX_BLOCKS;
+constexpr size_t NOALLOC_FLOATS = (size_t)((NOALLOC_MAX_BLOCKS * NROW + WGRAD_ROWS - 1) /
+                                           WGRAD_ROWS) * GSZ;  // 7 MiB
+__device__ __align__(256) float scratch[NOALLOC_FLOATS];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up on the first
+// call for that device.
+float* scratch_on_current_device() {
+  int dev = 0;
+  ck(hipGetDevice(&dev));
+  need(dev >= 0 && dev < kMaxDevices, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    ck(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    need(ptr != nullptr, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[dev] = static_cast<float*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
 void launch_conv(int mode, const uint16_t* x, const uint16_t* w, const float* bias,
                  const uint16_t* mask, uint16_t* out, float* pred, int nblocks, hipStream_t s) {
   static_assert(4 * CONV_ROWS == NB, "a work-group's four waves cover one z-plane");
@@ -788,6 +819,23 @@ int sfac_conv3x3x3_wgrad_bf16(const uint16_t* x, const uint16_t* gz, float* grad
       return;
     }
     launch_wgrad(x, gz, partial, grad, nblocks, stream);
+  });
+}
+
+int sfac_conv3x3x3_wgrad_bf16_noalloc(const uint16_t* x, const uint16_t* gz, float* grad,
+                                      int nblocks, hipStream_t stream) {
+  return guarded([&] {
+    need(nblocks >= 0, "negative block count");
+    need(nblocks <= NOALLOC_MAX_BLOCKS,
+         "sfac_conv3x3x3_wgrad_bf16_noalloc: nblocks exceeds the static scratch maximum "
+         "(SFAC_WGRAD_NOALLOC_MAX_BLOCKS = 16)");
+    need(x && gz && grad, "null pointer");
+    if (nblocks == 0) {
+      hipLaunchKernelGGL(k_zero_f32, dim3((GSZ + 255) / 256), dim3(256), 0, stream, grad, GSZ);
+      return;
+    }
+    // every wave of k_wgrad writes its whole slab, so nothing needs clearing
+    launch_wgrad(x, gz, scratch_on_current_device(), grad, nblocks, stream);
   });
 }
 
