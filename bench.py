@@ -399,6 +399,9 @@ def main():
                     help="deck of --mode profile / smoother")
     ap.add_argument("--coarse", choices=["redblack", "mg"], default="mg",
                     help="coarsest-level solver of --mode profile")
+    ap.add_argument("--no-fused-levels", action="store_true",
+                    help="--mode profile / smoother: patch-wise colour passes on the levels "
+                         "above the coarsest instead of the colour-fused sweeps")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: this build has no multi-GPU path "
@@ -427,6 +430,8 @@ def main():
 
     if args.mode == "profile":
         deck, h, u, f, D, s = problem(args.deck, MG_PARAMS if args.coarse == "mg" else PARAMS)
+        if args.no_fused_levels:
+            s.set_fused_levels(False)
         for k in range(args.warmup + args.steps):
             u.fill(0.0)
             t0 = time.perf_counter()
@@ -437,6 +442,8 @@ def main():
         return
     if args.mode == "smoother":
         deck, h, u, f, D, s = problem(args.deck, MG_PARAMS)
+        if args.no_fused_levels:
+            s.set_fused_levels(False)
         # a plain copy of known size calibrates the byte counters
         x = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
         y = torch.empty_like(x)
@@ -484,6 +491,13 @@ def main():
     gcells = {"level_%d" % ln: smoother_rate(s, deck, ln, args.smoother_sweeps)
               for ln in range(nl)}
     log("bench: smoother Gcell/s %s" % json.dumps(gcells))
+    shapes = [None, (16, 8, 8), (8, 8, 8), (32, 4, 4), (16, 16, 4), (16, 8, 16)]
+    level_smoother = {}
+    for ln in range(1, nl):
+        v = s.fused_level_variant(ln)
+        level_smoother["level_%d" % ln] = ("colour passes" if v < 0 else
+                                           "fused sweep, tile %dx%dx%d" % shapes[v + 1])
+    log("bench: finer-level smoother (autotuned) %s" % json.dumps(level_smoother))
     u_ref = u  # the FP64 solution (mixed-precision errors are measured against it)
 
     # red-black coarse solve beside it (does not reach 1e-10 on this deck)
@@ -565,7 +579,8 @@ def main():
                                   "coarsest level multigrid V(2,2) rel tol 1e-8 <= 10 V-cycles, "
                                   "rel tol 1e-10"),
                cycles=cycles, converged=all(converged), final_residual=float(hist[-1]),
-               smoother_gcells_per_s=gcells, roofline=roofline, cpu_baseline=cpu,
+               smoother_gcells_per_s=gcells, finer_level_smoother=level_smoother,
+               roofline=roofline, cpu_baseline=cpu,
                redblack_coarse=rb, mixed_precision=mixed, c1=c1, learned_smoother=learned)
     print(json.dumps(out), flush=True)
 

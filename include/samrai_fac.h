@@ -272,6 +272,20 @@ int sfac_solver_set_algorithm_choice(sfac_solver* s, const char* choice);
 int sfac_solver_set_fused_coarsest(sfac_solver* s, int on, int tile_variant);
 int sfac_solver_fused_coarsest_active(sfac_solver* s, int* active);
 int sfac_solver_fused_tile_variant(sfac_solver* s, int* variant);
+/* Execution choice with no reference counterpart (results are unchanged):
+ * in 3-D, each level above the coarsest smooths (smoothErrorByRedBlack
+ * without a tolerance, CellPoissonFACOps.C:1338-1514) with one colour-fused
+ * launch per sweep over all its patches, reading same-level neighbours
+ * straight from the neighbouring patches instead of filling ghosts between
+ * the colours, when the level qualifies (non-negative indices, patch boxes
+ * aligned to the ratio).  on = 0 keeps the patch-wise colour passes.
+ * tile_variant: the tile shape (0 .. 4: 16x8x8, 8x8x8, 32x4x4, 16x16x4,
+ * 16x8x16) or -1 (default) for the autotuner, which times the colour
+ * passes and every shape once per level and keeps the fastest.
+ * fused_level_variant reports the shape level ln uses after
+ * sfac_solver_initialize, -1 when the level runs the colour passes. */
+int sfac_solver_set_fused_levels(sfac_solver* s, int on, int tile_variant);
+int sfac_solver_fused_level_variant(sfac_solver* s, int ln, int* variant);
 /* The autotuner's measured milliseconds per sweep of each tile shape (n = 0
  * when the shape was set explicitly). */
 int sfac_solver_fused_tile_timings(sfac_solver* s, double* ms_per_sweep, int max, int* n);

@@ -379,6 +379,21 @@ int sfac_solver_fused_coarsest_active(sfac_solver* s, int* active) {
   });
 }
 
+int sfac_solver_set_fused_levels(sfac_solver* s, int on, int tile_variant) {
+  return guarded([&] {
+    need(s, "solver");
+    s->s->ops().setFusedLevels(on != 0, tile_variant);
+  });
+}
+
+int sfac_solver_fused_level_variant(sfac_solver* s, int ln, int* variant) {
+  return guarded([&] {
+    need(s, "solver");
+    need(variant, "variant");
+    *variant = s->s->ops().fusedLevelActive(ln) ? s->s->ops().fusedLevelVariant(ln) : -1;
+  });
+}
+
 int sfac_solver_set_graphs(sfac_solver* s, int on) {
   return guarded([&] {
     need(s, "solver");

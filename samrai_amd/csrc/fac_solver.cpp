@@ -25,6 +25,26 @@ int learned_attach(void* net, int delta);
 
 namespace pdat {
 
+std::vector<std::vector<long>> PatchDataSet::layout(const hier::PatchHierarchy& h, Centring c,
+                                                    int ghost, int ln, long* total_out) {
+  const int dim = h.dim();
+  const int ncomp = c == kCell ? 1 : dim;
+  const auto& boxes = h.level(ln).boxes;
+  const int np = (int)boxes.size();
+  std::vector<std::vector<long>> off(ncomp, std::vector<long>(np));
+  long total = 0;
+  for (int p = 0; p < np; ++p)
+    for (int a = 0; a < ncomp; ++a) {
+      long n = 1;
+      for (int d = 0; d < dim; ++d)
+        n *= boxes[p].hi[d] - boxes[p].lo[d] + 1 + 2 * ghost + (c == kSide && d == a ? 1 : 0);
+      off[a][p] = total;
+      total += (n + 31) / 32 * 32;  // 256-byte aligned patch arrays
+    }
+  if (total_out) *total_out = total;
+  return off;
+}
+
 PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost)
     : d_h(h), d_centring(c), d_ghost(ghost) {
   if (!h.finalized()) fail("PatchDataSet: hierarchy not finalized");
@@ -35,17 +55,15 @@ PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost)
     Lvl& L = d_levels[ln];
     const auto& boxes = h.level(ln).boxes;
     const int np = (int)boxes.size();
-    L.off.assign(ncomp, std::vector<long>(np));
-    L.len.assign(ncomp, std::vector<long>(np));
     long total = 0;
+    L.off = layout(h, c, ghost, ln, &total);
+    L.len.assign(ncomp, std::vector<long>(np));
     for (int p = 0; p < np; ++p)
       for (int a = 0; a < ncomp; ++a) {
         long n = 1;
         for (int d = 0; d < dim; ++d)
           n *= boxes[p].hi[d] - boxes[p].lo[d] + 1 + 2 * ghost + (c == kSide && d == a ? 1 : 0);
-        L.off[a][p] = total;
         L.len[a][p] = n;
-        total += (n + 31) / 32 * 32;  // 256-byte aligned patch arrays
       }
     L.total = total;
     hip_check(hipMalloc(&L.slab, sizeof(double) * (total > 0 ? total : 1)), "hipMalloc(data)");
@@ -211,6 +229,7 @@ void CellPoissonFACOps::initializeOperatorState(const SAMRAIVectorReal& u,
   d_partial_n = maxchunks;
   hip_check(hipMalloc(&d_partial, sizeof(double) * d_partial_n), "hipMalloc(partial)");
   setupBoxPath();
+  setupFusedLevels(f.data->ghost());
   if (d_learned) checkLearnedSpecification();
 }
 
@@ -226,6 +245,7 @@ void CellPoissonFACOps::deallocateOperatorState() {
   if (d_partial) (void)hipFree(d_partial);
   d_partial = nullptr;
   releaseBoxPath();
+  releaseFusedLevels();
   d_ln_min = d_ln_max = -1;
 }
 
@@ -314,6 +334,237 @@ int CellPoissonFACOps::tuneBoxVariant() {
   cache[key] = {best, times};
   d_box_timings = times;
   return best;
+}
+
+// ------------------------------------------- fused sweeps of the finer levels
+
+void CellPoissonFACOps::setFusedLevels(bool on, int variant) {
+  if (variant < -1 || variant >= sfac::fused_tile_variants())
+    fail("CellPoissonFACOps: fused tile variant out of range");
+  d_fused_enabled = on;
+  d_fused_variant = variant;
+  for (int ln = 0; ln < (int)d_fused.size(); ++ln)
+    if (d_fused[ln].ok) d_fused[ln].variant = variant >= 0 ? variant : tuneFusedVariant(ln);
+  ++d_generation;
+}
+
+// Per level above the coarsest: the tiles of every tile shape and their
+// halo segments, when the level qualifies (3-D; non-negative indices, since
+// the reference's truncating parity test colours negative index sums
+// differently; patch boxes aligned to the ratio, so the Ewing mirror cell
+// lies in the patch).
+void CellPoissonFACOps::setPoissonSpecifications(const PoissonSpecifications& s) {
+  d_spec = s;
+  ++d_generation;
+  // the fused sweeps' records hold the D and C arrays' layouts
+  if (d_ln_min >= 0) setupFusedLevels(d_fused_rg);
+}
+
+void CellPoissonFACOps::setupFusedLevels(int rg) {
+  releaseFusedLevels();
+  d_fused_rg = rg;
+  const int dim = d_h.dim();
+  const int nl = d_h.numberOfLevels();
+  d_fused.assign(nl, FusedLevel{});
+  if (dim != 3) return;
+  bool any = false;
+  for (int ln = std::max(d_ln_min + 1, 1); ln <= d_ln_max; ++ln) {
+    const hier::PatchLevel& L = d_h.level(ln);
+    const auto& boxes = L.boxes;
+    const int r = L.ratio;
+    bool ok = !boxes.empty();
+    for (const sfac::Box& b : boxes)
+      for (int d = 0; d < 3 && ok; ++d)
+        ok = b.lo[d] >= 0 && b.lo[d] % r == 0 && (b.hi[d] + 1) % r == 0;
+    if (!ok) continue;
+    FusedLevel& F = d_fused[ln];
+    F.rg = rg;
+    F.dg = d_spec.dIsConstant() ? 0 : d_spec.getD()->ghost();
+    F.cg = d_spec.cMode() == 2 ? d_spec.getC()->ghost() : 0;
+    std::vector<sfac::FusedOffsets> off(boxes.size());
+    {
+      const auto ou = PatchDataSet::layout(d_h, PatchDataSet::kCell, 1, ln);
+      const auto orr = PatchDataSet::layout(d_h, PatchDataSet::kCell, rg, ln);
+      const auto od = PatchDataSet::layout(d_h, PatchDataSet::kSide, F.dg, ln);
+      const auto oc = PatchDataSet::layout(d_h, PatchDataSet::kCell, F.cg, ln);
+      for (size_t p = 0; p < boxes.size(); ++p) {
+        off[p].u = ou[0][p];
+        off[p].r = orr[0][p];
+        for (int a = 0; a < 3; ++a) off[p].d[a] = od[a][p];
+        off[p].c = oc[0][p];
+      }
+    }
+    for (int v = 0; v < sfac::fused_tile_variants(); ++v) {
+      int sh[3];
+      sfac::fused_tile_shape(v, sh);
+      std::vector<sfac::FusedTile> tiles;
+      std::vector<sfac::FusedSeg> segs;
+      const bool built = hier::buildFusedTiles(L, d_h.tables(ln).cf_off, off, sh, tiles, segs);
+      ok = ok && built;
+      sfac::FusedTile* dt = nullptr;
+      sfac::FusedSeg* ds = nullptr;
+      hip_check(hipMalloc(&dt, sizeof(sfac::FusedTile) * tiles.size()), "hipMalloc(tiles)");
+      hip_check(hipMemcpy(dt, tiles.data(), sizeof(sfac::FusedTile) * tiles.size(),
+                          hipMemcpyHostToDevice),
+                "hipMemcpy(tiles)");
+      hip_check(hipMalloc(&ds, sizeof(sfac::FusedSeg) * segs.size()), "hipMalloc(segments)");
+      hip_check(hipMemcpy(ds, segs.data(), sizeof(sfac::FusedSeg) * segs.size(),
+                          hipMemcpyHostToDevice),
+                "hipMemcpy(segments)");
+      F.tiles.push_back(dt);
+      F.segs.push_back(ds);
+      F.ntiles.push_back((int)tiles.size());
+    }
+    for (int d = 0; d < 3; ++d) {
+      F.dom.lo[d] = L.dlo[d];
+      F.dom.hi[d] = L.dhi[d];
+    }
+    if (!ok) continue;  // released with the others
+    F.ok = true;
+    any = true;
+  }
+  if (any) {
+    d_fused_tmp.reset(new PatchDataSet(d_h, PatchDataSet::kCell, 1));
+    for (int ln = 0; ln < nl; ++ln)
+      if (d_fused[ln].ok)
+        d_fused[ln].variant = d_fused_variant >= 0 ? d_fused_variant : tuneFusedVariant(ln);
+  }
+}
+
+void CellPoissonFACOps::releaseFusedLevels() {
+  for (FusedLevel& F : d_fused) {
+    for (sfac::FusedTile* t : F.tiles) (void)hipFree(t);
+    for (sfac::FusedSeg* t : F.segs) (void)hipFree(t);
+  }
+  d_fused.clear();
+  d_fused_tmp.reset();
+}
+
+// Autotuner: time 6 sweeps of the patch-wise colour passes and of every
+// fused tile shape on level ln (zeroed arrays: timing does not depend on the
+// values) and keep the fastest; -1: the colour passes.  Cached per level
+// geometry.
+int CellPoissonFACOps::tuneFusedVariant(int ln) {
+  FusedLevel& F = d_fused[ln];
+  const auto& T = d_h.tables(ln);
+  const auto& boxes = d_h.level(ln).boxes;
+  std::vector<long> key = {(long)ln, (long)boxes.size(), d_spec.dIsConstant() ? 0 : 1,
+                           d_spec.cMode(), d_ewing ? 1 : 0, F.rg, F.dg, F.cg};
+  for (const sfac::Box& b : boxes)
+    for (int d = 0; d < 3; ++d) {
+      key.push_back(b.lo[d]);
+      key.push_back(b.hi[d]);
+    }
+  static std::mutex mu;
+  static std::map<std::vector<long>, int> cache;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
+  PatchDataSet& tmp = *d_fused_tmp;
+  PatchDataSet zero(d_h, PatchDataSet::kCell, 1);
+  PatchDataSet rhs(d_h, PatchDataSet::kCell, F.rg);
+  const sfac::PoissonDev ps = poissonDev(ln);
+  const double* Ds = d_spec.dIsConstant() ? nullptr : d_spec.getD()->slab(ln);
+  const double* Cs = d_spec.cMode() == 2 ? d_spec.getC()->slab(ln) : nullptr;
+  const sfac::RobinDev bc = robinDev();
+  hipEvent_t t0, t1;
+  hip_check(hipEventCreate(&t0), "hipEventCreate");
+  hip_check(hipEventCreate(&t1), "hipEventCreate");
+  const int ewing = (d_ewing && T.has_cf) ? 1 : 0;
+  auto elapsed = [&]() {
+    hip_check(hipEventRecord(t1, d_stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(t1), "hipEventSynchronize");
+    float ms = 0.0f;
+    hip_check(hipEventElapsedTime(&ms, t0, t1), "hipEventElapsedTime");
+    return ms;
+  };
+  // the patch-wise colour passes (what smoothErrorByRedBlack runs otherwise):
+  // per sweep, [Ewing snapshot] red, face ghost fill, [Ewing snapshot]
+  // black, face ghost fill
+  for (int i = 0; i < 8; ++i) {
+    if (i == 2) hip_check(hipEventRecord(t0, d_stream), "hipEventRecord");
+    for (int colour = 0; colour < 2; ++colour) {
+      if (ewing)
+        sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, zero.table(ln), 1, ps, nullptr,
+                                 d_stream);
+      sfac::launch_rbgs(T.smooth.smooth(), T.geom, zero.table(ln), 1, rhs.table(ln), F.rg, ps,
+                        ewing ? 2 : 0, colour, nullptr, nullptr, d_stream);
+      sfac::launch_ghost_items(T.face_nc.launch(), T.geom, zero.table(ln), 1, nullptr, 0,
+                               nullptr, 1, bc, nullptr, d_stream);
+    }
+  }
+  int best = -1;
+  float best_ms = elapsed();
+  for (int v = 0; v < sfac::fused_tile_variants(); ++v) {
+    const sfac::FusedLaunch w = F.launch(v);
+    for (int i = 0; i < 8; ++i) {
+      if (i == 2) hip_check(hipEventRecord(t0, d_stream), "hipEventRecord");
+      const PatchDataSet& src = (i & 1) ? tmp : zero;
+      PatchDataSet& dst = (i & 1) ? zero : tmp;
+      if (ewing)
+        sfac::launch_ewing_split(T.cf_faces.launch(), T.geom, src.table(ln), zero.table(ln), ps,
+                                 d_stream);
+      sfac::launch_level_rbgs(w, v, T.geom, F.dom, src.slab(ln), dst.slab(ln), zero.slab(ln),
+                              rhs.slab(ln), F.rg, Ds, Cs, ps, bc, ewing, d_stream);
+    }
+    const float ms = elapsed();
+    if (ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+  }
+  (void)hipEventDestroy(t0);
+  (void)hipEventDestroy(t1);
+  std::lock_guard<std::mutex> lk(mu);
+  cache[key] = best;
+  return best;
+}
+
+// smoothErrorByRedBlack on a level above the coarsest without a tolerance:
+// the coarse-fine ghosts once (they stay fixed), then one fused launch per
+// sweep (plus the Ewing flux snapshot) alternating between e and the
+// scratch copy, and the reference's closing same-level/physical ghost fill.
+// e ends bit-identical to the patch-wise colour passes.
+int CellPoissonFACOps::smoothFused(PatchDataSet& e, const PatchDataSet& r, int ln,
+                                   int num_sweeps) {
+  const auto& T = d_h.tables(ln);
+  const FusedLevel& F = d_fused[ln];
+  const sfac::PoissonDev ps = poissonDev(ln);
+  const sfac::RobinDev bc = robinDev();
+  const int ewing = d_ewing ? 1 : 0;
+  PatchDataSet& tmp = *d_fused_tmp;
+  const double* Ds = d_spec.dIsConstant() ? nullptr : d_spec.getD()->slab(ln);
+  const double* Cs = d_spec.cMode() == 2 ? d_spec.getC()->slab(ln) : nullptr;
+  ghostFill(e, ln, true, true, false);
+  const double launch_bytes = (d_spec.dIsConstant() ? 24.0 : 48.0) * (double)T.ncells;
+  for (int s = 0; s < num_sweeps; ++s) {
+    const PatchDataSet& src = (s & 1) ? tmp : e;
+    PatchDataSet& dst = (s & 1) ? e : tmp;
+    if (ewing && T.has_cf)
+      sfac::launch_ewing_split(T.cf_faces.launch(), T.geom, src.table(ln), e.table(ln), ps,
+                               d_stream);
+    TimedLaunch tl{};
+    if (d_timing) {
+      tl.start = nextEvent();
+      hip_check(hipEventRecord(tl.start, d_stream), "hipEventRecord");
+    }
+    sfac::launch_level_rbgs(F.launch(), F.variant, T.geom, F.dom, src.slab(ln), dst.slab(ln),
+                            e.slab(ln), r.slab(ln), r.ghost(), Ds, Cs, ps, bc, ewing, d_stream);
+    if (d_timing) {
+      tl.stop = nextEvent();
+      hip_check(hipEventRecord(tl.stop, d_stream), "hipEventRecord");
+      tl.level = ln;
+      tl.count = 1;
+      tl.bytes = launch_bytes;
+      d_timed.push_back(tl);
+    }
+  }
+  if (num_sweeps & 1)
+    sfac::launch_copy_interior(T.cells.launch(), T.geom, e.table(ln), tmp.table(ln), 1, d_stream);
+  ghostFill(e, ln, false, true, false);
+  return num_sweeps;
 }
 
 void CellPoissonFACOps::releaseBoxPath() {
@@ -770,6 +1021,12 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
                                              int num_sweeps, double tol, double* maxres_out) {
   if (d_box.ok && d_box_enabled && ln == d_box.ln)
     return smoothBox(e, r, num_sweeps, tol, maxres_out);
+  if (tol < 0.0 && fusedLevelActive(ln) && e.ghost() == 1 && e.centring() == PatchDataSet::kCell &&
+      r.centring() == PatchDataSet::kCell && r.ghost() == d_fused[ln].rg &&
+      &e != d_fused_tmp.get() && &r != d_fused_tmp.get() &&
+      (d_spec.dIsConstant() || d_spec.getD()->ghost() == d_fused[ln].dg) &&
+      (d_spec.cMode() != 2 || d_spec.getC()->ghost() == d_fused[ln].cg))
+    return smoothFused(e, r, ln, num_sweeps);
   const auto& T = d_h.tables(ln);
   const sfac::PoissonDev ps = poissonDev(ln);
   const int ewing = (d_ewing && ln > d_ln_min) ? 1 : 0;

@@ -18,6 +18,7 @@
 
 #include "box_kernels.h"
 #include "hierarchy.hpp"
+#include "level_fused.h"
 
 namespace samrai_amd {
 namespace pdat {
@@ -44,6 +45,10 @@ class PatchDataSet {
   // Device table indexed [p * ncomp + a].
   double* const* interleaved(int ln) const { return d_levels[ln].dil; }
   double* slab(int ln) const { return d_levels[ln].slab; }
+  // Element offsets [a][p] of every patch array of level ln in its slab, the
+  // same for every PatchDataSet of this hierarchy, centring and ghost width.
+  static std::vector<std::vector<long>> layout(const hier::PatchHierarchy& h, Centring c,
+                                               int ghost, int ln, long* total = nullptr);
   long slabSize(int ln) const { return d_levels[ln].total; }
 
  private:
@@ -134,7 +139,7 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   void setCoarsestLevelSolverMaxIterations(int n) { d_coarse_max = n; ++d_generation; }
   void setCoarseFineDiscretization(const std::string& c);
   void setProlongationMethod(const std::string& m);
-  void setPoissonSpecifications(const PoissonSpecifications& s) { d_spec = s; ++d_generation; }
+  void setPoissonSpecifications(const PoissonSpecifications& s);
   void setPhysicalBcCoefObject(const LocationIndexRobinBcCoefs& bc) { d_bc = bc; ++d_generation; }
   void enableLogging(bool on) { d_logging = on; ++d_generation; }
   // Whole FAC cycles may be captured in HIP graphs and replayed (no kernel
@@ -195,6 +200,20 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   // the autotuner's measured time per sweep of every tile shape (ms; empty
   // when the shape was given or came from the cache)
   const std::vector<float>& fusedTileTimings() const { return d_box_timings; }
+  // Colour-fused sweeps on the levels above the coarsest (level_fused.h):
+  // on by default where a level qualifies (3-D, non-negative indices,
+  // patches aligned to the ratio); variant picks the tile shape,
+  // -1 lets the autotuner time every shape and the patch-wise colour passes
+  // once per level and keep the fastest.  Results are bit-identical to the
+  // patch-wise colour passes.
+  void setFusedLevels(bool on, int variant);
+  bool fusedLevelActive(int ln) const {
+    return d_fused_enabled && ln > d_ln_min && ln < (int)d_fused.size() && d_fused[ln].ok &&
+           d_fused[ln].variant >= 0;
+  }
+  int fusedLevelVariant(int ln) const {
+    return ln >= 0 && ln < (int)d_fused.size() && d_fused[ln].ok ? d_fused[ln].variant : -1;
+  }
   // V-cycles and final relative residual of the last multigrid coarse solve
   void mgStats(int* iterations, double* relres);
   // Storage and arithmetic of the single-rectangle coarsest-level solve
@@ -297,6 +316,27 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   bool learnedLevel(int ln) const;
   void checkLearnedSpecification();
   int d_box_variant = -1;
+  // fused sweeps of the levels above the coarsest
+  struct FusedLevel {
+    bool ok = false;
+    int rg = 0, dg = 0, cg = 0;  // ghost widths the records' offsets assume
+    std::vector<sfac::FusedTile*> tiles;  // per tile-shape variant
+    std::vector<sfac::FusedSeg*> segs;
+    std::vector<int> ntiles;
+    int variant = 0;
+    sfac::Box dom{};
+    sfac::FusedLaunch launch(int v) const { return {tiles[v], ntiles[v], segs[v]}; }
+    sfac::FusedLaunch launch() const { return launch(variant); }
+  };
+  std::vector<FusedLevel> d_fused;
+  bool d_fused_enabled = true;
+  int d_fused_variant = -1;
+  std::unique_ptr<pdat::PatchDataSet> d_fused_tmp;
+  void setupFusedLevels(int rg);
+  int d_fused_rg = 0;
+  void releaseFusedLevels();
+  int tuneFusedVariant(int ln);
+  int smoothFused(pdat::PatchDataSet& e, const pdat::PatchDataSet& r, int ln, int num_sweeps);
   void setupBoxPath();
   void releaseBoxPath();
   int smoothBox(pdat::PatchDataSet& e, const pdat::PatchDataSet& r, int num_sweeps, double tol,

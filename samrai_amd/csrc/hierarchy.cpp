@@ -246,6 +246,143 @@ void PatchHierarchy::finalize() {
   d_final = true;
 }
 
+bool buildFusedTiles(const PatchLevel& L, const std::vector<long>& cf_off,
+                     const std::vector<sfac::FusedOffsets>& off, const int* tile,
+                     std::vector<sfac::FusedTile>& tiles, std::vector<sfac::FusedSeg>& segs) {
+  const int dim = 3;
+  const auto& boxes = L.boxes;
+  const int np = (int)boxes.size();
+  Box dom;
+  for (int d = 0; d < 3; ++d) {
+    dom.lo[d] = L.dlo[d];
+    dom.hi[d] = L.dhi[d];
+  }
+  std::vector<std::vector<int>> nbr(np);
+  for (int p = 0; p < np; ++p) {
+    const Box g = grow(boxes[p], 3, dim);
+    Box o;
+    for (int q = 0; q < np; ++q)
+      if (q != p && intersect(g, boxes[q], o, dim)) nbr[p].push_back(q);
+  }
+  bool ok = true;
+  int nhalo = 0;
+  auto seg = [&](int q, int kind, const Box& b) {
+    sfac::FusedSeg s{};
+    s.off = off[q];
+    s.q = q;
+    s.kind = kind;
+    s.base = nhalo;
+    for (int d = 0; d < 3; ++d) {
+      s.lo[d] = b.lo[d];
+      s.hi[d] = b.hi[d];
+      s.qlo[d] = boxes[q].lo[d];
+      s.qhi[d] = boxes[q].hi[d];
+    }
+    s.e0 = b.hi[0] - b.lo[0] + 1;
+    s.e1 = b.hi[1] - b.lo[1] + 1;
+    s.inv0 = 1.0f / (float)s.e0;
+    s.inv01 = 1.0f / (float)(s.e0 * s.e1);
+    for (int f = 0; f < 6; ++f) s.cfo[f] = (int)cf_off[(size_t)q * 6 + f];
+    nhalo += (int)boxsize(b, dim);
+    segs.push_back(s);
+  };
+  for (int p = 0; p < np; ++p) {
+    const Box& B = boxes[p];
+    for (int k = B.lo[2]; k <= B.hi[2]; k += tile[2])
+      for (int j = B.lo[1]; j <= B.hi[1]; j += tile[1])
+        for (int i = B.lo[0]; i <= B.hi[0]; i += tile[0]) {
+          Box T{{i, j, k}, {std::min(i + tile[0] - 1, B.hi[0]), std::min(j + tile[1] - 1, B.hi[1]),
+                            std::min(k + tile[2] - 1, B.hi[2])}};
+          // the cells within Manhattan distance 2 of the tile (what its red
+          // updates on the tile grown by one and its black updates read),
+          // outside the tile, as disjoint boxes
+          std::vector<Box> need;
+          for (int a = 0; a < 3; ++a)
+            for (int b2 = a; b2 < 3; ++b2) {
+              Box X = T;
+              if (a == b2) {
+                X.lo[a] -= 2;
+                X.hi[a] += 2;
+              } else {
+                X.lo[a] -= 1;
+                X.hi[a] += 1;
+                X.lo[b2] -= 1;
+                X.hi[b2] += 1;
+              }
+              std::vector<Box> parts{X};
+              for (const Box& n : need) parts = subtract(parts, n, dim);
+              parts = subtract(parts, T, dim);
+              need.insert(need.end(), parts.begin(), parts.end());
+            }
+          need = subtract(need, B, dim);
+          const int seg0 = (int)segs.size();
+          nhalo = 0;
+          for (int q : nbr[p]) {
+            std::vector<Box> rest;
+            for (const Box& pc : need) {
+              Box I;
+              if (intersect(pc, boxes[q], I, dim)) {
+                seg(q, 0, I);
+                subtract_one(pc, boxes[q], dim, rest);
+              } else {
+                rest.push_back(pc);
+              }
+            }
+            need.swap(rest);
+          }
+          // coarse-fine ghosts: the rest inside the domain, from the ghost
+          // layer of the tile's patch where it reaches, else of a neighbour
+          // (positions no interior cell of the grown tile borders are not
+          // needed)
+          {
+            std::vector<Box> pieces;
+            for (const Box& pc : need) {
+              Box I;
+              if (intersect(pc, dom, I, dim)) pieces.push_back(I);
+            }
+            auto take = [&](int q) {
+              const Box gq = grow(boxes[q], 1, dim);
+              std::vector<Box> rest;
+              for (const Box& pc : pieces) {
+                Box J;
+                if (intersect(pc, gq, J, dim)) {
+                  seg(q, 1, J);
+                  subtract_one(pc, gq, dim, rest);
+                } else {
+                  rest.push_back(pc);
+                }
+              }
+              pieces.swap(rest);
+            };
+            take(p);
+            for (int q : nbr[p]) take(q);
+          }
+          sfac::FusedTile t{};
+          t.off = off[p];
+          t.p = p;
+          t.seg0 = seg0;
+          t.nseg = (int)segs.size() - seg0;
+          t.nhalo = nhalo;
+          ok = ok && t.nseg <= sfac::kFusedMaxSegments;
+          for (int d = 0; d < 3; ++d) {
+            t.lo[d] = T.lo[d];
+            t.hi[d] = T.hi[d];
+            t.plo[d] = B.lo[d];
+            t.phi[d] = B.hi[d];
+          }
+          for (int f = 0; f < 6; ++f) t.cfo[f] = (int)cf_off[(size_t)p * 6 + f];
+          tiles.push_back(t);
+        }
+  }
+  while (tiles.size() % 8) {
+    sfac::FusedTile t{};
+    t.p = -1;
+    tiles.push_back(t);
+  }
+  if (segs.empty()) segs.push_back(sfac::FusedSeg{});
+  return ok;
+}
+
 void PatchHierarchy::buildLevel(int ln) {
   const int dim = d_dim;
   const PatchLevel& L = d_levels[ln];
@@ -395,6 +532,7 @@ void PatchHierarchy::buildLevel(int ln) {
   if (cf.empty()) cf.push_back(0);
   T.d_cf = upload(cf);
   T.d_cf_off = upload(cf_off);
+  T.cf_off = cf_off;
   {
     std::vector<double> zero(cf.size(), 0.0);
     T.d_cf_flux = upload(zero);

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "level_fused.h"
 #include "level_kernels.h"
 
 namespace samrai_amd {
@@ -66,6 +67,7 @@ class PatchHierarchy {
     double* d_cf_flux = nullptr;
     DevList cf_faces;         // p patch, q face, interior layer next to a C/F face
     bool has_cf = false;
+    std::vector<long> cf_off;  // host copy of d_cf_off
     sfac::LevelGeom geom{};
     DevList cells;            // interior of every patch
     DevList smooth;           // colour half-rows of every patch
@@ -92,6 +94,18 @@ class PatchHierarchy {
   T* upload(const std::vector<T>& v);
   DevList uploadList(const std::vector<BoxItem>& items, bool colour_space = false);
 };
+
+// Tiles of shape tile[3] over every patch of level L (3-D, patch-major,
+// padded with p = -1 to a multiple of 8) and the halo segments of each
+// tile's box grown by two: the parts other patches of the level cover
+// (kind 0) and, on the tile faces that lie on its patch's faces, the
+// coarse-fine ghosts two cells out that border another patch (kind 1).
+// cf_off: the level's coarse-fine face table offsets (LevelTables::cf_off).
+// Returns false when some tile needs more than kFusedMaxSegments segments.
+// off: each patch's operand offsets (FusedOffsets).
+bool buildFusedTiles(const PatchLevel& L, const std::vector<long>& cf_off,
+                     const std::vector<sfac::FusedOffsets>& off, const int* tile,
+                     std::vector<sfac::FusedTile>& tiles, std::vector<sfac::FusedSeg>& segs);
 
 }  // namespace hier
 

@@ -81,6 +81,8 @@ def lib():
     L.sfac_solver_fused_coarsest_active.argtypes = [_vp, _ip]
     L.sfac_solver_fused_tile_variant.argtypes = [_vp, _ip]
     L.sfac_solver_fused_tile_timings.argtypes = [_vp, _dp, C.c_int, _ip]
+    L.sfac_solver_set_fused_levels.argtypes = [_vp, C.c_int, C.c_int]
+    L.sfac_solver_fused_level_variant.argtypes = [_vp, C.c_int, _ip]
     L.sfac_solver_coarse_mg_stats.argtypes = [_vp, _ip, _dp]
     L.sfac_solver_set_graphs.argtypes = [_vp, C.c_int]
     L.sfac_solver_set_algorithm_choice.argtypes = [_vp, C.c_char_p]
@@ -327,6 +329,18 @@ class CellPoissonFACSolver:
         tile_variant -1: autotuned tile shape."""
         _check(lib().sfac_solver_set_fused_coarsest(self.s, int(on), int(tile_variant)))
 
+    def set_fused_levels(self, on=True, tile_variant=-1):
+        """Colour-fused sweeps on the levels above the coarsest (3-D; one
+        launch per sweep over all patches, same results); off: patch-wise
+        colour passes.  tile_variant -1: autotuned tile shape."""
+        _check(lib().sfac_solver_set_fused_levels(self.s, int(on), int(tile_variant)))
+
+    def fused_level_variant(self, ln):
+        """Tile shape level ln's fused sweeps use, -1 if it runs colour passes."""
+        v = C.c_int()
+        _check(lib().sfac_solver_fused_level_variant(self.s, int(ln), C.byref(v)))
+        return v.value
+
     def set_algorithm_choice(self, choice):
         """FACPreconditioner::setAlgorithmChoice: "default", "mccormick-s4.3"
         or "pernice"."""
@@ -379,6 +393,8 @@ class CellPoissonFACSolver:
     def smoother_kernel_name(self, ln):
         """Name of the kernel that smoothErrorByRedBlack launches on level ln
         (for reports)."""
+        if ln > 0 and self.fused_level_variant(ln) >= 0:
+            return "k_level_rbgs (colour-fused sweep)"
         return "k_rbgs (one colour pass)"
 
     def internal_vector(self, which):
