@@ -508,17 +508,34 @@ def main():
     sr.close()
     del ur, fr, Dr
 
-    # mixed precision (coarsest-level solve in FP32 inside the FP64 iteration)
-    _, _, um, fm, Dm, sm = problem("c2", MG_PARAMS)
-    sm.set_coarse_precision(32)
-    mixed = timed_solves(torch, sm, um, fm, max(1, min(args.steps, 5)))
-    mixed["error_vs_fp64_per_level"] = level_errors(deck, um, u_ref)
-    mixed["fp64_s_per_solve"] = round(ms_per_step / 1e3, 4)
-    mixed["scope"] = ("coarsest-level solve (multigrid) in FP32 storage and arithmetic; FAC "
-                      "iteration, residuals, finer levels and u in FP64")
-    log("bench: C2 mixed precision %s" % json.dumps(mixed))
-    sm.close()
-    del um, fm, Dm
+    # mixed precision inside the FP64 iteration: the coarsest-level solve in
+    # FP32, then the error vector on every level in FP32 and in BF16 storage
+    mixed = dict(fp64_s_per_solve=round(ms_per_step / 1e3, 4))
+    runs = [("fp32_coarse_only", 64, 32,
+             "coarsest-level solve (multigrid) in FP32 storage and arithmetic; FAC iteration, "
+             "residuals, finer levels and u in FP64"),
+            ("fp32_all_levels", 32, 32,
+             "error vector in FP32 on every level: smoothing, its ghost fills and the "
+             "prolongation in FP32 (D in an FP32 copy), coarsest-level solve in FP32; composite "
+             "residuals, coarse-fine flux corrections, restriction, u += e and the "
+             "convergence test in FP64"),
+            ("bf16_all_levels", 16, 32,
+             "error vector in BF16 storage on every level (FP32 arithmetic, D in a BF16 "
+             "copy), coarsest-level solve in FP32; residuals, flux corrections, restriction, "
+             "u += e and the convergence test in FP64")]
+    for name, bits, coarse_bits, scope in runs:
+        _, _, um, fm, Dm, sm = problem("c2", MG_PARAMS)
+        sm.set_coarse_precision(coarse_bits)
+        if bits != 64:
+            sm.set_mixed_precision(bits)
+        r = timed_solves(torch, sm, um, fm, max(1, min(args.steps, 5)))
+        r["speedup_vs_fp64"] = round(ms_per_step / 1e3 / r["s_per_solve"], 3)
+        r["error_vs_fp64_per_level"] = level_errors(deck, um, u_ref)
+        r["scope"] = scope
+        mixed[name] = r
+        log("bench: C2 mixed precision %s %s" % (name, json.dumps(r)))
+        sm.close()
+        del um, fm, Dm
 
     c1 = None
     if not args.no_c1:

@@ -324,6 +324,21 @@ int sfac_berger_rigoutsos(int dim, const int* lo, const int* hi, const unsigned 
  * arithmetic.  Takes effect at once (re-creates the coarse buffers). */
 int sfac_solver_set_coarse_precision(sfac_solver* s, int bits);
 
+/* Mixed precision on every level, no reference counterpart: bits = 32 (FP32)
+ * or 16 (BF16 storage) makes the error vector of the FAC iteration
+ * low-precision storage.  The V-cycle's smoothing (red-black sweeps with the
+ * Ewing fix of the smoother's fluxes), the error's ghost fills and the
+ * prolongation of the coarse error run in FP32 arithmetic on it, with a
+ * low-precision copy of D made at the start of each solve; the composite
+ * residuals, the coarse-fine flux corrections and synchronisation, the
+ * restriction of residuals, u += e and the convergence test stay FP64, so
+ * the FP64 relative-residual target is still met (iterative refinement).
+ * bits = 64 (default): the reference arithmetic.  Only with the default
+ * (recursive) cycle and without the learned smoother.  Independent of
+ * sfac_solver_set_coarse_precision (the coarsest-level solve's own
+ * storage). */
+int sfac_solver_set_mixed_precision(sfac_solver* s, int bits);
+
 /* Execution choice with no reference counterpart (results are unchanged):
  * on = 1 (default) captures one FACPreconditioner cycle (e = 0, V-cycle,
  * u += e, restriction, composite residual and its norm reductions) as a HIP

@@ -94,9 +94,13 @@ void launch_box_final_check(const BoxCheck& K, int nsweeps, hipStream_t s);
 // box[0] or box[1] by the parity of *count (count == nullptr: of count_host),
 // which lets the scatter follow a sweep count known only on the device.
 // fp32: the box arrays hold float (converted from / to the FP64 patch data).
+// patch_elem: bytes per patch-data value (8 FP64; 4 FP32 and 2 BF16 for the
+// mixed-precision error vector, whose table is then typed double* only
+// nominally).
 void launch_box_cells(const GhostLaunch& cells, const LevelGeom& L, double* const* x, int g,
                       double* box0, double* box1, const int* count, int count_host,
-                      const BoxLevel& G, int to_box, hipStream_t s, int fp32 = 0);
+                      const BoxLevel& G, int to_box, hipStream_t s, int fp32 = 0,
+                      int patch_elem = 8);
 // Gather side data of axis a into the box face array; with verify, faces
 // shared by two patches are then compared and *mismatch set to 1 if the
 // patches hold different values there.

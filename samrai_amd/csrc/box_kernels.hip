@@ -387,9 +387,9 @@ __global__ void __launch_bounds__(kThreads) k_box_final_check(BoxCheck K, int ns
   }
 }
 
-template <typename T>
+template <typename T, typename PT = double>
 __global__ void __launch_bounds__(kThreads)
-k_box_cells(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* x, int g,
+k_box_cells(const BoxItem* items, const Chunk* chunks, LevelGeom L, PT* const* x, int g,
             T* box0, T* box1, const int* count, int count_host, BoxLevel G, int to_box) {
   T* box = ((count ? *count : count_host) & 1) ? box1 : box0;
   const Chunk ch = chunks[xcd_chunk()];
@@ -397,14 +397,14 @@ k_box_cells(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* cons
   const BoxItem it = items[ch.item];
   const Box pb = L.box[it.p];
   const View v = make_view(pb.lo, pb.hi, g, L.dim);
-  double* xp = x[it.p];
+  PT* xp = x[it.p];
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
     int c[3];
     if (!decode_box(it, (long)ch.start + t, c)) break;
     const long o = (long)(c[0] - G.lo[0]) +
                    (long)G.n[0] * ((c[1] - G.lo[1]) + (long)G.n[1] * (L.dim == 3 ? c[2] - G.lo[2] : 0));
-    if (to_box) box[o] = (T)xp[v.at(c[0], c[1], c[2])];
-    else xp[v.at(c[0], c[1], c[2])] = (double)box[o];
+    if (to_box) box[o] = (T)ld(xp[v.at(c[0], c[1], c[2])]);
+    else st(xp[v.at(c[0], c[1], c[2])], box[o]);
   }
 }
 
@@ -813,8 +813,34 @@ void launch_box_final_check(const BoxCheck& K, int nsweeps, hipStream_t s) {
 
 void launch_box_cells(const GhostLaunch& cells, const LevelGeom& L, double* const* x, int g,
                       double* box0, double* box1, const int* count, int count_host,
-                      const BoxLevel& G, int to_box, hipStream_t s, int fp32) {
+                      const BoxLevel& G, int to_box, hipStream_t s, int fp32, int patch_elem) {
   if (cells.nchunks == 0) return;
+  if (patch_elem != 8) {
+    // the mixed-precision error vector: FP32 / BF16 patch data, box in the
+    // coarse solve's precision
+    const dim3 g_(xcd_grid(cells.nchunks)), b_(kThreads);
+    if (patch_elem == 4) {
+      auto xp = reinterpret_cast<float* const*>(x);
+      if (fp32)
+        hipLaunchKernelGGL((k_box_cells<float, float>), g_, b_, 0, s, cells.items, cells.chunks,
+                           L, xp, g, reinterpret_cast<float*>(box0),
+                           reinterpret_cast<float*>(box1), count, count_host, G, to_box);
+      else
+        hipLaunchKernelGGL((k_box_cells<double, float>), g_, b_, 0, s, cells.items,
+                           cells.chunks, L, xp, g, box0, box1, count, count_host, G, to_box);
+    } else {
+      auto xp = reinterpret_cast<bf16* const*>(x);
+      if (fp32)
+        hipLaunchKernelGGL((k_box_cells<float, bf16>), g_, b_, 0, s, cells.items, cells.chunks,
+                           L, xp, g, reinterpret_cast<float*>(box0),
+                           reinterpret_cast<float*>(box1), count, count_host, G, to_box);
+      else
+        hipLaunchKernelGGL((k_box_cells<double, bf16>), g_, b_, 0, s, cells.items, cells.chunks,
+                           L, xp, g, box0, box1, count, count_host, G, to_box);
+    }
+    check_launch();
+    return;
+  }
   if (fp32)
     hipLaunchKernelGGL(k_box_cells<float>, dim3(xcd_grid(cells.nchunks)), dim3(kThreads), 0, s,
                        cells.items, cells.chunks, L, x, g, reinterpret_cast<float*>(box0),

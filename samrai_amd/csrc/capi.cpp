@@ -1,10 +1,12 @@
 // extern "C" wrappers of the hierarchy / patch data / solver API.
 #include <exception>
 #include <memory>
+#include <cstring>
 #include <string>
 
 #include "clustering.hpp"
 #include "fac_solver.hpp"
+#include "level_mixed.h"
 #include "samrai_fac.h"
 
 using namespace samrai_amd;
@@ -158,13 +160,38 @@ int sfac_patch_data_pointer(sfac_patch_data* d, int ln, int p, int axis, double*
   });
 }
 
+// Host copies of low-precision patch data (the solver's mixed-precision
+// error vector) convert to and from double.
+static std::vector<unsigned char> to_elem(const double* v, long n, int elem) {
+  std::vector<unsigned char> out((size_t)n * elem);
+  for (long i = 0; i < n; ++i) {
+    if (elem == 4) {
+      const float f = (float)v[i];
+      std::memcpy(&out[(size_t)i * 4], &f, 4);
+    } else {
+      const sfac::bf16 b = sfac::float_to_bf16((float)v[i]);
+      std::memcpy(&out[(size_t)i * 2], &b.bits, 2);
+    }
+  }
+  return out;
+}
+
 int sfac_patch_data_put(sfac_patch_data* d, int ln, int p, int axis, const double* host) {
   return guarded([&] {
     check_index(d, ln, p, axis);
     need(host, "host array");
-    hip_check(hipMemcpy(d->d->pointer(ln, p, axis), host, sizeof(double) * d->d->size(ln, p, axis),
-                        hipMemcpyHostToDevice),
-              "hipMemcpy(put)");
+    const long n = d->d->size(ln, p, axis);
+    const int elem = d->d->elemBytes();
+    if (elem == 8) {
+      hip_check(hipMemcpy(d->d->pointer(ln, p, axis), host, sizeof(double) * n,
+                          hipMemcpyHostToDevice),
+                "hipMemcpy(put)");
+    } else {
+      const auto bytes = to_elem(host, n, elem);
+      hip_check(hipMemcpy(d->d->pointer(ln, p, axis), bytes.data(), bytes.size(),
+                          hipMemcpyHostToDevice),
+                "hipMemcpy(put)");
+    }
   });
 }
 
@@ -173,9 +200,16 @@ int sfac_patch_data_put_device(sfac_patch_data* d, int ln, int p, int axis,
   return guarded([&] {
     check_index(d, ln, p, axis);
     need(device_src, "device array");
-    hip_check(hipMemcpy(d->d->pointer(ln, p, axis), device_src,
-                        sizeof(double) * d->d->size(ln, p, axis), hipMemcpyDeviceToDevice),
-              "hipMemcpy(put_device)");
+    const int elem = d->d->elemBytes();
+    if (elem == 8) {
+      hip_check(hipMemcpy(d->d->pointer(ln, p, axis), device_src,
+                          sizeof(double) * d->d->size(ln, p, axis), hipMemcpyDeviceToDevice),
+                "hipMemcpy(put_device)");
+    } else {
+      sfac::launch_convert(elem, device_src, d->d->pointer(ln, p, axis), d->d->size(ln, p, axis),
+                           nullptr);
+      hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    }
   });
 }
 
@@ -184,17 +218,40 @@ int sfac_patch_data_get(sfac_patch_data* d, int ln, int p, int axis, double* hos
     check_index(d, ln, p, axis);
     need(host, "host array");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    hip_check(hipMemcpy(host, d->d->pointer(ln, p, axis), sizeof(double) * d->d->size(ln, p, axis),
-                        hipMemcpyDeviceToHost),
-              "hipMemcpy(get)");
+    const long n = d->d->size(ln, p, axis);
+    const int elem = d->d->elemBytes();
+    if (elem == 8) {
+      hip_check(hipMemcpy(host, d->d->pointer(ln, p, axis), sizeof(double) * n,
+                          hipMemcpyDeviceToHost),
+                "hipMemcpy(get)");
+    } else {
+      std::vector<unsigned char> bytes((size_t)n * elem);
+      hip_check(hipMemcpy(bytes.data(), d->d->pointer(ln, p, axis), bytes.size(),
+                          hipMemcpyDeviceToHost),
+                "hipMemcpy(get)");
+      for (long i = 0; i < n; ++i) {
+        if (elem == 4) {
+          float f;
+          std::memcpy(&f, &bytes[(size_t)i * 4], 4);
+          host[i] = f;
+        } else {
+          sfac::bf16 b;
+          std::memcpy(&b.bits, &bytes[(size_t)i * 2], 2);
+          host[i] = sfac::bf16_to_float(b);
+        }
+      }
+    }
   });
 }
 
 int sfac_patch_data_fill(sfac_patch_data* d, double value) {
   return guarded([&] {
     need(d, "patch data");
+    if (d->d->elemBytes() != 8 && value != 0.0)
+      fail("low-precision patch data can only be filled with 0");
     for (int ln = 0; ln < d->d->numberOfLevels(); ++ln)
-      sfac::launch_fill(d->d->slab(ln), d->d->slabSize(ln), value, nullptr);
+      sfac::launch_fill(d->d->slab(ln), d->d->slabSize(ln) * d->d->elemBytes() / 8, value,
+                        nullptr);
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   });
 }
@@ -440,6 +497,16 @@ int sfac_solver_set_coarse_precision(sfac_solver* s, int bits) {
   return guarded([&] {
     need(s, "solver");
     s->s->ops().setCoarsePrecision(bits);
+  });
+}
+
+int sfac_solver_set_mixed_precision(sfac_solver* s, int bits) {
+  return guarded([&] {
+    need(s, "solver");
+    if (bits != 64 && bits != 32 && bits != 16) fail("mixed precision: bits must be 64, 32 or 16");
+    const int elem = bits / 8;
+    s->s->ops().setMixedPrecision(elem);
+    s->s->preconditioner().setErrorStorage(elem);
   });
 }
 

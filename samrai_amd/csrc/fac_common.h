@@ -14,6 +14,65 @@ namespace sfac {
 
 constexpr int kMaxDim = 3;
 
+// Storage of the error vector in mixed precision: FP64 (the reference), FP32
+// or BF16 (bfloat16: the upper half of an FP32 word); FP32 or BF16 storage
+// is computed in FP32 (Arith<T>).
+struct bf16 {
+  unsigned short bits;
+};
+template <class T>
+struct Arith {
+  using type = T;
+};
+template <>
+struct Arith<bf16> {
+  using type = float;
+};
+__host__ __device__ inline float bf16_to_float(bf16 v) {
+  union {
+    unsigned u;
+    float f;
+  } x;
+  x.u = (unsigned)v.bits << 16;
+  return x.f;
+}
+// round to nearest, ties to even (NaN stays NaN)
+__host__ __device__ inline bf16 float_to_bf16(float f) {
+  union {
+    unsigned u;
+    float f;
+  } x;
+  x.f = f;
+  bf16 r;
+  if ((x.u & 0x7f800000u) == 0x7f800000u && (x.u & 0x007fffffu)) {
+    r.bits = (unsigned short)((x.u >> 16) | 0x40u);
+    return r;
+  }
+  x.u += 0x7fffu + ((x.u >> 16) & 1u);
+  r.bits = (unsigned short)(x.u >> 16);
+  return r;
+}
+template <class T>
+__host__ __device__ inline typename Arith<T>::type ld(const T& v) {
+  return v;
+}
+template <>
+__host__ __device__ inline float ld<bf16>(const bf16& v) {
+  return bf16_to_float(v);
+}
+template <class T, class A>
+__host__ __device__ inline void st(T& dst, A v) {
+  dst = (T)v;
+}
+template <>
+__host__ __device__ inline void st<bf16, float>(bf16& dst, float v) {
+  dst = float_to_bf16(v);
+}
+template <>
+__host__ __device__ inline void st<bf16, double>(bf16& dst, double v) {
+  dst = float_to_bf16((float)v);
+}
+
 struct Box {
   int lo[3];
   int hi[3];

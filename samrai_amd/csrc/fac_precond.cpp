@@ -69,6 +69,8 @@ void FACPreconditioner::setAlgorithmChoice(const std::string& c) {
   if (lower != "default" && lower != "mccormick-s4.3" && lower != "pernice")
     fail("FACPreconditioner: algorithm should be one of 'default', 'mccormick-s4.3' or "
          "'pernice'");
+  if (lower != "default" && d_e_elem != 8)
+    fail("FACPreconditioner: mixed precision runs the default (recursive) cycle only");
   d_algorithm = lower;
   releaseCycleGraph();
   if (d_initialized && d_algorithm == "mccormick-s4.3" && !d_te_data) allocateTmpError();
@@ -95,7 +97,7 @@ void FACPreconditioner::initializeSolverState(const SAMRAIVectorReal& u,
   if (d_lmin < 0 || d_lmin > d_lmax || d_lmax >= h.numberOfLevels())
     fail("FACPreconditioner: bad level range");
   // error clones u, residual and its temporary clone f (FACPreconditioner.C:157-229)
-  d_e_data.reset(new PatchDataSet(h, PatchDataSet::kCell, u.data->ghost()));
+  d_e_data.reset(new PatchDataSet(h, PatchDataSet::kCell, u.data->ghost(), d_e_elem));
   d_r_data.reset(new PatchDataSet(h, PatchDataSet::kCell, f.data->ghost()));
   d_tr_data.reset(new PatchDataSet(h, PatchDataSet::kCell, f.data->ghost()));
   d_e = {d_e_data.get(), u.weights, d_lmin, d_lmax};
@@ -104,6 +106,20 @@ void FACPreconditioner::initializeSolverState(const SAMRAIVectorReal& u,
   if (d_algorithm == "mccormick-s4.3") allocateTmpError();
   d_ops.initializeOperatorState(u, f);
   d_initialized = true;
+}
+
+void FACPreconditioner::setErrorStorage(int elem) {
+  if (elem != 8 && elem != 4 && elem != 2) fail("FACPreconditioner: error storage must be 8, 4 or 2 bytes");
+  if (elem != 8 && d_algorithm != "default")
+    fail("FACPreconditioner: mixed precision runs the default (recursive) cycle only");
+  if (elem == d_e_elem) return;
+  d_e_elem = elem;
+  releaseCycleGraph();
+  if (d_initialized) {
+    d_e_data.reset(new PatchDataSet(d_e_data->hierarchy(), PatchDataSet::kCell,
+                                    d_e_data->ghost(), d_e_elem));
+    d_e.data = d_e_data.get();
+  }
 }
 
 void FACPreconditioner::deallocateSolverState() {
@@ -225,6 +241,7 @@ void FACPreconditioner::enqueueCycle(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
 }
 
 void FACPreconditioner::runOneCycle(SAMRAIVectorReal& u) {
+  if (d_e_data->elemBytes() != 8) d_gpu->prepareMixedSolve();
   d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
   runCycle(u);
 }
@@ -236,6 +253,11 @@ bool FACPreconditioner::solveSystem(SAMRAIVectorReal& u, SAMRAIVectorReal& f) {
     initializeSolverState(u, f);
   }
   auto t0 = std::chrono::steady_clock::now();
+  if (d_e_data->elemBytes() != 8) {
+    if (d_gpu->learnedActive())
+      fail("FACPreconditioner: the learned smoother runs on an FP64 error vector only");
+    d_gpu->prepareMixedSolve();
+  }
   d_gpu->setToScalar(*d_e.data, 0.0, d_lmin, d_lmax);
   if (d_te_data) d_gpu->setToScalar(*d_te.data, 0.0, d_lmin, d_lmax);
   d_gpu->setToScalar(*d_r.data, 0.0, d_lmin, d_lmax);

@@ -6,6 +6,7 @@
 // solveCoarsestLevel :1680-1730, computeVectorWeights :2004-2102); each
 // level operation is one or a few level-batched launches on one stream.
 #include "fac_solver.hpp"
+#include "level_mixed.h"
 
 #include <algorithm>
 #include <array>
@@ -45,9 +46,10 @@ std::vector<std::vector<long>> PatchDataSet::layout(const hier::PatchHierarchy& 
   return off;
 }
 
-PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost)
-    : d_h(h), d_centring(c), d_ghost(ghost) {
+PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost, int elem)
+    : d_h(h), d_centring(c), d_ghost(ghost), d_elem(elem) {
   if (!h.finalized()) fail("PatchDataSet: hierarchy not finalized");
+  if (elem != 8 && elem != 4 && elem != 2) fail("PatchDataSet: element size must be 8, 4 or 2");
   const int dim = h.dim();
   const int ncomp = c == kCell ? 1 : dim;
   d_levels.resize(h.numberOfLevels());
@@ -66,11 +68,15 @@ PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost)
         L.len[a][p] = n;
       }
     L.total = total;
-    hip_check(hipMalloc(&L.slab, sizeof(double) * (total > 0 ? total : 1)), "hipMalloc(data)");
-    hip_check(hipMemset(L.slab, 0, sizeof(double) * (total > 0 ? total : 1)), "hipMemset");
+    const size_t bytes = (size_t)elem * (size_t)(total > 0 ? total : 32);
+    hip_check(hipMalloc(&L.slab, bytes), "hipMalloc(data)");
+    hip_check(hipMemset(L.slab, 0, bytes), "hipMemset");
+    auto at = [&](long off) {
+      return reinterpret_cast<double*>(reinterpret_cast<char*>(L.slab) + (size_t)elem * off);
+    };
     for (int a = 0; a < ncomp; ++a) {
       std::vector<double*> tab(np);
-      for (int p = 0; p < np; ++p) tab[p] = L.slab + L.off[a][p];
+      for (int p = 0; p < np; ++p) tab[p] = at(L.off[a][p]);
       hip_check(hipMalloc(&L.dtab[a], sizeof(double*) * (np > 0 ? np : 1)), "hipMalloc(tab)");
       if (np)
         hip_check(hipMemcpy(L.dtab[a], tab.data(), sizeof(double*) * np, hipMemcpyHostToDevice),
@@ -78,7 +84,7 @@ PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost)
     }
     std::vector<double*> il((size_t)np * ncomp);
     for (int p = 0; p < np; ++p)
-      for (int a = 0; a < ncomp; ++a) il[(size_t)p * ncomp + a] = L.slab + L.off[a][p];
+      for (int a = 0; a < ncomp; ++a) il[(size_t)p * ncomp + a] = at(L.off[a][p]);
     hip_check(hipMalloc(&L.dil, sizeof(double*) * (il.size() + 1)), "hipMalloc(tab)");
     if (!il.empty())
       hip_check(hipMemcpy(L.dil, il.data(), sizeof(double*) * il.size(), hipMemcpyHostToDevice),
@@ -96,7 +102,8 @@ PatchDataSet::~PatchDataSet() {
 }
 
 double* PatchDataSet::pointer(int ln, int p, int a) const {
-  return d_levels[ln].slab + d_levels[ln].off[a][p];
+  return reinterpret_cast<double*>(reinterpret_cast<char*>(d_levels[ln].slab) +
+                                   (size_t)d_elem * d_levels[ln].off[a][p]);
 }
 long PatchDataSet::size(int ln, int p, int a) const { return d_levels[ln].len[a][p]; }
 
@@ -236,6 +243,7 @@ void CellPoissonFACOps::initializeOperatorState(const SAMRAIVectorReal& u,
 void CellPoissonFACOps::deallocateOperatorState() {
   ++d_generation;
   d_flux.reset();
+  d_D_lp.reset();
   for (double* p : d_oflux_slab) (void)hipFree(p);
   for (double** p : d_oflux_tab) (void)hipFree(p);
   for (double* p : d_maxres) (void)hipFree(p);
@@ -703,9 +711,9 @@ sfac::BoxArrays CellPoissonFACOps::gatherBox(PatchDataSet& e, const PatchDataSet
     A.C = B.C;
   }
   sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), B.e[0], B.e[0],
-                         nullptr, 0, G, 1, d_stream, B.fp32);
+                         nullptr, 0, G, 1, d_stream, B.fp32, e.elemBytes());
   sfac::launch_box_cells(T.cells.launch(), T.geom, const_cast<double* const*>(r.table(ln)),
-                         r.ghost(), B.r, B.r, nullptr, 0, G, 1, d_stream, B.fp32);
+                         r.ghost(), B.r, B.r, nullptr, 0, G, 1, d_stream, B.fp32, r.elemBytes());
   A.r = B.r;
   return A;
 }
@@ -873,7 +881,8 @@ void CellPoissonFACOps::solveBoxMG(PatchDataSet& e, const PatchDataSet& r) {
                           d_mg_scal + 2, d_stream);
   }
   sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), d_box.e[0],
-                         d_box.e[0], nullptr, 0, d_box.G, 0, d_stream, d_box.fp32);
+                         d_box.e[0], nullptr, 0, d_box.G, 0, d_stream, d_box.fp32,
+                         e.elemBytes());
   (void)dim;
   if (d_logging) {
     int it = 0;
@@ -966,7 +975,7 @@ int CellPoissonFACOps::smoothBox(PatchDataSet& e, const PatchDataSet& r, int num
   // select the buffer holding the result
   sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), B.e[0], B.e[1],
                          (check && num_sweeps > 0) ? d_stop + 1 : nullptr, num_sweeps, G, 0,
-                         d_stream, B.fp32);
+                         d_stream, B.fp32, e.elemBytes());
   // edges/corners too, as the reference's last NoCoarse fill does
   ghostFill(e, ln, false, true, false);
   if (check && (maxres_out || d_logging)) {
@@ -1005,22 +1014,29 @@ void CellPoissonFACOps::ghostFill(PatchDataSet& x, int ln, bool with_coarse, boo
     if (cfr.method != sfac::kProlongConstant)
       ghostFill(x, ln - 1, ln - 1 > d_ln_min, homogeneous, false);
   }
+  const int elem = x.elemBytes();
+  auto fill = [&](const sfac::GhostLaunch& w) {
+    if (elem != 8)
+      sfac::launch_ghost_items_lp(elem, w, T.geom, reinterpret_cast<void* const*>(x.table(ln)),
+                                  x.ghost(), reinterpret_cast<const void* const*>(coarse),
+                                  x.ghost(), cbox, homogeneous, bc, nullptr, d_stream, nullptr,
+                                  &cfr);
+    else
+      sfac::launch_ghost_items(w, T.geom, x.table(ln), x.ghost(), coarse, x.ghost(), cbox,
+                               homogeneous, bc, nullptr, d_stream, nullptr, &cfr);
+  };
   if (faces_only) {
-    sfac::launch_ghost_items((wc ? T.face_wc : T.face_nc).launch(), T.geom, x.table(ln),
-                             x.ghost(), coarse, x.ghost(), cbox, homogeneous, bc, nullptr,
-                             d_stream, nullptr, &cfr);
+    fill((wc ? T.face_wc : T.face_nc).launch());
     return;
   }
-  for (int ph = 0; ph < 4; ++ph)
-    sfac::launch_ghost_items((wc ? T.full_wc[ph] : T.full_nc[ph]).launch(), T.geom, x.table(ln),
-                             x.ghost(), coarse, x.ghost(), cbox, homogeneous, bc, nullptr,
-                             d_stream, nullptr, &cfr);
+  for (int ph = 0; ph < 4; ++ph) fill((wc ? T.full_wc[ph] : T.full_nc[ph]).launch());
 }
 
 int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet& r, int ln,
                                              int num_sweeps, double tol, double* maxres_out) {
   if (d_box.ok && d_box_enabled && ln == d_box.ln)
     return smoothBox(e, r, num_sweeps, tol, maxres_out);
+  if (e.elemBytes() != 8) return smoothColourLP(e, r, ln, num_sweeps, tol, maxres_out);
   if (tol < 0.0 && fusedLevelActive(ln) && e.ghost() == 1 && e.centring() == PatchDataSet::kCell &&
       r.centring() == PatchDataSet::kCell && r.ghost() == d_fused[ln].rg &&
       &e != d_fused_tmp.get() && &r != d_fused_tmp.get() &&
@@ -1108,7 +1124,137 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
   return num_sweeps;
 }
 
+// ------------------------------------------------------- mixed precision
+
+void CellPoissonFACOps::setMixedPrecision(int elem) {
+  if (elem != 8 && elem != 4 && elem != 2)
+    fail("CellPoissonFACOps: mixed precision storage must be 8, 4 or 2 bytes");
+  if (elem != 8 && d_learned) fail("CellPoissonFACOps: detach the learned smoother first");
+  d_mixed_elem = elem;
+  d_D_lp.reset();
+  ++d_generation;
+}
+
+void CellPoissonFACOps::prepareMixedSolve() {
+  if (d_mixed_elem == 8 || d_spec.dIsConstant()) {
+    d_D_lp.reset();
+    return;
+  }
+  const PatchDataSet& D = *d_spec.getD();
+  if (!d_D_lp || d_D_lp->ghost() != D.ghost() || d_D_lp->elemBytes() != d_mixed_elem) {
+    d_D_lp.reset(new PatchDataSet(d_h, PatchDataSet::kSide, D.ghost(), d_mixed_elem));
+    ++d_generation;
+  }
+  // identical layouts: one conversion per level slab
+  for (int ln = 0; ln < d_h.numberOfLevels(); ++ln)
+    sfac::launch_convert(d_mixed_elem, D.slab(ln), d_D_lp->slab(ln), D.slabSize(ln), d_stream);
+}
+
+sfac::PoissonLPRaw CellPoissonFACOps::poissonLP(int ln) const {
+  sfac::PoissonLPRaw ps{};
+  for (int a = 0; a < 3; ++a) ps.D[a] = nullptr;
+  ps.dghost = 0;
+  ps.dconst = d_spec.getDConstant();
+  if (!d_spec.dIsConstant()) {
+    if (!d_D_lp) fail("CellPoissonFACOps: the low-precision copy of D is missing");
+    for (int a = 0; a < d_h.dim(); ++a)
+      ps.D[a] = reinterpret_cast<const void* const*>(d_D_lp->table(ln, a));
+    ps.dghost = d_D_lp->ghost();
+  }
+  ps.cmode = d_spec.cMode();
+  ps.cconst = d_spec.getCConstant();
+  ps.C = d_spec.cMode() == 2 ? d_spec.getC()->table(ln) : nullptr;
+  ps.cghost = d_spec.cMode() == 2 ? d_spec.getC()->ghost() : 0;
+  return ps;
+}
+
+// smoothErrorByRedBlack's colour passes on a low-precision error vector
+// (the same sequence as the FP64 passes below, FP32 arithmetic).
+int CellPoissonFACOps::smoothColourLP(PatchDataSet& e, const PatchDataSet& r, int ln,
+                                      int num_sweeps, double tol, double* maxres_out) {
+  const auto& T = d_h.tables(ln);
+  if (e.elemBytes() != d_mixed_elem)
+    fail("CellPoissonFACOps: the error vector's precision differs from the mixed-precision "
+         "setting");
+  // a direct call (outside a solve) makes the low-precision copy of D here
+  if (!d_spec.dIsConstant() && !d_D_lp) prepareMixedSolve();
+  const sfac::PoissonLPRaw ps = poissonLP(ln);
+  const int ewing = (d_ewing && ln > d_ln_min) ? 1 : 0;
+  const int np = T.geom.npatch;
+  const int elem = e.elemBytes();
+  void* const* et = reinterpret_cast<void* const*>(e.table(ln));
+  const sfac::RobinDev bc = robinDev();
+  ghostFill(e, ln, false, true, false);
+  if (ln > d_ln_min) ghostFill(e, ln, true, true, false);
+  const int nsc = T.smooth.nchunks;
+  double* red = tol >= 0.0 ? d_maxres[ln] : nullptr;
+  double* blk = tol >= 0.0 ? d_maxres[ln] + nsc : nullptr;
+  const int* stop = tol >= 0.0 ? d_stop : nullptr;
+  sfac::SweepCheck chk{};
+  if (tol >= 0.0) {
+    sfac::launch_fill_int(d_stop, 2, 0, d_stream);
+    chk.red = red;
+    chk.blk = blk;
+    chk.c0 = np > 0 ? T.smooth.first_chunk[np - 1] : 0;
+    chk.c1 = nsc;
+    chk.tol = tol;
+    chk.stop = d_stop;
+    chk.sweeps = d_stop + 1;
+    chk.maxres_out = d_scalars + 1;
+  }
+  const double launch_bytes =
+      (d_spec.dIsConstant() ? 12.0 : 24.0) * (double)elem / 8.0 * (double)T.ncells;
+  auto colour_pass = [&](int colour, double* mr) {
+    TimedLaunch tl{};
+    if (d_timing) {
+      tl.start = nextEvent();
+      hip_check(hipEventRecord(tl.start, d_stream), "hipEventRecord");
+    }
+    sfac::launch_rbgs_lp(elem, T.smooth.smooth(), T.geom, et, e.ghost(), r.table(ln), r.ghost(),
+                         ps, ewing ? 2 : 0, colour, mr, stop, d_stream);
+    if (d_timing) {
+      tl.stop = nextEvent();
+      hip_check(hipEventRecord(tl.stop, d_stream), "hipEventRecord");
+      tl.level = ln;
+      tl.count = 1;
+      tl.bytes = launch_bytes;
+      d_timed.push_back(tl);
+    }
+  };
+  for (int isweep = 0; isweep < num_sweeps; ++isweep) {
+    if (ewing)
+      sfac::launch_ewing_faces_lp(elem, T.cf_faces.launch(), T.geom, et, e.ghost(), ps, stop,
+                                  d_stream);
+    colour_pass(0, red);
+    sfac::launch_ghost_items_lp(elem, T.face_nc.launch(), T.geom, et, e.ghost(), nullptr, 0,
+                                nullptr, 1, bc, stop, d_stream);
+    if (ewing)
+      sfac::launch_ewing_faces_lp(elem, T.cf_faces.launch(), T.geom, et, e.ghost(), ps, stop,
+                                  d_stream);
+    colour_pass(1, blk);
+    sfac::launch_ghost_items_lp(elem, T.face_nc.launch(), T.geom, et, e.ghost(), nullptr, 0,
+                                nullptr, 1, bc, stop, d_stream,
+                                (tol >= 0.0 && np > 0) ? &chk : nullptr);
+  }
+  ghostFill(e, ln, false, true, false);
+  if (tol >= 0.0 && (maxres_out || d_logging)) {
+    int sweeps = 0;
+    double mr = 0.0;
+    hip_check(hipMemcpyAsync(&sweeps, d_stop + 1, sizeof(int), hipMemcpyDeviceToHost, d_stream),
+              "hipMemcpyAsync");
+    hip_check(hipMemcpyAsync(&mr, d_scalars + 1, sizeof(double), hipMemcpyDeviceToHost,
+                             d_stream),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(d_stream), "hipStreamSynchronize");
+    if (maxres_out) *maxres_out = mr;
+    return sweeps;
+  }
+  return num_sweeps;
+}
+
 void CellPoissonFACOps::setLearnedSmoother(void* net, double omega) {
+  if (net && d_mixed_elem != 8)
+    fail("learned smoother: not available with the mixed-precision error vector");
   if (net && !(omega > 0.0 && std::isfinite(omega)))
     fail("learned smoother: omega must be positive and finite");
   if (net && net != d_learned && learned_attach(net, 1) != 0)
@@ -1237,10 +1383,17 @@ void CellPoissonFACOps::computeCompositeResidualOnLevel(SAMRAIVectorReal& r,
   } else {
     G.fmask = nullptr;  // the finer levels of the hierarchy are not part of this solve
   }
-  sfac::launch_residual_fused(T.cells.launch(), G, r.data->table(ln), r.data->ghost(),
-                              f.data->table(ln), f.data->ghost(), u.data->table(ln),
-                              u.data->ghost(), d_flux->interleaved(ln), ps, ewing,
-                              ln > d_ln_min ? d_oflux_tab[ln] : nullptr, d_stream);
+  if (u.data->elemBytes() != 8)
+    sfac::launch_residual_lp(u.data->elemBytes(), T.cells.launch(), G, r.data->table(ln),
+                             r.data->ghost(), f.data->table(ln), f.data->ghost(),
+                             reinterpret_cast<const void* const*>(u.data->table(ln)),
+                             u.data->ghost(), d_flux->interleaved(ln), ps, ewing,
+                             ln > d_ln_min ? d_oflux_tab[ln] : nullptr, d_stream);
+  else
+    sfac::launch_residual_fused(T.cells.launch(), G, r.data->table(ln), r.data->ghost(),
+                                f.data->table(ln), f.data->ghost(), u.data->table(ln),
+                                u.data->ghost(), d_flux->interleaved(ln), ps, ewing,
+                                ln > d_ln_min ? d_oflux_tab[ln] : nullptr, d_stream);
 }
 
 void CellPoissonFACOps::restrictSolution(const SAMRAIVectorReal& s, SAMRAIVectorReal& d,
@@ -1267,12 +1420,21 @@ void CellPoissonFACOps::prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAI
   // coarse values outside the coarse patch come from its same-level and
   // homogeneous physical ghosts, as RefineSchedule's coarse fill sets them
   ghostFill(*s.data, dest_ln - 1, false, true, false);
-  sfac::launch_prolong_add(Tf.prolong_from_coarser.launch(), Tc.geom, Tf.geom,
-                           s.data->table(dest_ln - 1), s.data->ghost(), d.data->table(dest_ln),
-                           d.data->ghost(), d_prolongation == "CONSTANT_REFINE" ? sfac::kProlongConstant
-                           : d_prolongation == "LINEAR_REFINE" ? sfac::kProlongLinear
-                                                               : sfac::kProlongConservative,
-                           d_stream);
+  const int method = d_prolongation == "CONSTANT_REFINE" ? sfac::kProlongConstant
+                     : d_prolongation == "LINEAR_REFINE" ? sfac::kProlongLinear
+                                                         : sfac::kProlongConservative;
+  if (s.data->elemBytes() != d.data->elemBytes())
+    fail("CellPoissonFACOps: prolongation between vectors of different precision");
+  if (d.data->elemBytes() != 8)
+    sfac::launch_prolong_add_lp(
+        d.data->elemBytes(), Tf.prolong_from_coarser.launch(), Tc.geom, Tf.geom,
+        reinterpret_cast<const void* const*>(s.data->table(dest_ln - 1)), s.data->ghost(),
+        reinterpret_cast<void* const*>(d.data->table(dest_ln)), d.data->ghost(), method,
+        d_stream);
+  else
+    sfac::launch_prolong_add(Tf.prolong_from_coarser.launch(), Tc.geom, Tf.geom,
+                             s.data->table(dest_ln - 1), s.data->ghost(), d.data->table(dest_ln),
+                             d.data->ghost(), method, d_stream);
 }
 
 double CellPoissonFACOps::rmsNorm(const PatchDataSet& x, const PatchDataSet& w, int coarse_ln,
@@ -1321,8 +1483,12 @@ void CellPoissonFACOps::computeVectorWeights(PatchDataSet& w, int coarsest_ln, i
 }
 
 void CellPoissonFACOps::setToScalar(PatchDataSet& x, double v, int coarse_ln, int fine_ln) {
+  if (x.elemBytes() != 8 && v != 0.0)
+    fail("CellPoissonFACOps: low-precision vectors are only set to 0");
+  // a low-precision slab is zeroed as doubles (slabs hold a multiple of 32
+  // values, so its byte size is a multiple of 8)
   for (int ln = coarse_ln; ln <= fine_ln; ++ln)
-    sfac::launch_fill(x.slab(ln), x.slabSize(ln), v, d_stream);
+    sfac::launch_fill(x.slab(ln), x.slabSize(ln) * x.elemBytes() / 8, v, d_stream);
 }
 
 void CellPoissonFACOps::setToScalarInterior(PatchDataSet& x, double v, int ln) {
@@ -1331,10 +1497,16 @@ void CellPoissonFACOps::setToScalarInterior(PatchDataSet& x, double v, int ln) {
 }
 
 void CellPoissonFACOps::add(PatchDataSet& y, const PatchDataSet& x, int coarse_ln, int fine_ln) {
+  if (y.elemBytes() != 8) fail("CellPoissonFACOps: add into a low-precision vector");
   for (int ln = coarse_ln; ln <= fine_ln; ++ln) {
     const auto& T = d_h.tables(ln);
-    sfac::launch_axpy(T.cells.launch(), T.geom, y.table(ln), y.ghost(), x.table(ln), x.ghost(),
-                      d_stream);
+    if (x.elemBytes() != 8)
+      sfac::launch_axpy_lp(x.elemBytes(), T.cells.launch(), T.geom, y.table(ln), y.ghost(),
+                           reinterpret_cast<const void* const*>(x.table(ln)), x.ghost(),
+                           d_stream);
+    else
+      sfac::launch_axpy(T.cells.launch(), T.geom, y.table(ln), y.ghost(), x.table(ln), x.ghost(),
+                        d_stream);
   }
 }
 

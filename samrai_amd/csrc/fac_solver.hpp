@@ -19,6 +19,7 @@
 #include "box_kernels.h"
 #include "hierarchy.hpp"
 #include "level_fused.h"
+#include "level_mixed.h"
 
 namespace samrai_amd {
 namespace pdat {
@@ -28,13 +29,17 @@ namespace pdat {
 class PatchDataSet {
  public:
   enum Centring { kCell = 0, kSide = 1 };
-  PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost);
+  // elem: bytes per value, 8 (double, every user-visible patch data) or, for
+  // the solver's mixed-precision error vector, 4 (float) or 2 (bfloat16); the
+  // pointer tables are then typed double* only nominally.
+  PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost, int elem = 8);
   ~PatchDataSet();
   PatchDataSet(const PatchDataSet&) = delete;
   PatchDataSet& operator=(const PatchDataSet&) = delete;
 
   Centring centring() const { return d_centring; }
   int ghost() const { return d_ghost; }
+  int elemBytes() const { return d_elem; }
   int numberOfLevels() const { return (int)d_levels.size(); }
   const hier::PatchHierarchy& hierarchy() const { return d_h; }
   // ArrayData::getPointer for patch p of level ln (component/axis a).
@@ -63,6 +68,7 @@ class PatchDataSet {
   const hier::PatchHierarchy& d_h;
   Centring d_centring;
   int d_ghost;
+  int d_elem;
   std::vector<Lvl> d_levels;
 };
 
@@ -222,6 +228,16 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   // (the residuals the outer loop tests stay FP64), and the multigrid
   // tolerance is floored at kCoarseFP32Tol.
   void setCoarsePrecision(int bits);
+  // Mixed precision on every level (no reference counterpart): elem = 4
+  // (FP32) or 2 (BF16) makes the preconditioner's error vector low-precision
+  // storage: smoothing, the error's ghost fills and prolongation run in FP32
+  // on it, with a low-precision copy of D; the composite residuals, the
+  // coarse-fine flux corrections, restriction and u += e stay FP64.  8
+  // (default): the reference arithmetic.
+  void setMixedPrecision(int elem);
+  int mixedElem() const { return d_mixed_elem; }
+  // At the start of a mixed-precision solve: D into its low-precision copy.
+  void prepareMixedSolve();
   // Learned block correction (learned_kernels.hip) before the red-black
   // sweeps of smoothError on every level above the coarsest whose patches
   // are all 16^3 with cubic cells: the error equation's residual on the level
@@ -337,6 +353,11 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   void releaseFusedLevels();
   int tuneFusedVariant(int ln);
   int smoothFused(pdat::PatchDataSet& e, const pdat::PatchDataSet& r, int ln, int num_sweeps);
+  int d_mixed_elem = 8;
+  std::unique_ptr<pdat::PatchDataSet> d_D_lp;
+  sfac::PoissonLPRaw poissonLP(int ln) const;
+  int smoothColourLP(pdat::PatchDataSet& e, const pdat::PatchDataSet& r, int ln, int num_sweeps,
+                     double tol, double* maxres_out);
   void setupBoxPath();
   void releaseBoxPath();
   int smoothBox(pdat::PatchDataSet& e, const pdat::PatchDataSet& r, int num_sweeps, double tol,
@@ -376,6 +397,9 @@ class FACPreconditioner {
   void setPostsmoothingSweeps(int n) { d_post = n; releaseCycleGraph(); }
   void setAlgorithmChoice(const std::string& c);
   const std::string& algorithmChoice() const { return d_algorithm; }
+  // Bytes per value of the error vector (8, or 4 / 2 in mixed precision;
+  // the recursive cycle only).  Re-creates it when initialized.
+  void setErrorStorage(int elem);
 
   void initializeSolverState(const SAMRAIVectorReal& u, const SAMRAIVectorReal& f);
   void deallocateSolverState();
@@ -422,6 +446,7 @@ class FACPreconditioner {
   void restoreSolution(SAMRAIVectorReal& u);
   FACOperatorStrategy& d_ops;
   CellPoissonFACOps* d_gpu;
+  int d_e_elem = 8;
   int d_max_cycles = 10;
   double d_residual_tol = 1e-6;
   double d_rel_tol = -1.0;

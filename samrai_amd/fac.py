@@ -87,6 +87,7 @@ def lib():
     L.sfac_solver_set_graphs.argtypes = [_vp, C.c_int]
     L.sfac_solver_set_algorithm_choice.argtypes = [_vp, C.c_char_p]
     L.sfac_solver_set_coarse_precision.argtypes = [_vp, C.c_int]
+    L.sfac_solver_set_mixed_precision.argtypes = [_vp, C.c_int]
     L.sfac_berger_rigoutsos.argtypes = [C.c_int, _ip, _ip, C.c_void_p, C.c_double, C.c_double,
                                         _ip, _ip, _ip, C.c_int, _ip]
     L.sfac_fused_tile_variants.argtypes = [C.c_int]
@@ -350,6 +351,12 @@ class CellPoissonFACSolver:
         """64 (default) or 32: precision of the coarsest-level solve of the
         error equation inside the FP64 FAC iteration."""
         _check(lib().sfac_solver_set_coarse_precision(self.s, int(bits)))
+
+    def set_mixed_precision(self, bits):
+        """64 (default), 32 or 16: storage of the FAC iteration's error vector
+        (FP64, FP32 or BF16); smoothing, the error's ghost fills and the
+        prolongation then compute in FP32, residuals and u += e in FP64."""
+        _check(lib().sfac_solver_set_mixed_precision(self.s, int(bits)))
 
     def set_graphs(self, on=True):
         """Capture each FAC cycle as a HIP graph and replay it (same results)."""
