@@ -399,6 +399,9 @@ def main():
                     help="deck of --mode profile / smoother")
     ap.add_argument("--coarse", choices=["redblack", "mg"], default="mg",
                     help="coarsest-level solver of --mode profile")
+    ap.add_argument("--mixed", type=int, choices=[64, 32, 16], default=64,
+                    help="--mode profile: storage bits of the error vector on every level "
+                         "(with 32 / 16 the coarsest-level solve runs in FP32)")
     ap.add_argument("--no-fused-levels", action="store_true",
                     help="--mode profile / smoother: patch-wise colour passes on the levels "
                          "above the coarsest instead of the colour-fused sweeps")
@@ -432,6 +435,9 @@ def main():
         deck, h, u, f, D, s = problem(args.deck, MG_PARAMS if args.coarse == "mg" else PARAMS)
         if args.no_fused_levels:
             s.set_fused_levels(False)
+        if args.mixed != 64:
+            s.set_coarse_precision(32)
+            s.set_mixed_precision(args.mixed)
         for k in range(args.warmup + args.steps):
             u.fill(0.0)
             t0 = time.perf_counter()

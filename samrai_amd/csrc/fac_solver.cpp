@@ -1049,8 +1049,11 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
   const int np = T.geom.npatch;
   sfac::RobinDev bc = robinDev();
   // full fills (edges and corners included), as xeqScheduleGhostFill[NoCoarse]
-  ghostFill(e, ln, false, true, false);
+  // (CellPoissonFACOps.C:1364-1373); above the coarsest level the fill with
+  // the coarser level rewrites every ghost the fill without it sets, with
+  // the same values, so it alone is run
   if (ln > d_ln_min) ghostFill(e, ln, true, true, false);
+  else ghostFill(e, ln, false, true, false);
   const int nsc = T.smooth.nchunks;
   double* red = tol >= 0.0 ? d_maxres[ln] : nullptr;
   double* blk = tol >= 0.0 ? d_maxres[ln] + nsc : nullptr;
@@ -1101,10 +1104,13 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
       sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, e.table(ln), e.ghost(), ps, stop,
                                d_stream);
     colour_pass(1, blk);
-    // the fill after black also runs the convergence test (work-group 0)
-    sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
-                             nullptr, 1, bc, stop, d_stream,
-                             (tol >= 0.0 && np > 0) ? &chk : nullptr);
+    // the fill after black also runs the convergence test (work-group 0);
+    // after the last sweep without one, the closing full fill below sets
+    // the same face ghosts
+    if (tol >= 0.0 || isweep + 1 < num_sweeps)
+      sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
+                               nullptr, 1, bc, stop, d_stream,
+                               (tol >= 0.0 && np > 0) ? &chk : nullptr);
   }
   // edges/corners too, as the reference's last NoCoarse fill does
   ghostFill(e, ln, false, true, false);
@@ -1184,8 +1190,8 @@ int CellPoissonFACOps::smoothColourLP(PatchDataSet& e, const PatchDataSet& r, in
   const int elem = e.elemBytes();
   void* const* et = reinterpret_cast<void* const*>(e.table(ln));
   const sfac::RobinDev bc = robinDev();
-  ghostFill(e, ln, false, true, false);
   if (ln > d_ln_min) ghostFill(e, ln, true, true, false);
+  else ghostFill(e, ln, false, true, false);
   const int nsc = T.smooth.nchunks;
   double* red = tol >= 0.0 ? d_maxres[ln] : nullptr;
   double* blk = tol >= 0.0 ? d_maxres[ln] + nsc : nullptr;
@@ -1232,9 +1238,10 @@ int CellPoissonFACOps::smoothColourLP(PatchDataSet& e, const PatchDataSet& r, in
       sfac::launch_ewing_faces_lp(elem, T.cf_faces.launch(), T.geom, et, e.ghost(), ps, stop,
                                   d_stream);
     colour_pass(1, blk);
-    sfac::launch_ghost_items_lp(elem, T.face_nc.launch(), T.geom, et, e.ghost(), nullptr, 0,
-                                nullptr, 1, bc, stop, d_stream,
-                                (tol >= 0.0 && np > 0) ? &chk : nullptr);
+    if (tol >= 0.0 || isweep + 1 < num_sweeps)
+      sfac::launch_ghost_items_lp(elem, T.face_nc.launch(), T.geom, et, e.ghost(), nullptr, 0,
+                                  nullptr, 1, bc, stop, d_stream,
+                                  (tol >= 0.0 && np > 0) ? &chk : nullptr);
   }
   ghostFill(e, ln, false, true, false);
   if (tol >= 0.0 && (maxres_out || d_logging)) {
