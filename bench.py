@@ -462,9 +462,14 @@ def main():
         s.smooth(e, r, 0, 10)
         for ln in range(1, nl):
             s.smooth(e, r, ln, 2)
+        # the composite residual of the error equation on every level, twice
+        w = s.internal_vector("w")
+        for _ in range(2):
+            for ln in range(nl - 1, -1, -1):
+                s.residual(w, e, r, ln, True)
         s.synchronize()
         log("bench: smoother mode done (copy 256 MiB, 10 coarsest sweeps, 2 sweeps per finer "
-            "level)")
+            "level, 2 composite residuals per level)")
         return
 
     # ---------------- the headline: C2, plain FP64, multigrid coarse solve

@@ -1390,17 +1390,15 @@ void CellPoissonFACOps::computeCompositeResidualOnLevel(SAMRAIVectorReal& r,
   } else {
     G.fmask = nullptr;  // the finer levels of the hierarchy are not part of this solve
   }
-  if (u.data->elemBytes() != 8)
-    sfac::launch_residual_lp(u.data->elemBytes(), T.cells.launch(), G, r.data->table(ln),
-                             r.data->ghost(), f.data->table(ln), f.data->ghost(),
-                             reinterpret_cast<const void* const*>(u.data->table(ln)),
-                             u.data->ghost(), d_flux->interleaved(ln), ps, ewing,
-                             ln > d_ln_min ? d_oflux_tab[ln] : nullptr, d_stream);
-  else
-    sfac::launch_residual_fused(T.cells.launch(), G, r.data->table(ln), r.data->ghost(),
-                                f.data->table(ln), f.data->ghost(), u.data->table(ln),
-                                u.data->ghost(), d_flux->interleaved(ln), ps, ewing,
-                                ln > d_ln_min ? d_oflux_tab[ln] : nullptr, d_stream);
+  if (ewing)
+    sfac::launch_residual_faces(u.data->elemBytes(), T.cf_faces.launch(), G,
+                                reinterpret_cast<const void* const*>(u.data->table(ln)),
+                                u.data->ghost(), ps, d_stream);
+  sfac::launch_residual(u.data->elemBytes(), T.rows.smooth(), G, r.data->table(ln),
+                        r.data->ghost(), f.data->table(ln), f.data->ghost(),
+                        reinterpret_cast<const void* const*>(u.data->table(ln)), u.data->ghost(),
+                        d_flux->interleaved(ln), ps, ewing,
+                        ln > d_ln_min ? d_oflux_tab[ln] : nullptr, d_stream);
 }
 
 void CellPoissonFACOps::restrictSolution(const SAMRAIVectorReal& s, SAMRAIVectorReal& d,

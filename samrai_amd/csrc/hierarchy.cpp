@@ -180,20 +180,21 @@ T* PatchHierarchy::upload(const std::vector<T>& v) {
   return p;
 }
 
-DevList PatchHierarchy::uploadList(const std::vector<BoxItem>& items, bool colour_space) {
+DevList PatchHierarchy::uploadList(const std::vector<BoxItem>& items, int space) {
   DevList L;
   std::vector<Chunk> chunks;
   for (size_t i = 0; i < items.size(); ++i) {
     const BoxItem& it = items[i];
     L.first_chunk.push_back((int)chunks.size());
     long n;
-    if (colour_space) {
-      // chunks of whole rows (j,k): a row's colour cells sit on consecutive
-      // lanes; rows per chunk as the colour kernel computes it
+    if (space != kCellSpace) {
+      // chunks of whole rows (j,k): a row's cells (colour cells) sit on
+      // consecutive lanes; rows per chunk as the kernels compute it
       int nx = it.hi[0] - it.lo[0] + 1;
       int half = (nx + 1) / 2;
       if (half > sfac::kThreads) fail("PatchHierarchy: patch wider than 512 cells in x");
-      int rpc = std::max(1, sfac::kColourCellsPerChunk / half);
+      int rpc = space == kColourRows ? std::max(1, sfac::kColourCellsPerChunk / half)
+                                     : std::max(1, sfac::kRowCellsPerChunk / nx);
       long rows = (long)(it.hi[1] - it.lo[1] + 1) * (it.hi[2] - it.lo[2] + 1);
       for (long s = 0; s < rows; s += rpc) chunks.push_back(Chunk{(int)i, (int)s});
       continue;
@@ -397,7 +398,8 @@ void PatchHierarchy::buildLevel(int ln) {
     T.ncells += boxsize(L.boxes[p], dim);
   }
   T.cells = uploadList(cells);
-  T.smooth = uploadList(smooth, true);
+  T.smooth = uploadList(smooth, kColourRows);
+  T.rows = uploadList(smooth, kCellRows);
 
   // neighbour candidates on this level
   std::vector<std::vector<int>> nbr(np);

@@ -71,6 +71,7 @@ class PatchHierarchy {
     sfac::LevelGeom geom{};
     DevList cells;            // interior of every patch
     DevList smooth;           // colour half-rows of every patch
+    DevList rows;             // whole rows of every patch, ~kRowCellsPerChunk cells per chunk
     DevList face_nc, face_wc; // face ghosts: no coarse / with coarse
     DevList full_nc[4], full_wc[4];  // all ghosts by phase
     DevList restrict_to_coarser;     // p coarse, q fine, coarse box
@@ -92,7 +93,11 @@ class PatchHierarchy {
   bool d_final = false;
   template <class T>
   T* upload(const std::vector<T>& v);
-  DevList uploadList(const std::vector<BoxItem>& items, bool colour_space = false);
+  // chunk spaces: kCellSpace kCellsPerChunk cells of the item's box;
+  // kColourRows whole rows, about kColourCellsPerChunk cells (both colours);
+  // kCellRows whole rows, about kRowCellsPerChunk cells (one row if wider)
+  enum { kCellSpace = 0, kColourRows = 1, kCellRows = 2 };
+  DevList uploadList(const std::vector<BoxItem>& items, int space = kCellSpace);
 };
 
 // Tiles of shape tile[3] over every patch of level L (3-D, patch-major,

@@ -16,6 +16,9 @@ constexpr int kCellsPerChunk = 256; // cells a work-group handles per chunk
 // colour-space chunks of the red-black colour pass: whole rows, about this
 // many cells (both colours) per chunk
 constexpr int kColourCellsPerChunk = 512;
+// whole-row chunks of the one-pass level kernels (composite residual): about
+// this many cells per chunk, a few per lane
+constexpr int kRowCellsPerChunk = 1024;
 
 // Chunk tables are padded with item -1 to a multiple of 8 entries and
 // launched with one work-group per entry in an XCD-aware order: work-groups
@@ -26,6 +29,31 @@ inline int xcd_grid(int nchunks) { return 8 * ((nchunks + 7) / 8); }
 __device__ inline int xcd_chunk() {
   const int per = (int)(gridDim.x >> 3);
   return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+}
+
+// Global-memory view of a pointer read from a per-patch pointer table.  The
+// compiler cannot infer the address space of such a pointer, so accesses
+// through it would compile to flat instructions (64-bit VGPR addresses,
+// counted against both the vector-memory and LDS wait counters); through
+// this view they are global loads and stores.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ inline gptr<T> gmem(T* p) {
+  return (gptr<T>)p;
+}
+// Value i of a global array of FP64, FP32 or BF16 storage, as FP64.
+template <class I>
+__device__ inline double ld64(gptr<const double> p, I i) {
+  return p[i];
+}
+template <class I>
+__device__ inline double ld64(gptr<const float> p, I i) {
+  return (double)p[i];
+}
+template <class I>
+__device__ inline double ld64(gptr<const bf16> p, I i) {
+  return (double)bf16_to_float(bf16{p[i].bits});
 }
 
 // One rectangular piece of work: destination patch p, source patch q,

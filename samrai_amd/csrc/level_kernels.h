@@ -57,6 +57,11 @@ void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, in
 // (items: p patch, q face, box = interior layer next to the face).
 void launch_ewing_faces(const GhostLaunch& w, const LevelGeom& L, const double* const* u, int ug,
                         const PoissonDev& ps, const int* stop, hipStream_t s);
+// The flux of every face of the cf_faces layers (Ewing-corrected where the
+// face is coarse-fine, else plain; FP64 from u in elem-byte storage) into
+// L.cf_flux, for launch_residual with ewing.
+void launch_residual_faces(int elem, const GhostLaunch& w, const LevelGeom& L,
+                           const void* const* u, int ug, const PoissonDev& ps, hipStream_t s);
 
 // Coarse faces under the finer level's patch boundaries take the
 // area-weighted mean of the saved fine flux (outerside) values.
@@ -67,10 +72,13 @@ void launch_flux_coarsen(const GhostLaunch& w, const LevelGeom& coarse, const Le
 // the Ewing fix when ewing), except faces marked in L.fmask, whose flux is
 // read from flux (written there by launch_flux_coarsen): r = f - div F - C u
 // on the interior, and the patch boundary fluxes into oflux (if not null).
-void launch_residual_fused(const GhostLaunch& cells, const LevelGeom& L, double* const* r,
-                           int rg, const double* const* f, int fg, const double* const* u,
-                           int ug, const double* const* flux, const PoissonDev& ps, int ewing,
-                           double* const* oflux, hipStream_t s);
+// rows: the level's whole-row chunk table (about kRowCellsPerChunk cells a
+// chunk).
+// u holds elem-byte values (8 FP64, 4 FP32, 2 BF16); the arithmetic is FP64.
+void launch_residual(int elem, const SmoothLaunch& rows, const LevelGeom& L, double* const* r,
+                     int rg, const double* const* f, int fg, const void* const* u, int ug,
+                     const double* const* flux, const PoissonDev& ps, int ewing,
+                     double* const* oflux, hipStream_t s);
 
 // Cell weighted average fine -> coarse (items: p coarse, q fine, coarse box).
 void launch_restrict(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,

@@ -94,61 +94,44 @@ __device__ inline long cf_index(const LevelGeom& L, int p, int face, const int* 
 
 // Flux D grad u on the face of axis a with face index c (between c-e_a and
 // c), with the Ewing coarse-fine correction on flagged patch-boundary faces
-// (solv_cellpoissonfacops3d.f:10-74 and 604-1011).  ewing: 0 none, 1 compute
-// the correction, 2 read the corrected flux from L.cf_flux.
-template <int DIM, bool DVAR>
-__device__ inline double face_flux(const LevelGeom& L, int p, const Box& b, const double* u,
-                                   const View& vu, const double* const* Dp, const View* vd,
-                                   double dconst, int a, int i, int j, int k, double dai,
-                                   int ewing) {
+// (solv_cellpoissonfacops3d.f:10-74 and 604-1011), in FP64 from u in
+// storage FP64, FP32 or BF16 (P a global pointer to it).
+template <int DIM, bool DVAR, class P>
+__device__ inline double face_flux(const LevelGeom& L, int p, const Box& b, P u, const View& vu,
+                                   const double* const* Dp, const View* vd, double dconst, int a,
+                                   int i, int j, int k, double dai) {
   int c[3] = {i, j, k};
   int m[3] = {i, j, k};
   m[a] -= 1;
-  if (ewing == 2) {
-    int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
-    if (side >= 0) {
-      int in[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) in[d] = side == 0 ? c[d] : m[d];
-      long x = cf_index(L, p, 2 * a + side, in, b);
-      if (x >= 0) return L.cf_flux[x];
-    }
-  }
   double D = DVAR ? Dp[a][vd[a].at(i, j, k)] : dconst;
-  double F = dai * D * (u[vu.at(i, j, k)] - u[vu.at(m[0], m[1], m[2])]);
-  if (ewing == 1) {
-    int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
-    if (side >= 0) {
-      // the interior cell next to the face, by value (a pointer chosen at run
-      // time between two local arrays would force them into scratch memory)
-      int in[3];
+  double F = dai * D * (ld64(u, vu.at(i, j, k)) - ld64(u, vu.at(m[0], m[1], m[2])));
+  int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
+  if (side >= 0) {
+    // the interior cell next to the face, by value (a pointer chosen at run
+    // time between two local arrays would force them into scratch memory)
+    int in[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) in[d] = side == 0 ? c[d] : m[d];
-      if (cf_flag(L, p, 2 * a + side, in, b)) {
-        int r = L.ratio;
-        int q[3] = {in[0], in[1], in[2]};
-        int t1, t2;
-        tangential(a, DIM, t1, t2);
-        int bs1 = in[t1] - (in[t1] - cdiv(in[t1], r) * r);
-        q[t1] = bs1 + (r - 1 - (in[t1] - bs1));
-        if (DIM == 3) {
-          int bs2 = in[t2] - (in[t2] - cdiv(in[t2], r) * r);
-          q[t2] = bs2 + (r - 1 - (in[t2] - bs2));
-        }
-        double tranwt = ewing_tranwt(r);
-        double longwt = 2 * tranwt;
-        double h = L.dx[a];
-        double tv = tranwt * D * (u[vu.at(q[0], q[1], q[2])] - u[vu.at(in[0], in[1], in[2])]) / h;
-        F = side == 0 ? longwt * F + tv : longwt * F - tv;
+    for (int d = 0; d < 3; ++d) in[d] = side == 0 ? c[d] : m[d];
+    if (cf_flag(L, p, 2 * a + side, in, b)) {
+      int r = L.ratio;
+      int q[3] = {in[0], in[1], in[2]};
+      int t1, t2;
+      tangential(a, DIM, t1, t2);
+      int bs1 = in[t1] - (in[t1] - cdiv(in[t1], r) * r);
+      q[t1] = bs1 + (r - 1 - (in[t1] - bs1));
+      if (DIM == 3) {
+        int bs2 = in[t2] - (in[t2] - cdiv(in[t2], r) * r);
+        q[t2] = bs2 + (r - 1 - (in[t2] - bs2));
       }
+      double tranwt = ewing_tranwt(r);
+      double longwt = 2 * tranwt;
+      double h = L.dx[a];
+      double tv = tranwt * D *
+                  (ld64(u, vu.at(q[0], q[1], q[2])) - ld64(u, vu.at(in[0], in[1], in[2]))) / h;
+      F = side == 0 ? longwt * F + tv : longwt * F - tv;
     }
   }
   return F;
-}
-
-__device__ inline void atomic_max_nonneg(double* p, double v) {
-  atomicMax(reinterpret_cast<unsigned long long*>(p),
-            (unsigned long long)__double_as_longlong(v));
 }
 
 // Coarse-solve convergence test of smoothErrorByRedBlack
@@ -408,11 +391,14 @@ k_rbgs(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* e,
 // colour update (CellPoissonFACOps.C:1392-1428); the corrected flux reads the
 // tangentially mirrored interior cell, which in 3-D has the same colour, so
 // it must be taken from the pre-pass state.  items: p patch, q face
-// (2*axis+side), box = the interior cell layer next to that face.
-template <int DIM, bool DVAR>
+// (2*axis+side), box = the interior cell layer next to that face.  all: the
+// flux of every face of the layer (the plain flux where the face is not
+// coarse-fine), for the composite residual, which then reads each boundary
+// face of such a layer from the snapshot without testing its flag.
+template <int DIM, bool DVAR, class T>
 __global__ void __launch_bounds__(kThreads)
-k_ewing_faces(const BoxItem* items, const Chunk* chunks, LevelGeom L, const double* const* uu,
-              int ug, PoissonDev ps, const int* stop) {
+k_ewing_faces(const BoxItem* items, const Chunk* chunks, LevelGeom L, const T* const* uu, int ug,
+              PoissonDev ps, const int* stop, int all) {
   if (stop && *stop) return;
   const int ci = xcd_chunk();
   const Chunk ch = chunks[ci];
@@ -430,24 +416,30 @@ k_ewing_faces(const BoxItem* items, const Chunk* chunks, LevelGeom L, const doub
       Dp[d] = ps.D[d][p];
     }
   }
+  const auto u = gmem(uu[p]);
   const double dai = 1.0 / (a == 0 ? L.dx[0] : (a == 1 ? L.dx[1] : L.dx[2]));
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
     int c[3];
     if (!decode_box(it, (long)ch.start + t, c)) break;
-    long x = cf_index(L, p, face, c, b);
-    if (x < 0) continue;
+    long x;
+    if (all) {
+      x = L.cf_off[p * 6 + face] + cf_pos(L, face, c, b);
+    } else {
+      x = cf_index(L, p, face, c, b);
+      if (x < 0) continue;
+    }
     // one call per axis, each with a constant axis (a run-time axis index
     // into the local arrays would put them in scratch memory)
     double F;
     if (a == 0)
-      F = face_flux<DIM, DVAR>(L, p, b, uu[p], vu, Dp, vd, ps.dconst, 0, c[0] + side, c[1],
-                               c[2], dai, 1);
+      F = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 0, c[0] + side, c[1], c[2],
+                               dai);
     else if (a == 1)
-      F = face_flux<DIM, DVAR>(L, p, b, uu[p], vu, Dp, vd, ps.dconst, 1, c[0], c[1] + side,
-                               c[2], dai, 1);
+      F = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 1, c[0], c[1] + side, c[2],
+                               dai);
     else
-      F = face_flux<DIM, DVAR>(L, p, b, uu[p], vu, Dp, vd, ps.dconst, 2, c[0], c[1],
-                               c[2] + side, dai, 1);
+      F = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 2, c[0], c[1], c[2] + side,
+                               dai);
     L.cf_flux[x] = F;
   }
 }
@@ -461,77 +453,148 @@ __device__ inline long oslab(const Box& b, int a, int dim, const int* c) {
   return idx;
 }
 
-// Composite residual of a level in one pass (the fluxes of k_flux formed on
-// the fly, the coarsened fine fluxes read where L.fmask marks them, then
-// the residual of k_residual): the same per-face values and the same
-// residual expression, without writing and re-reading the side arrays.
-template <int DIM, bool DVAR, int CMODE>
+// Composite residual r = f - div F(u) - C u of every patch of a level in one
+// pass (computeCompositeResidualOnLevel, CellPoissonFACOps.C:1791-1956: the
+// fluxes of compflux*, solv_cellpoissonfacops3d.f:10-133, the coarsened fine
+// fluxes where L.fmask marks a face the finer level covers, and the residual
+// of compresidual*, :485-600).  u is in storage T (FP64, or the error of the
+// mixed-precision cycle in FP32 / BF16) and the arithmetic is FP64.
+// Work-group = a chunk of whole rows (j,k) of one patch, a few cells per
+// lane (one row per chunk when it is wider than kRowCellsPerChunk, the lanes
+// then stride along it); a row's cells lie on consecutive lanes, so every
+// neighbour, coefficient and mask load is a fixed offset from the cell's
+// row base.  Only cells on the patch boundary take the coarse-fine paths
+// (Ewing fix, outerside flux saved for the coarser level's correction).
+template <int DIM, bool DVAR, int CMODE, bool EWING, class T>
 __global__ void __launch_bounds__(kThreads)
-k_residual_fused(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* rr,
-                 int rg, const double* const* ff, int fg, const double* const* uu, int ug,
-                 const double* const* flux, PoissonDev ps, int ewing, double* const* oflux) {
+k_residual_rows(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* rr, int rg,
+                const double* const* ff, int fg, const T* const* uu, int ug,
+                const double* const* flux, PoissonDev ps, double* const* oflux) {
   const int ci = xcd_chunk();
   const Chunk ch = chunks[ci];
   if (ch.item < 0) return;
-  const BoxItem it = items[ch.item];
-  const int p = it.p;
+  const int p = items[ch.item].p;
   const Box b = L.box[p];
-  const View vu = vview(b, ug, DIM), vr = vview(b, rg, DIM), vfg = vview(b, fg, DIM);
-  const View vc = vview(b, ps.cghost, DIM);
-  View vd[3], vf[3];
-  const double* Dp[3] = {nullptr, nullptr, nullptr};
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    vf[a] = vview(b, 0, DIM, a);
-    if (DVAR) {
-      vd[a] = vview(b, ps.dghost, DIM, a);
-      Dp[a] = ps.D[a][p];
-    }
-  }
-  const double* u = uu[p];
-  const double* F[3] = {flux[p * DIM + 0], flux[p * DIM + 1],
-                        DIM == 3 ? flux[p * DIM + 2] : nullptr};
-  const unsigned char* mask = L.fmask ? L.fmask + L.fmask_off[p] : nullptr;
-  const double di[3] = {1.0 / L.dx[0], 1.0 / L.dx[1], DIM == 3 ? 1.0 / L.dx[2] : 0.0};
   const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
-  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
-    int c[3];
-    if (!decode_box(it, (long)ch.start + t, c)) break;
-    const int i = c[0], j = c[1], k = c[2];
-    const int m = mask ? mask[(i - b.lo[0]) + nx * ((j - b.lo[1]) +
-                                                     ny * (DIM == 3 ? k - b.lo[2] : 0))]
-                       : 0;
-    double fl[3] = {0.0, 0.0, 0.0}, fh[3] = {0.0, 0.0, 0.0};
+  const int nz = DIM == 3 ? b.hi[2] - b.lo[2] + 1 : 1;
+  const int rpc = kRowCellsPerChunk / nx > 0 ? kRowCellsPerChunk / nx : 1;
+  const int row_end = min(ch.start + rpc, ny * nz);
+  const int lanes = nx < kThreads ? nx : kThreads;
+  const int rows_per_iter = kThreads / lanes;
+  if ((int)threadIdx.x >= rows_per_iter * lanes) return;
+  const int us1 = nx + 2 * ug, us2 = us1 * (ny + 2 * ug);
+  const int fs1 = nx + 2 * fg, fs2 = fs1 * (ny + 2 * fg);
+  const int rs1 = nx + 2 * rg, rs2 = rs1 * (ny + 2 * rg);
+  const int dgw = ps.dghost;
+  const int dxs1 = nx + 1 + 2 * dgw, dxs2 = dxs1 * (ny + 2 * dgw);
+  const int dys1 = nx + 2 * dgw, dys2 = dys1 * (ny + 1 + 2 * dgw);
+  const int dzs1 = nx + 2 * dgw, dzs2 = dzs1 * (ny + 2 * dgw);
+  const int cs1 = nx + 2 * ps.cghost, cs2 = cs1 * (ny + 2 * ps.cghost);
+  const auto u = gmem(uu[p]);
+  const auto f = gmem(ff[p]);
+  const auto res = gmem(rr[p]);
+  const auto Dx = gmem(DVAR ? ps.D[0][p] : nullptr);
+  const auto Dy = gmem(DVAR ? ps.D[1][p] : nullptr);
+  const auto Dz = gmem((DVAR && DIM == 3) ? ps.D[2][p] : nullptr);
+  const auto Cp = gmem(CMODE == 2 ? ps.C[p] : nullptr);
+  const auto mask = gmem(L.fmask ? L.fmask + L.fmask_off[p] : nullptr);
+  const auto Fx = gmem(flux[p * DIM + 0]);
+  const auto Fy = gmem(flux[p * DIM + 1]);
+  const auto Fz = gmem(DIM == 3 ? flux[p * DIM + 2] : nullptr);
+  const double dxi = 1.0 / L.dx[0], dyi = 1.0 / L.dx[1];
+  const double dzi = DIM == 3 ? 1.0 / L.dx[2] : 0.0;
+  const double dc = ps.dconst;
+  // per-face table offsets of the patch (-1: no coarse-fine part on it)
+  long cfo[6] = {-1, -1, -1, -1, -1, -1};
+  if (EWING) {
 #pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      int h[3] = {i, j, k};
-      h[a] += 1;
-      fl[a] = (m & (1 << (2 * a)))
-                  ? F[a][vf[a].at(i, j, k)]
-                  : face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, a, i, j, k, di[a],
-                                         ewing);
-      fh[a] = (m & (2 << (2 * a)))
-                  ? F[a][vf[a].at(h[0], h[1], h[2])]
-                  : face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, a, h[0], h[1],
-                                         h[2], di[a], ewing);
-    }
-    const double dxi = di[0], dyi = di[1], dzi = di[2];
-    const double scal = CMODE == 2 ? ps.C[p][vc.at(i, j, k)] : (CMODE == 1 ? ps.cconst : 0.0);
-    double res;
-    if (DIM == 3)
-      res = ff[p][vfg.at(i, j, k)] -
-            (dxi * (fh[0] - fl[0]) + dyi * (fh[1] - fl[1]) + dzi * (fh[2] - fl[2])) -
-            scal * u[vu.at(i, j, k)];
-    else
-      res = ff[p][vfg.at(i, j, k)] - (dxi * (fh[0] - fl[0]) + dyi * (fh[1] - fl[1])) -
-            scal * u[vu.at(i, j, k)];
-    rr[p][vr.at(i, j, k)] = res;
-    if (oflux) {
-      // OutersideData::copy(SideData): the patch's boundary faces
-#pragma unroll
-      for (int a = 0; a < DIM; ++a) {
-        if (c[a] == b.lo[a]) oflux[p * 6 + 2 * a][oslab(b, a, DIM, c)] = fl[a];
-        if (c[a] == b.hi[a]) oflux[p * 6 + 2 * a + 1][oslab(b, a, DIM, c)] = fh[a];
+    for (int fc = 0; fc < 2 * DIM; ++fc) cfo[fc] = L.cf_off[p * 6 + fc];
+  }
+  const auto cff = gmem((const double*)L.cf_flux);
+  const int m0 = (int)threadIdx.x % lanes;
+  for (int row = ch.start + (int)threadIdx.x / lanes; row < row_end; row += rows_per_iter) {
+    const int jj = row % ny, kk = DIM == 3 ? row / ny : 0;
+    for (int ii = m0; ii < nx; ii += lanes) {
+      const int o = (ii + ug) + us1 * (jj + ug) + us2 * (DIM == 3 ? kk + ug : 0);
+      const int orr = (ii + rg) + rs1 * (jj + rg) + rs2 * (DIM == 3 ? kk + rg : 0);
+      const double uc = ld64(u, o);
+      double Dxl, Dxh, Dyl, Dyh, Dzl = 0.0, Dzh = 0.0;
+      if (DVAR) {
+        const int ox = (ii + dgw) + dxs1 * (jj + dgw) + dxs2 * (DIM == 3 ? kk + dgw : 0);
+        const int oy = (ii + dgw) + dys1 * (jj + dgw) + dys2 * (DIM == 3 ? kk + dgw : 0);
+        Dxl = Dx[ox];
+        Dxh = Dx[ox + 1];
+        Dyl = Dy[oy];
+        Dyh = Dy[oy + dys1];
+        if (DIM == 3) {
+          const int oz = (ii + dgw) + dzs1 * (jj + dgw) + dzs2 * (kk + dgw);
+          Dzl = Dz[oz];
+          Dzh = Dz[oz + dzs2];
+        }
+      } else {
+        Dxl = Dxh = Dyl = Dyh = Dzl = Dzh = dc;
+      }
+      double fxl = dxi * Dxl * (uc - ld64(u, o - 1));
+      double fxh = dxi * Dxh * (ld64(u, o + 1) - uc);
+      double fyl = dyi * Dyl * (uc - ld64(u, o - us1));
+      double fyh = dyi * Dyh * (ld64(u, o + us1) - uc);
+      double fzl = 0.0, fzh = 0.0;
+      if (DIM == 3) {
+        fzl = dzi * Dzl * (uc - ld64(u, o - us2));
+        fzh = dzi * Dzh * (ld64(u, o + us2) - uc);
+      }
+      const bool edge = ii == 0 || ii == nx - 1 || jj == 0 || jj == ny - 1 ||
+                        (DIM == 3 && (kk == 0 || kk == nz - 1));
+      const int c[3] = {b.lo[0] + ii, b.lo[1] + jj, DIM == 3 ? b.lo[2] + kk : 0};
+      if (EWING && edge) {
+        // boundary faces of a face with coarse-fine parts: the snapshot
+        if (ii == 0 && cfo[0] >= 0) fxl = cff[cfo[0] + (jj + ny * kk)];
+        if (ii == nx - 1 && cfo[1] >= 0) fxh = cff[cfo[1] + (jj + ny * kk)];
+        if (jj == 0 && cfo[2] >= 0) fyl = cff[cfo[2] + (ii + nx * kk)];
+        if (jj == ny - 1 && cfo[3] >= 0) fyh = cff[cfo[3] + (ii + nx * kk)];
+        if (DIM == 3) {
+          if (kk == 0 && cfo[4] >= 0) fzl = cff[cfo[4] + (ii + nx * jj)];
+          if (kk == nz - 1 && cfo[5] >= 0) fzh = cff[cfo[5] + (ii + nx * jj)];
+        }
+      }
+      const int cell = ii + nx * (jj + ny * kk);
+      if (mask) {
+        const int mk = mask[cell];
+        if (mk) {
+          // faces under the finer level: its coarsened boundary flux
+          const int bx = ii + (nx + 1) * (jj + ny * kk);
+          const int by = ii + nx * (jj + (ny + 1) * kk);
+          if (mk & 1) fxl = Fx[bx];
+          if (mk & 2) fxh = Fx[bx + 1];
+          if (mk & 4) fyl = Fy[by];
+          if (mk & 8) fyh = Fy[by + nx];
+          if (DIM == 3) {
+            if (mk & 16) fzl = Fz[cell];
+            if (mk & 32) fzh = Fz[cell + nx * ny];
+          }
+        }
+      }
+      const double scal = CMODE == 2
+                              ? Cp[(ii + ps.cghost) + cs1 * (jj + ps.cghost) +
+                                   cs2 * (DIM == 3 ? kk + ps.cghost : 0)]
+                              : (CMODE == 1 ? ps.cconst : 0.0);
+      const double fv = f[(ii + fg) + fs1 * (jj + fg) + fs2 * (DIM == 3 ? kk + fg : 0)];
+      double rv;
+      if (DIM == 3)
+        rv = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
+      else
+        rv = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
+      res[orr] = rv;
+      if (oflux && edge) {
+        // OutersideData::copy(SideData): the patch's boundary faces
+        if (ii == 0) gmem(oflux[p * 6 + 0])[oslab(b, 0, DIM, c)] = fxl;
+        if (ii == nx - 1) gmem(oflux[p * 6 + 1])[oslab(b, 0, DIM, c)] = fxh;
+        if (jj == 0) gmem(oflux[p * 6 + 2])[oslab(b, 1, DIM, c)] = fyl;
+        if (jj == ny - 1) gmem(oflux[p * 6 + 3])[oslab(b, 1, DIM, c)] = fyh;
+        if (DIM == 3) {
+          if (kk == 0) gmem(oflux[p * 6 + 4])[oslab(b, 2, DIM, c)] = fzl;
+          if (kk == nz - 1) gmem(oflux[p * 6 + 5])[oslab(b, 2, DIM, c)] = fzh;
+        }
       }
     }
   }
@@ -784,25 +847,42 @@ void launch_rbgs(const SmoothLaunch& w, const LevelGeom& L, double* const* e, in
   check_launch();
 }
 
+template <class T>
+static void ewing_faces_t(const GhostLaunch& w, const LevelGeom& L, const void* const* u, int ug,
+                          const PoissonDev& ps, const int* stop, int all, hipStream_t s) {
+  const T* const* ut = reinterpret_cast<const T* const*>(u);
+  const dim3 g(xcd_grid(w.nchunks)), bl(kThreads);
+  const bool dvar = ps.D[0] != nullptr;
+  if (L.dim == 3) {
+    if (dvar)
+      hipLaunchKernelGGL((k_ewing_faces<3, true, T>), g, bl, 0, s, w.items, w.chunks, L, ut, ug,
+                         ps, stop, all);
+    else
+      hipLaunchKernelGGL((k_ewing_faces<3, false, T>), g, bl, 0, s, w.items, w.chunks, L, ut, ug,
+                         ps, stop, all);
+  } else {
+    if (dvar)
+      hipLaunchKernelGGL((k_ewing_faces<2, true, T>), g, bl, 0, s, w.items, w.chunks, L, ut, ug,
+                         ps, stop, all);
+    else
+      hipLaunchKernelGGL((k_ewing_faces<2, false, T>), g, bl, 0, s, w.items, w.chunks, L, ut, ug,
+                         ps, stop, all);
+  }
+}
+
 void launch_ewing_faces(const GhostLaunch& w, const LevelGeom& L, const double* const* u, int ug,
                         const PoissonDev& ps, const int* stop, hipStream_t s) {
   if (w.nchunks == 0) return;
-  bool dvar = ps.D[0] != nullptr;
-  if (L.dim == 3) {
-    if (dvar)
-      hipLaunchKernelGGL((k_ewing_faces<3, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
-                         w.items, w.chunks, L, u, ug, ps, stop);
-    else
-      hipLaunchKernelGGL((k_ewing_faces<3, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
-                         w.items, w.chunks, L, u, ug, ps, stop);
-  } else {
-    if (dvar)
-      hipLaunchKernelGGL((k_ewing_faces<2, true>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
-                         w.items, w.chunks, L, u, ug, ps, stop);
-    else
-      hipLaunchKernelGGL((k_ewing_faces<2, false>), dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s,
-                         w.items, w.chunks, L, u, ug, ps, stop);
-  }
+  ewing_faces_t<double>(w, L, reinterpret_cast<const void* const*>(u), ug, ps, stop, 0, s);
+  check_launch();
+}
+
+void launch_residual_faces(int elem, const GhostLaunch& w, const LevelGeom& L,
+                           const void* const* u, int ug, const PoissonDev& ps, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  if (elem == 8) ewing_faces_t<double>(w, L, u, ug, ps, nullptr, 1, s);
+  else if (elem == 4) ewing_faces_t<float>(w, L, u, ug, ps, nullptr, 1, s);
+  else ewing_faces_t<bf16>(w, L, u, ug, ps, nullptr, 1, s);
   check_launch();
 }
 
@@ -818,36 +898,57 @@ void launch_flux_coarsen(const GhostLaunch& w, const LevelGeom& coarse, const Le
   check_launch();
 }
 
-template <int DIM, bool DVAR>
-static void residual_fused_c(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
-                             const double* const* f, int fg, const double* const* u, int ug,
-                             const double* const* flux, const PoissonDev& ps, int ewing,
-                             double* const* oflux, hipStream_t s) {
+template <int DIM, bool DVAR, int CMODE, class T>
+static void residual_rows_e(const SmoothLaunch& w, const LevelGeom& L, double* const* r, int rg,
+                            const double* const* f, int fg, const void* const* u, int ug,
+                            const double* const* flux, const PoissonDev& ps, int ewing,
+                            double* const* oflux, hipStream_t s) {
+  const T* const* ut = reinterpret_cast<const T* const*>(u);
   const dim3 g(xcd_grid(w.nchunks)), bl(kThreads);
-  if (ps.cmode == 2)
-    hipLaunchKernelGGL((k_residual_fused<DIM, DVAR, 2>), g, bl, 0, s, w.items, w.chunks, L, r, rg,
-                       f, fg, u, ug, flux, ps, ewing, oflux);
-  else if (ps.cmode == 1)
-    hipLaunchKernelGGL((k_residual_fused<DIM, DVAR, 1>), g, bl, 0, s, w.items, w.chunks, L, r, rg,
-                       f, fg, u, ug, flux, ps, ewing, oflux);
+  if (ewing)
+    hipLaunchKernelGGL((k_residual_rows<DIM, DVAR, CMODE, true, T>), g, bl, 0, s, w.items,
+                       w.chunks, L, r, rg, f, fg, ut, ug, flux, ps, oflux);
   else
-    hipLaunchKernelGGL((k_residual_fused<DIM, DVAR, 0>), g, bl, 0, s, w.items, w.chunks, L, r, rg,
-                       f, fg, u, ug, flux, ps, ewing, oflux);
+    hipLaunchKernelGGL((k_residual_rows<DIM, DVAR, CMODE, false, T>), g, bl, 0, s, w.items,
+                       w.chunks, L, r, rg, f, fg, ut, ug, flux, ps, oflux);
 }
 
-void launch_residual_fused(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
-                           const double* const* f, int fg, const double* const* u, int ug,
-                           const double* const* flux, const PoissonDev& ps, int ewing,
-                           double* const* oflux, hipStream_t s) {
-  if (w.nchunks == 0) return;
+template <int DIM, bool DVAR, class T>
+static void residual_rows_c(const SmoothLaunch& w, const LevelGeom& L, double* const* r, int rg,
+                            const double* const* f, int fg, const void* const* u, int ug,
+                            const double* const* flux, const PoissonDev& ps, int ewing,
+                            double* const* oflux, hipStream_t s) {
+  if (ps.cmode == 2)
+    residual_rows_e<DIM, DVAR, 2, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+  else if (ps.cmode == 1)
+    residual_rows_e<DIM, DVAR, 1, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+  else
+    residual_rows_e<DIM, DVAR, 0, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+}
+
+template <class T>
+static void residual_rows_t(const SmoothLaunch& w, const LevelGeom& L, double* const* r, int rg,
+                            const double* const* f, int fg, const void* const* u, int ug,
+                            const double* const* flux, const PoissonDev& ps, int ewing,
+                            double* const* oflux, hipStream_t s) {
   const bool dvar = ps.D[0] != nullptr;
   if (L.dim == 3) {
-    if (dvar) residual_fused_c<3, true>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
-    else residual_fused_c<3, false>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+    if (dvar) residual_rows_c<3, true, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+    else residual_rows_c<3, false, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
   } else {
-    if (dvar) residual_fused_c<2, true>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
-    else residual_fused_c<2, false>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+    if (dvar) residual_rows_c<2, true, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+    else residual_rows_c<2, false, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
   }
+}
+
+void launch_residual(int elem, const SmoothLaunch& w, const LevelGeom& L, double* const* r,
+                     int rg, const double* const* f, int fg, const void* const* u, int ug,
+                     const double* const* flux, const PoissonDev& ps, int ewing,
+                     double* const* oflux, hipStream_t s) {
+  if (w.nchunks == 0) return;
+  if (elem == 8) residual_rows_t<double>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+  else if (elem == 4) residual_rows_t<float>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
+  else residual_rows_t<bf16>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
   check_launch();
 }
 

@@ -41,11 +41,6 @@ void launch_rbgs_lp(int elem, const SmoothLaunch& w, const LevelGeom& L, void* c
 void launch_ewing_faces_lp(int elem, const GhostLaunch& w, const LevelGeom& L,
                            const void* const* u, int ug, const PoissonLPRaw& ps, const int* stop,
                            hipStream_t s);
-// r = f - div F(u) - C u with u in elem storage, FP64 arithmetic and D.
-void launch_residual_lp(int elem, const GhostLaunch& w, const LevelGeom& L, double* const* r,
-                        int rg, const double* const* f, int fg, const void* const* u, int ug,
-                        const double* const* flux, const PoissonDev& ps, int ewing,
-                        double* const* oflux, hipStream_t s);
 void launch_prolong_add_lp(int elem, const GhostLaunch& w, const LevelGeom& coarse,
                            const LevelGeom& fine, const void* const* ce, int cg, void* const* fe,
                            int fg, int method, hipStream_t s);

@@ -9,8 +9,9 @@
 // prolongation of the coarse error run on T storage with FP32 arithmetic and
 // the face coefficients D in T storage.  The composite residual of the error
 // equation (fluxes, Ewing fix, the coarsened fine fluxes that replace coarse
-// faces) reads the error in T and computes in FP64 with the FP64 D; the
-// restriction of residuals, u += e and the outer residual stay FP64.
+// faces) reads the error in T and computes in FP64 with the FP64 D
+// (launch_residual, level_kernels.hip); the restriction of residuals,
+// u += e and the outer residual stay FP64.
 // Expression orders follow the FP64 kernels (level_kernels.hip), which
 // follow the reference Fortran.
 #include "level_mixed.h"
@@ -392,104 +393,6 @@ k_rbgs_lp(const BoxItem* items, const Chunk* chunks, LevelGeom L, T* const* e, i
   }
 }
 
-// Composite residual of the error equation with the error in T storage,
-// computed in FP64 with the FP64 coefficients (k_residual_fused).
-template <int DIM, bool DVAR, int CMODE, class T>
-__global__ void __launch_bounds__(kThreads)
-k_residual_lp(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* rr, int rg,
-              const double* const* ff, int fg, const T* const* uu, int ug,
-              const double* const* flux, PoissonDev ps, int ewing, double* const* oflux) {
-  const int ci = xcd_chunk();
-  const Chunk ch = chunks[ci];
-  if (ch.item < 0) return;
-  const BoxItem it = items[ch.item];
-  const int p = it.p;
-  const Box b = L.box[p];
-  const View vu = vv(b, ug, DIM), vr = vv(b, rg, DIM), vfg = vv(b, fg, DIM);
-  const View vc = vv(b, ps.cghost, DIM);
-  View vd[3], vf[3];
-#pragma unroll
-  for (int a = 0; a < DIM; ++a) {
-    vf[a] = vv(b, 0, DIM, a);
-    if (DVAR) vd[a] = vv(b, ps.dghost, DIM, a);
-  }
-  const T* u = uu[p];
-  const unsigned char* mask = L.fmask ? L.fmask + L.fmask_off[p] : nullptr;
-  const double di[3] = {1.0 / L.dx[0], 1.0 / L.dx[1], DIM == 3 ? 1.0 / L.dx[2] : 0.0};
-  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
-  auto U = [&](int i, int j, int k) { return (double)ld(u[vu.at(i, j, k)]); };
-  // face flux of axis a at face index c (FP64; Ewing fix when flagged)
-  auto flux_at = [&](int a, const int* c) {
-    int m[3] = {c[0], c[1], c[2]};
-    m[a] -= 1;
-    const double D = DVAR ? ps.D[a][p][vd[a].at(c[0], c[1], c[2])] : ps.dconst;
-    double F = di[a] * D * (U(c[0], c[1], c[2]) - U(m[0], m[1], m[2]));
-    if (ewing) {
-      const int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
-      if (side >= 0) {
-        int in[3];
-        for (int d = 0; d < 3; ++d) in[d] = side == 0 ? c[d] : m[d];
-        if (cf_idx(L, p, 2 * a + side, in, b) >= 0) {
-          const int r = L.ratio;
-          int q[3] = {in[0], in[1], in[2]};
-          int t1, t2;
-          tang(a, DIM, t1, t2);
-          const int bs1 = in[t1] - (in[t1] - cdiv(in[t1], r) * r);
-          q[t1] = bs1 + (r - 1 - (in[t1] - bs1));
-          if (DIM == 3) {
-            const int bs2 = in[t2] - (in[t2] - cdiv(in[t2], r) * r);
-            q[t2] = bs2 + (r - 1 - (in[t2] - bs2));
-          }
-          const double tranwt = ewing_tranwt(r);
-          const double longwt = 2 * tranwt;
-          const double h = L.dx[a];
-          const double tv = tranwt * D * (U(q[0], q[1], q[2]) - U(in[0], in[1], in[2])) / h;
-          F = side == 0 ? longwt * F + tv : longwt * F - tv;
-        }
-      }
-    }
-    return F;
-  };
-  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
-    int c[3];
-    if (!decode_box(it, (long)ch.start + t, c)) break;
-    const int i = c[0], j = c[1], k = c[2];
-    const int mk = mask ? mask[(i - b.lo[0]) + nx * ((j - b.lo[1]) +
-                                                      ny * (DIM == 3 ? k - b.lo[2] : 0))]
-                        : 0;
-    double fl[3] = {0.0, 0.0, 0.0}, fh[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int a = 0; a < DIM; ++a) {
-      int h[3] = {i, j, k};
-      h[a] += 1;
-      fl[a] = (mk & (1 << (2 * a))) ? flux[p * DIM + a][vf[a].at(i, j, k)] : flux_at(a, c);
-      fh[a] = (mk & (2 << (2 * a))) ? flux[p * DIM + a][vf[a].at(h[0], h[1], h[2])]
-                                    : flux_at(a, h);
-    }
-    const double scal = CMODE == 2 ? ps.C[p][vc.at(i, j, k)] : (CMODE == 1 ? ps.cconst : 0.0);
-    double res;
-    if (DIM == 3)
-      res = ff[p][vfg.at(i, j, k)] -
-            (di[0] * (fh[0] - fl[0]) + di[1] * (fh[1] - fl[1]) + di[2] * (fh[2] - fl[2])) -
-            scal * U(i, j, k);
-    else
-      res = ff[p][vfg.at(i, j, k)] - (di[0] * (fh[0] - fl[0]) + di[1] * (fh[1] - fl[1])) -
-            scal * U(i, j, k);
-    rr[p][vr.at(i, j, k)] = res;
-    if (oflux) {
-#pragma unroll
-      for (int a = 0; a < DIM; ++a) {
-        int t1, t2;
-        tang(a, DIM, t1, t2);
-        long idx = s3(c, t1) - s3(b.lo, t1);
-        if (DIM == 3) idx += (long)(s3(b.hi, t1) - s3(b.lo, t1) + 1) * (s3(c, t2) - s3(b.lo, t2));
-        if (c[a] == b.lo[a]) oflux[p * 6 + 2 * a][idx] = fl[a];
-        if (c[a] == b.hi[a]) oflux[p * 6 + 2 * a + 1][idx] = fh[a];
-      }
-    }
-  }
-}
-
 // e_f += refine(e_c), FP32 on T storage.
 template <int DIM, class T>
 __global__ void __launch_bounds__(kThreads)
@@ -616,39 +519,6 @@ void ewing_t(const GhostLaunch& w, const LevelGeom& L, const void* const* u, int
   }
 }
 
-template <int DIM, bool DVAR, class T>
-void residual_c(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
-                const double* const* f, int fg, const void* const* u, int ug,
-                const double* const* flux, const PoissonDev& ps, int ewing,
-                double* const* oflux, hipStream_t s) {
-  const T* const* ut = reinterpret_cast<const T* const*>(u);
-  const dim3 g(xcd_grid(w.nchunks)), bl(kThreads);
-  if (ps.cmode == 2)
-    hipLaunchKernelGGL((k_residual_lp<DIM, DVAR, 2, T>), g, bl, 0, s, w.items, w.chunks, L, r,
-                       rg, f, fg, ut, ug, flux, ps, ewing, oflux);
-  else if (ps.cmode == 1)
-    hipLaunchKernelGGL((k_residual_lp<DIM, DVAR, 1, T>), g, bl, 0, s, w.items, w.chunks, L, r,
-                       rg, f, fg, ut, ug, flux, ps, ewing, oflux);
-  else
-    hipLaunchKernelGGL((k_residual_lp<DIM, DVAR, 0, T>), g, bl, 0, s, w.items, w.chunks, L, r,
-                       rg, f, fg, ut, ug, flux, ps, ewing, oflux);
-}
-
-template <class T>
-void residual_t(const GhostLaunch& w, const LevelGeom& L, double* const* r, int rg,
-                const double* const* f, int fg, const void* const* u, int ug,
-                const double* const* flux, const PoissonDev& ps, int ewing,
-                double* const* oflux, hipStream_t s) {
-  const bool dvar = ps.D[0] != nullptr;
-  if (L.dim == 3) {
-    if (dvar) residual_c<3, true, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
-    else residual_c<3, false, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
-  } else {
-    if (dvar) residual_c<2, true, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
-    else residual_c<2, false, T>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
-  }
-}
-
 }  // namespace
 
 void launch_ghost_items_lp(int elem, const GhostLaunch& w, const LevelGeom& L, void* const* dst,
@@ -684,16 +554,6 @@ void launch_ewing_faces_lp(int elem, const GhostLaunch& w, const LevelGeom& L,
   if (w.nchunks == 0) return;
   if (elem == 4) ewing_t<float>(w, L, u, ug, ps, stop, s);
   else ewing_t<bf16>(w, L, u, ug, ps, stop, s);
-  check_launch();
-}
-
-void launch_residual_lp(int elem, const GhostLaunch& w, const LevelGeom& L, double* const* r,
-                        int rg, const double* const* f, int fg, const void* const* u, int ug,
-                        const double* const* flux, const PoissonDev& ps, int ewing,
-                        double* const* oflux, hipStream_t s) {
-  if (w.nchunks == 0) return;
-  if (elem == 4) residual_t<float>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
-  else residual_t<bf16>(w, L, r, rg, f, fg, u, ug, flux, ps, ewing, oflux, s);
   check_launch();
 }
 
