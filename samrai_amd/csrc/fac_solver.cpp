@@ -232,6 +232,7 @@ void CellPoissonFACOps::initializeOperatorState(const SAMRAIVectorReal& u,
     hip_check(hipMalloc(&d_maxres[ln], sizeof(double) * (2 * nsc + 2)), "hipMalloc(maxres)");
     hip_check(hipMemset(d_maxres[ln], 0, sizeof(double) * (2 * nsc + 2)), "hipMemset");
     maxchunks = std::max(maxchunks, (long)d_h.tables(ln).cells.nchunks);
+    maxchunks = std::max(maxchunks, (long)d_h.tables(ln).rows.nchunks);
   }
   d_partial_n = maxchunks;
   hip_check(hipMalloc(&d_partial, sizeof(double) * d_partial_n), "hipMalloc(partial)");
@@ -1453,10 +1454,10 @@ void CellPoissonFACOps::rmsNormEnqueue(const PatchDataSet& x, const PatchDataSet
   for (int pass = 0; pass < 2; ++pass)
     for (int ln = coarse_ln; ln <= fine_ln; ++ln) {
       const auto& T = d_h.tables(ln);
-      sfac::launch_dot_partial(T.cells.launch(), T.geom, pass == 0 ? x.table(ln) : nullptr,
+      sfac::launch_dot_partial(T.rows.smooth(), T.geom, pass == 0 ? x.table(ln) : nullptr,
                                x.ghost(), pass == 0 ? x.table(ln) : nullptr, x.ghost(),
                                w.table(ln), d_partial, d_stream);
-      sfac::launch_sum(d_partial, T.cells.nchunks, d_scalars + 2 * pass, ln > coarse_ln,
+      sfac::launch_sum(d_partial, T.rows.nchunks, d_scalars + 2 * pass, ln > coarse_ln,
                        d_stream);
     }
 }

@@ -92,9 +92,9 @@ void launch_prolong_add(const GhostLaunch& w, const LevelGeom& coarse, const Lev
                         const double* const* ce, int cg, double* const* fe, int fg,
                         int method, hipStream_t s);
 
-// Partial sums of w*x*y over interior chunks (one double per chunk);
-// x == nullptr sums w alone.
-void launch_dot_partial(const GhostLaunch& cells, const LevelGeom& L, const double* const* x,
+// Partial sums of w*x*y over the interior, one double per chunk of rows (the
+// level's whole-row chunk table); x == nullptr sums w alone.
+void launch_dot_partial(const SmoothLaunch& rows, const LevelGeom& L, const double* const* x,
                         int xg, const double* const* y, int yg, const double* const* w,
                         double* partial, hipStream_t s);
 // out[0] = sum of n partials in a fixed order.

@@ -691,6 +691,9 @@ k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom 
 
 // --------------------------------------------------------- reductions
 
+// Partial sums of w*x*y over the interior of every patch (x == nullptr:
+// of w alone, sumControlVolumes): one value per whole-row chunk, each lane
+// summing its cells in row order, then the work-group in a fixed order.
 template <int DIM>
 __global__ void __launch_bounds__(kThreads)
 k_dot_partial(const BoxItem* items, const Chunk* chunks, LevelGeom L, const double* const* xx,
@@ -699,42 +702,61 @@ k_dot_partial(const BoxItem* items, const Chunk* chunks, LevelGeom L, const doub
   const int ci = xcd_chunk();
   const Chunk ch = chunks[ci];
   if (ch.item < 0) return;
-  const BoxItem it = items[ch.item];
-  const Box b = L.box[it.p];
-  const View vx = vview(b, xg, DIM), vy = vview(b, yg, DIM), vw = vview(b, 0, DIM);
+  const int p = items[ch.item].p;
+  const Box b = L.box[p];
+  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
+  const int nz = DIM == 3 ? b.hi[2] - b.lo[2] + 1 : 1;
+  const int rpc = kRowCellsPerChunk / nx > 0 ? kRowCellsPerChunk / nx : 1;
+  const int row_end = min(ch.start + rpc, ny * nz);
+  const int lanes = nx < kThreads ? nx : kThreads;
+  const int rows_per_iter = kThreads / lanes;
+  const int xs1 = nx + 2 * xg, xs2 = xs1 * (ny + 2 * xg);
+  const int ys1 = nx + 2 * yg, ys2 = ys1 * (ny + 2 * yg);
+  const auto x = gmem(xx ? xx[p] : nullptr);
+  const auto y = gmem(xx ? yy[p] : nullptr);
+  const auto w = gmem(ww[p]);
   double acc = 0.0;
-  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
-    int c[3];
-    if (!decode_box(it, (long)ch.start + t, c)) break;
-    // xx == nullptr: sum of the weights (sumControlVolumes)
-    if (xx)
-      acc += xx[it.p][vx.at(c[0], c[1], c[2])] * yy[it.p][vy.at(c[0], c[1], c[2])] *
-             ww[it.p][vw.at(c[0], c[1], c[2])];
-    else
-      acc += ww[it.p][vw.at(c[0], c[1], c[2])];
+  if ((int)threadIdx.x < rows_per_iter * lanes) {
+    const int m0 = (int)threadIdx.x % lanes;
+    for (int row = ch.start + (int)threadIdx.x / lanes; row < row_end; row += rows_per_iter) {
+      const int jj = row % ny, kk = DIM == 3 ? row / ny : 0;
+      for (int ii = m0; ii < nx; ii += lanes) {
+        const double wv = w[ii + nx * (jj + ny * kk)];
+        if (xx)
+          acc += x[(ii + xg) + xs1 * (jj + xg) + xs2 * (DIM == 3 ? kk + xg : 0)] *
+                 y[(ii + yg) + ys1 * (jj + yg) + ys2 * (DIM == 3 ? kk + yg : 0)] * wv;
+        else
+          acc += wv;
+      }
+    }
   }
   __shared__ double sm[kThreads / 64];
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int w = 0; w < kThreads / 64; ++w) s += sm[w];
-    partial[ci] = s;
+    double t = 0.0;
+    for (int v = 0; v < kThreads / 64; ++v) t += sm[v];
+    partial[ci] = t;
   }
 }
 
-__global__ void __launch_bounds__(kThreads)
+// out = (accumulate ? out : 0) + the sum of n partials, in a fixed order:
+// lane t sums partials t, t + kSumThreads, ... (four loads in flight), then
+// the work-group.
+constexpr int kSumThreads = 1024;
+__global__ void __launch_bounds__(kSumThreads)
 k_sum(const double* partial, int n, double* out, int accumulate) {
   double acc = 0.0;
-  for (int t = threadIdx.x; t < n; t += kThreads) acc += partial[t];
-  __shared__ double sm[kThreads / 64];
+#pragma unroll 4
+  for (int t = threadIdx.x; t < n; t += kSumThreads) acc += partial[t];
+  __shared__ double sm[kSumThreads / 64];
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
-    for (int w = 0; w < kThreads / 64; ++w) s += sm[w];
+    for (int w = 0; w < kSumThreads / 64; ++w) s += sm[w];
     out[0] = accumulate ? out[0] + s : s;
   }
 }
@@ -978,7 +1000,7 @@ void launch_prolong_add(const GhostLaunch& w, const LevelGeom& coarse, const Lev
   check_launch();
 }
 
-void launch_dot_partial(const GhostLaunch& w, const LevelGeom& L, const double* const* x,
+void launch_dot_partial(const SmoothLaunch& w, const LevelGeom& L, const double* const* x,
                         int xg, const double* const* y, int yg, const double* const* wts,
                         double* partial, hipStream_t s) {
   if (w.nchunks == 0) return;
@@ -992,7 +1014,7 @@ void launch_dot_partial(const GhostLaunch& w, const LevelGeom& L, const double* 
 }
 
 void launch_sum(const double* partial, int n, double* out, int accumulate, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum, dim3(1), dim3(kThreads), 0, s, partial, n, out, accumulate);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(kSumThreads), 0, s, partial, n, out, accumulate);
   check_launch();
 }
 
