@@ -183,6 +183,21 @@ int sfac_hierarchy_destroy(sfac_hierarchy* h);
  * ArrayData layout (pdat/ArrayData.C:81-193). */
 int sfac_cell_data_create(sfac_hierarchy* h, int ghost, sfac_patch_data** out);
 int sfac_side_data_create(sfac_hierarchy* h, int ghost, sfac_patch_data** out);
+/* The same over device arrays the caller owns (each patch's
+ * ArrayData<double>::getPointer(), pdat/ArrayData.C:81-90, in the layout
+ * above with ghost width `ghost`), bound without copying, as the reference
+ * solver works on the caller's patch data ids (solv/CellPoissonFACSolver.h:
+ * 182-188, solv/CellPoissonFACOps.h:96-101).  arrays[(n + p) * ncomp + a] is
+ * component a of patch p of level ln, n = the number of patches on the
+ * levels below ln, ncomp = 1 (cell) or dim (side, one array per axis);
+ * narrays must equal the total.  Arrays must be 8-byte aligned and stay
+ * allocated until the patch data and every solver using it are destroyed;
+ * sfac_patch_data_destroy never frees them.  Solves write the solution in
+ * place. */
+int sfac_cell_data_wrap(sfac_hierarchy* h, int ghost, double* const* arrays, long narrays,
+                        sfac_patch_data** out);
+int sfac_side_data_wrap(sfac_hierarchy* h, int ghost, double* const* arrays, long narrays,
+                        sfac_patch_data** out);
 int sfac_patch_data_destroy(sfac_patch_data* d);
 /* ArrayData::getPointer(axis) of patch p on level ln, and its element count. */
 int sfac_patch_data_pointer(sfac_patch_data* d, int ln, int p, int axis, double** ptr,

@@ -673,11 +673,15 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArraysT<T> Af, BoxLevel Gc, T* Dc0, T* Dc1, T*
 
 // The smallest grid (at most kMGSmallEdge cells per axis): u = 0, then
 // `sweeps` red-black sweeps with the whole grid and its ghosts in LDS and
-// each thread's cell coefficients in registers; one work-group.
+// each thread's cell coefficients in registers; one work-group of one
+// wavefront (one cell per lane), so its barriers cost no more than the LDS
+// wait they carry.
 constexpr int kMGSmallEdge = 4;
+constexpr int kMGSmallThreads = 64;
 template <int DIM, bool DVAR, int CMODE, typename T>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kMGSmallThreads)
 k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, const int* stop) {
+  constexpr int kThreads = kMGSmallThreads;
   constexpr int L = kMGSmallEdge + 2;
   constexpr int SY = L, SZ = L * L;
   constexpr int NMAX = DIM == 3 ? kMGSmallEdge * kMGSmallEdge * kMGSmallEdge
@@ -921,7 +925,7 @@ struct SmallRun {
   template <typename T>
   static void run(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc, T* u,
                   int sweeps, const int* stop, hipStream_t s) {
-    hipLaunchKernelGGL((k_mg_small_solve<DIM, DVAR, CMODE, T>), dim3(1), dim3(kThreads), 0, s, G,
+    hipLaunchKernelGGL((k_mg_small_solve<DIM, DVAR, CMODE, T>), dim3(1), dim3(kMGSmallThreads), 0, s, G,
                        A, bc, u, sweeps, stop);
     check_launch();
   }

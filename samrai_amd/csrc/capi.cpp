@@ -138,6 +138,35 @@ int sfac_side_data_create(sfac_hierarchy* h, int ghost, sfac_patch_data** out) {
   });
 }
 
+static int wrap_data(sfac_hierarchy* h, pdat::PatchDataSet::Centring c, int ghost,
+                     double* const* arrays, long narrays, sfac_patch_data** out) {
+  return guarded([&] {
+    need(h, "hierarchy");
+    need(out, "out");
+    if (narrays < 0 || (narrays > 0 && !arrays)) fail("patch data wrap: no arrays");
+    if (!h->h->finalized()) h->h->finalize();
+    std::vector<double*> v(arrays, arrays + narrays);
+    auto* d = new sfac_patch_data;
+    try {
+      d->d.reset(new pdat::PatchDataSet(*h->h, c, ghost, v));
+    } catch (...) {
+      delete d;
+      throw;
+    }
+    *out = d;
+  });
+}
+
+int sfac_cell_data_wrap(sfac_hierarchy* h, int ghost, double* const* arrays, long narrays,
+                        sfac_patch_data** out) {
+  return wrap_data(h, pdat::PatchDataSet::kCell, ghost, arrays, narrays, out);
+}
+
+int sfac_side_data_wrap(sfac_hierarchy* h, int ghost, double* const* arrays, long narrays,
+                        sfac_patch_data** out) {
+  return wrap_data(h, pdat::PatchDataSet::kSide, ghost, arrays, narrays, out);
+}
+
 int sfac_patch_data_destroy(sfac_patch_data* d) {
   return guarded([&] { delete d; });
 }
@@ -249,9 +278,7 @@ int sfac_patch_data_fill(sfac_patch_data* d, double value) {
     need(d, "patch data");
     if (d->d->elemBytes() != 8 && value != 0.0)
       fail("low-precision patch data can only be filled with 0");
-    for (int ln = 0; ln < d->d->numberOfLevels(); ++ln)
-      sfac::launch_fill(d->d->slab(ln), d->d->slabSize(ln) * d->d->elemBytes() / 8, value,
-                        nullptr);
+    d->d->fill(value, nullptr);
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   });
 }

@@ -34,19 +34,11 @@ void FACPreconditioner::saveSolution(const SAMRAIVectorReal& u) {
   const hier::PatchHierarchy& h = u.data->hierarchy();
   if (!d_u_save || d_u_save->ghost() != u.data->ghost())
     d_u_save.reset(new PatchDataSet(h, PatchDataSet::kCell, u.data->ghost()));
-  for (int ln = d_lmin; ln <= d_lmax; ++ln)
-    hip_check(hipMemcpyAsync(d_u_save->slab(ln), u.data->slab(ln),
-                             sizeof(double) * u.data->slabSize(ln), hipMemcpyDeviceToDevice,
-                             d_gpu->stream()),
-              "hipMemcpyAsync(save u)");
+  d_u_save->copyFrom(*u.data, d_lmin, d_lmax, d_gpu->stream());
 }
 
 void FACPreconditioner::restoreSolution(SAMRAIVectorReal& u) {
-  for (int ln = d_lmin; ln <= d_lmax; ++ln)
-    hip_check(hipMemcpyAsync(u.data->slab(ln), d_u_save->slab(ln),
-                             sizeof(double) * u.data->slabSize(ln), hipMemcpyDeviceToDevice,
-                             d_gpu->stream()),
-              "hipMemcpyAsync(restore u)");
+  u.data->copyFrom(*d_u_save, d_lmin, d_lmax, d_gpu->stream());
 }
 
 void FACPreconditioner::setMaxCycles(int n) {

@@ -71,40 +71,124 @@ PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost,
     const size_t bytes = (size_t)elem * (size_t)(total > 0 ? total : 32);
     hip_check(hipMalloc(&L.slab, bytes), "hipMalloc(data)");
     hip_check(hipMemset(L.slab, 0, bytes), "hipMemset");
-    auto at = [&](long off) {
-      return reinterpret_cast<double*>(reinterpret_cast<char*>(L.slab) + (size_t)elem * off);
-    };
-    for (int a = 0; a < ncomp; ++a) {
-      std::vector<double*> tab(np);
-      for (int p = 0; p < np; ++p) tab[p] = at(L.off[a][p]);
-      hip_check(hipMalloc(&L.dtab[a], sizeof(double*) * (np > 0 ? np : 1)), "hipMalloc(tab)");
-      if (np)
-        hip_check(hipMemcpy(L.dtab[a], tab.data(), sizeof(double*) * np, hipMemcpyHostToDevice),
-                  "hipMemcpy(tab)");
-    }
-    std::vector<double*> il((size_t)np * ncomp);
+    L.ptr.assign(ncomp, std::vector<double*>(np));
+    for (int a = 0; a < ncomp; ++a)
+      for (int p = 0; p < np; ++p)
+        L.ptr[a][p] = reinterpret_cast<double*>(reinterpret_cast<char*>(L.slab) +
+                                                (size_t)elem * L.off[a][p]);
+    uploadTables(L, ncomp);
+  }
+}
+
+PatchDataSet::PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost,
+                           const std::vector<double*>& arrays)
+    : d_h(h), d_centring(c), d_ghost(ghost), d_elem(8), d_external(true) {
+  if (!h.finalized()) fail("PatchDataSet: hierarchy not finalized");
+  if (ghost < 0) fail("PatchDataSet: negative ghost width");
+  const int dim = h.dim();
+  const int ncomp = c == kCell ? 1 : dim;
+  size_t need = 0;
+  for (int ln = 0; ln < h.numberOfLevels(); ++ln) need += h.level(ln).boxes.size() * ncomp;
+  if (arrays.size() != need)
+    fail("PatchDataSet: " + std::to_string(arrays.size()) + " arrays given, the hierarchy has " +
+         std::to_string(need) + " patch arrays");
+  size_t k = 0;
+  d_levels.resize(h.numberOfLevels());
+  for (int ln = 0; ln < h.numberOfLevels(); ++ln) {
+    Lvl& L = d_levels[ln];
+    const auto& boxes = h.level(ln).boxes;
+    const int np = (int)boxes.size();
+    L.len.assign(ncomp, std::vector<long>(np));
+    L.ptr.assign(ncomp, std::vector<double*>(np));
     for (int p = 0; p < np; ++p)
-      for (int a = 0; a < ncomp; ++a) il[(size_t)p * ncomp + a] = at(L.off[a][p]);
-    hip_check(hipMalloc(&L.dil, sizeof(double*) * (il.size() + 1)), "hipMalloc(tab)");
-    if (!il.empty())
-      hip_check(hipMemcpy(L.dil, il.data(), sizeof(double*) * il.size(), hipMemcpyHostToDevice),
+      for (int a = 0; a < ncomp; ++a, ++k) {
+        long n = 1;
+        for (int d = 0; d < dim; ++d)
+          n *= boxes[p].hi[d] - boxes[p].lo[d] + 1 + 2 * ghost + (c == kSide && d == a ? 1 : 0);
+        L.len[a][p] = n;
+        double* q = arrays[k];
+        if (!q) fail("PatchDataSet: null array for level " + std::to_string(ln) + " patch " +
+                     std::to_string(p));
+        if (reinterpret_cast<uintptr_t>(q) % sizeof(double))
+          fail("PatchDataSet: array of level " + std::to_string(ln) + " patch " +
+               std::to_string(p) + " is not 8-byte aligned");
+        L.ptr[a][p] = q;
+      }
+  }
+  // device tables only once every array checked (a throw leaves nothing
+  // allocated)
+  for (Lvl& L : d_levels) uploadTables(L, ncomp);
+}
+
+void PatchDataSet::uploadTables(Lvl& L, int ncomp) {
+  const int np = ncomp ? (int)L.ptr[0].size() : 0;
+  for (int a = 0; a < ncomp; ++a) {
+    hip_check(hipMalloc(&L.dtab[a], sizeof(double*) * (np > 0 ? np : 1)), "hipMalloc(tab)");
+    if (np)
+      hip_check(hipMemcpy(L.dtab[a], L.ptr[a].data(), sizeof(double*) * np,
+                          hipMemcpyHostToDevice),
                 "hipMemcpy(tab)");
   }
+  std::vector<double*> il((size_t)np * ncomp);
+  for (int p = 0; p < np; ++p)
+    for (int a = 0; a < ncomp; ++a) il[(size_t)p * ncomp + a] = L.ptr[a][p];
+  hip_check(hipMalloc(&L.dil, sizeof(double*) * (il.size() + 1)), "hipMalloc(tab)");
+  if (!il.empty())
+    hip_check(hipMemcpy(L.dil, il.data(), sizeof(double*) * il.size(), hipMemcpyHostToDevice),
+              "hipMemcpy(tab)");
 }
 
 PatchDataSet::~PatchDataSet() {
   for (Lvl& L : d_levels) {
-    (void)hipFree(L.slab);
+    if (L.slab) (void)hipFree(L.slab);
     for (int a = 0; a < 3; ++a)
       if (L.dtab[a]) (void)hipFree(L.dtab[a]);
     if (L.dil) (void)hipFree(L.dil);
   }
 }
 
-double* PatchDataSet::pointer(int ln, int p, int a) const {
-  return reinterpret_cast<double*>(reinterpret_cast<char*>(d_levels[ln].slab) +
-                                   (size_t)d_elem * d_levels[ln].off[a][p]);
+double* PatchDataSet::slab(int ln) const {
+  if (d_external) fail("PatchDataSet: caller-owned patch arrays have no level slab");
+  return d_levels[ln].slab;
 }
+
+void PatchDataSet::fill(double v, hipStream_t s) const {
+  if (d_elem != 8 && v != 0.0) fail("PatchDataSet: low-precision data is only set to 0");
+  for (int ln = 0; ln < (int)d_levels.size(); ++ln) {
+    const Lvl& L = d_levels[ln];
+    // a low-precision slab is zeroed as doubles (slabs hold a multiple of
+    // 32 values, so its byte size is a multiple of 8)
+    if (!d_external) {
+      sfac::launch_fill(L.slab, L.total * d_elem / 8, v, s);
+      continue;
+    }
+    for (size_t a = 0; a < L.ptr.size(); ++a)
+      for (size_t p = 0; p < L.ptr[a].size(); ++p) sfac::launch_fill(L.ptr[a][p], L.len[a][p], v, s);
+  }
+}
+
+void PatchDataSet::copyFrom(const PatchDataSet& src, int coarse_ln, int fine_ln, hipStream_t s) {
+  if (&src.d_h != &d_h || src.d_centring != d_centring || src.d_ghost != d_ghost ||
+      src.d_elem != d_elem)
+    fail("PatchDataSet::copyFrom: different hierarchy, centring, ghost width or element size");
+  for (int ln = coarse_ln; ln <= fine_ln; ++ln) {
+    const Lvl& D = d_levels[ln];
+    const Lvl& S = src.d_levels[ln];
+    if (!d_external && !src.d_external) {
+      hip_check(hipMemcpyAsync(D.slab, S.slab, (size_t)d_elem * D.total, hipMemcpyDeviceToDevice,
+                               s),
+                "hipMemcpyAsync(copy)");
+      continue;
+    }
+    for (size_t a = 0; a < D.ptr.size(); ++a)
+      for (size_t p = 0; p < D.ptr[a].size(); ++p)
+        hip_check(hipMemcpyAsync(D.ptr[a][p], S.ptr[a][p], (size_t)d_elem * D.len[a][p],
+                                 hipMemcpyDeviceToDevice, s),
+                  "hipMemcpyAsync(copy)");
+  }
+}
+
+double* PatchDataSet::pointer(int ln, int p, int a) const { return d_levels[ln].ptr[a][p]; }
 long PatchDataSet::size(int ln, int p, int a) const { return d_levels[ln].len[a][p]; }
 
 }  // namespace pdat
@@ -376,6 +460,10 @@ void CellPoissonFACOps::setupFusedLevels(int rg) {
   const int nl = d_h.numberOfLevels();
   d_fused.assign(nl, FusedLevel{});
   if (dim != 3) return;
+  // the fused sweeps address D and C through level slabs
+  if ((!d_spec.dIsConstant() && !d_spec.getD()->contiguous()) ||
+      (d_spec.cMode() == 2 && !d_spec.getC()->contiguous()))
+    return;
   bool any = false;
   for (int ln = std::max(d_ln_min + 1, 1); ln <= d_ln_max; ++ln) {
     const hier::PatchLevel& L = d_h.level(ln);
@@ -1039,6 +1127,7 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
     return smoothBox(e, r, num_sweeps, tol, maxres_out);
   if (e.elemBytes() != 8) return smoothColourLP(e, r, ln, num_sweeps, tol, maxres_out);
   if (tol < 0.0 && fusedLevelActive(ln) && e.ghost() == 1 && e.centring() == PatchDataSet::kCell &&
+      e.contiguous() && r.contiguous() &&
       r.centring() == PatchDataSet::kCell && r.ghost() == d_fused[ln].rg &&
       &e != d_fused_tmp.get() && &r != d_fused_tmp.get() &&
       (d_spec.dIsConstant() || d_spec.getD()->ghost() == d_fused[ln].dg) &&
@@ -1152,9 +1241,19 @@ void CellPoissonFACOps::prepareMixedSolve() {
     d_D_lp.reset(new PatchDataSet(d_h, PatchDataSet::kSide, D.ghost(), d_mixed_elem));
     ++d_generation;
   }
-  // identical layouts: one conversion per level slab
-  for (int ln = 0; ln < d_h.numberOfLevels(); ++ln)
-    sfac::launch_convert(d_mixed_elem, D.slab(ln), d_D_lp->slab(ln), D.slabSize(ln), d_stream);
+  // identical layouts: one conversion per level slab (per patch array for
+  // caller-owned D)
+  for (int ln = 0; ln < d_h.numberOfLevels(); ++ln) {
+    if (D.contiguous()) {
+      sfac::launch_convert(d_mixed_elem, D.slab(ln), d_D_lp->slab(ln), D.slabSize(ln), d_stream);
+      continue;
+    }
+    const int np = d_h.level(ln).boxes.size();
+    for (int a = 0; a < d_h.dim(); ++a)
+      for (int p = 0; p < np; ++p)
+        sfac::launch_convert(d_mixed_elem, D.pointer(ln, p, a), d_D_lp->pointer(ln, p, a),
+                             D.size(ln, p, a), d_stream);
+  }
 }
 
 sfac::PoissonLPRaw CellPoissonFACOps::poissonLP(int ln) const {
@@ -1491,6 +1590,14 @@ void CellPoissonFACOps::computeVectorWeights(PatchDataSet& w, int coarsest_ln, i
 void CellPoissonFACOps::setToScalar(PatchDataSet& x, double v, int coarse_ln, int fine_ln) {
   if (x.elemBytes() != 8 && v != 0.0)
     fail("CellPoissonFACOps: low-precision vectors are only set to 0");
+  if (!x.contiguous()) {
+    // caller-owned arrays: each one whole
+    for (int ln = coarse_ln; ln <= fine_ln; ++ln) {
+      const int np = (int)d_h.level(ln).boxes.size();
+      for (int p = 0; p < np; ++p) sfac::launch_fill(x.pointer(ln, p), x.size(ln, p), v, d_stream);
+    }
+    return;
+  }
   // a low-precision slab is zeroed as doubles (slabs hold a multiple of 32
   // values, so its byte size is a multiple of 8)
   for (int ln = coarse_ln; ln <= fine_ln; ++ln)

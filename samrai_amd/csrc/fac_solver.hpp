@@ -33,6 +33,13 @@ class PatchDataSet {
   // the solver's mixed-precision error vector, 4 (float) or 2 (bfloat16); the
   // pointer tables are then typed double* only nominally.
   PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost, int elem = 8);
+  // FP64 arrays the caller owns, in ArrayData layout with ghost width
+  // `ghost`, bound without copying: arrays[(n + p) * ncomp + a] is component
+  // a of patch p of level ln, n = the number of patches on the levels below
+  // ln, ncomp = 1 (cell) or dim (side).  The caller keeps them allocated
+  // while this object (and every solver bound to it) lives.
+  PatchDataSet(const hier::PatchHierarchy& h, Centring c, int ghost,
+               const std::vector<double*>& arrays);
   ~PatchDataSet();
   PatchDataSet(const PatchDataSet&) = delete;
   PatchDataSet& operator=(const PatchDataSet&) = delete;
@@ -49,7 +56,15 @@ class PatchDataSet {
   double* const* table(int ln, int a = 0) const { return d_levels[ln].dtab[a]; }
   // Device table indexed [p * ncomp + a].
   double* const* interleaved(int ln) const { return d_levels[ln].dil; }
-  double* slab(int ln) const { return d_levels[ln].slab; }
+  // One library-owned allocation per level holding every patch array
+  // (false for caller-owned arrays, which have no slab).
+  bool contiguous() const { return !d_external; }
+  double* slab(int ln) const;
+  // Every value of every patch array (ghosts included) = v, on stream s.
+  void fill(double v, hipStream_t s) const;
+  // Every patch array = src's (same hierarchy, centring, ghost width and
+  // element size), levels [coarse_ln, fine_ln], on stream s.
+  void copyFrom(const PatchDataSet& src, int coarse_ln, int fine_ln, hipStream_t s);
   // Element offsets [a][p] of every patch array of level ln in its slab, the
   // same for every PatchDataSet of this hierarchy, centring and ghost width.
   static std::vector<std::vector<long>> layout(const hier::PatchHierarchy& h, Centring c,
@@ -62,6 +77,7 @@ class PatchDataSet {
     long total = 0;
     std::vector<std::vector<long>> off;  // [a][p]
     std::vector<std::vector<long>> len;
+    std::vector<std::vector<double*>> ptr;  // [a][p] host copy of the tables
     double** dtab[3] = {nullptr, nullptr, nullptr};
     double** dil = nullptr;
   };
@@ -69,7 +85,9 @@ class PatchDataSet {
   Centring d_centring;
   int d_ghost;
   int d_elem;
+  bool d_external = false;
   std::vector<Lvl> d_levels;
+  void uploadTables(Lvl& L, int ncomp);
 };
 
 }  // namespace pdat

@@ -50,6 +50,10 @@ def lib():
     L.sfac_hierarchy_destroy.argtypes = [_vp]
     L.sfac_cell_data_create.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
     L.sfac_side_data_create.argtypes = [_vp, C.c_int, C.POINTER(_vp)]
+    L.sfac_cell_data_wrap.argtypes = [_vp, C.c_int, C.POINTER(C.c_void_p), C.c_long,
+                                      C.POINTER(_vp)]
+    L.sfac_side_data_wrap.argtypes = [_vp, C.c_int, C.POINTER(C.c_void_p), C.c_long,
+                                      C.POINTER(_vp)]
     L.sfac_patch_data_destroy.argtypes = [_vp]
     L.sfac_patch_data_pointer.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.POINTER(_dp),
                                           C.POINTER(C.c_long)]
@@ -174,6 +178,25 @@ class PatchData:
         fn = lib().sfac_cell_data_create if centring == "cell" else lib().sfac_side_data_create
         _check(fn(hierarchy.h, ghost, C.byref(d)))
         self.d = d
+
+    @classmethod
+    def wrap(cls, hierarchy, ghost, arrays, centring="cell"):
+        """Patch data over device arrays the caller owns, bound without
+        copying (sfac_cell_data_wrap / sfac_side_data_wrap): `arrays` holds
+        one contiguous float64 device tensor per patch array (per patch, per
+        axis for side data), levels in order, each the whole ghost box with i
+        fastest.  The tensors are kept referenced by the returned object."""
+        for t in arrays:
+            if t.element_size() != 8 or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError("wrap needs contiguous float64 device tensors")
+        ptrs = (C.c_void_p * max(1, len(arrays)))(*[t.data_ptr() for t in arrays])
+        d = _vp()
+        fn = lib().sfac_cell_data_wrap if centring == "cell" else lib().sfac_side_data_wrap
+        _check(fn(hierarchy.h, ghost, ptrs, len(arrays), C.byref(d)))
+        self = cls(hierarchy, ghost, centring, _handle=d)
+        self.owned = True
+        self.arrays = list(arrays)
+        return self
 
     def pointer(self, ln, p, axis=0):
         ptr = _dp()
