@@ -91,15 +91,43 @@ __device__ inline Coef<T> load_coef(const BoxLevel& G, const BoxArraysT<T>& A, i
   return q;
 }
 
-// Gauss-Seidel update of one cell held in LDS at q (neighbours at q -+ 1,
-// q -+ SY, q - SZd and q + SZu), in the expression order of
-// rbgswithfluxmax*.  Returns |residual|.
+// 1 / diagonal of a cell's stencil, in the expression order of
+// rbgswithfluxmax*.  It depends only on the cell's coefficients, which do
+// not change during a solve: callers compute it once per cell (once per
+// launch when D and C are constant) and pass it to cell_update, with the
+// same bits as forming it inside every update.
 template <int DIM, bool DVAR, int CMODE, typename T>
-__device__ inline T cell_update(T* U, int q, int SY, int SZd, int SZu, const Coef<T>& k,
-                                const BoxLevel& G, const BoxArraysT<T>& A) {
+__device__ inline T cell_dudr(const Coef<T>& k, const BoxLevel& G, const BoxArraysT<T>& A) {
   const T dxi = (T)(1.0 / G.dx[0]), dyi = (T)(1.0 / G.dx[1]);
   const T dzi = DIM == 3 ? (T)(1.0 / G.dx[2]) : (T)0;
   const T dxi2 = dxi * dxi, dyi2 = dyi * dyi, dzi2 = dzi * dzi;
+  const T dc = (T)A.dconst;
+  T scal;
+  if (CMODE == 2) scal = k.c;
+  else scal = CMODE == 1 ? (T)A.cconst : (T)0;
+  T diag;
+  if (DIM == 3) {
+    if (DVAR)
+      diag = (dxi2 * (k.xh + k.xl) + dyi2 * (k.yh + k.yl) + dzi2 * (k.zh + k.zl)) - scal;
+    else
+      diag = (dxi2 * (dc + dc) + dyi2 * (dc + dc) + dzi2 * (dc + dc)) - scal;
+  } else {
+    if (DVAR)
+      diag = (dxi * dxi * (k.xh + k.xl) + dyi * dyi * (k.yh + k.yl)) - scal;
+    else
+      diag = (dxi * dxi * (dc + dc) + dyi * dyi * (dc + dc)) - scal;
+  }
+  return (T)1 / diag;
+}
+
+// Gauss-Seidel update of one cell held in LDS at q (neighbours at q -+ 1,
+// q -+ SY, q - SZd and q + SZu), in the expression order of
+// rbgswithfluxmax*, with dudr = cell_dudr of the cell.  Returns |residual|.
+template <int DIM, bool DVAR, int CMODE, typename T>
+__device__ inline T cell_update(T* U, int q, int SY, int SZd, int SZu, const Coef<T>& k,
+                                const BoxLevel& G, const BoxArraysT<T>& A, T dudr) {
+  const T dxi = (T)(1.0 / G.dx[0]), dyi = (T)(1.0 / G.dx[1]);
+  const T dzi = DIM == 3 ? (T)(1.0 / G.dx[2]) : (T)0;
   const T dc = (T)A.dconst;
   const T uc = U[q];
   T Dxl, Dxh, Dyl, Dyh, Dzl = 0, Dzh = 0;
@@ -128,21 +156,11 @@ __device__ inline T cell_update(T* U, int q, int SY, int SZd, int SZu, const Coe
   if (CMODE == 2) scal = k.c;
   else scal = CMODE == 1 ? (T)A.cconst : (T)0;
   const T fv = k.f;
-  T residual, diag;
-  if (DIM == 3) {
+  T residual;
+  if (DIM == 3)
     residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
-    if (DVAR)
-      diag = (dxi2 * (Dxh + Dxl) + dyi2 * (Dyh + Dyl) + dzi2 * (Dzh + Dzl)) - scal;
-    else
-      diag = (dxi2 * (dc + dc) + dyi2 * (dc + dc) + dzi2 * (dc + dc)) - scal;
-  } else {
+  else
     residual = fv - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
-    if (DVAR)
-      diag = (dxi * dxi * (Dxh + Dxl) + dyi * dyi * (Dyh + Dyl)) - scal;
-    else
-      diag = (dxi * dxi * (dc + dc) + dyi * dyi * (dc + dc)) - scal;
-  }
-  const T dudr = (T)1 / diag;
   const T du = -residual * dudr;
   U[q] = uc + du * (T)1;
   return fabs(residual);
@@ -330,12 +348,15 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
   __syncthreads();
 
   double mr = 0.0;
+  // constant D and C: one reciprocal diagonal for the whole level
+  const T dudr_c = (!DVAR && CMODE != 2) ? cell_dudr<DIM, DVAR, CMODE>(Coef<T>{}, G, A) : (T)0;
 
   // 3. red cells of the tile grown by one
 #pragma unroll
   for (int u = 0; u < NR; ++u) {
     if (rf[u] & 1) {
-      const T ra = cell_update<DIM, DVAR, CMODE>(U, rq[u], SY, SZ, SZ, cr[u], G, A);
+      const T dudr = (!DVAR && CMODE != 2) ? dudr_c : cell_dudr<DIM, DVAR, CMODE>(cr[u], G, A);
+      const T ra = cell_update<DIM, DVAR, CMODE>(U, rq[u], SY, SZ, SZ, cr[u], G, A, dudr);
       if (rf[u] & 2) mr = fmax(mr, (double)ra);
     }
   }
@@ -347,7 +368,8 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
 #pragma unroll
   for (int u = 0; u < NB; ++u) {
     if (bf[u] & 1) {
-      const T ra = cell_update<DIM, DVAR, CMODE>(U, bq[u], SY, SZ, SZ, cb[u], G, A);
+      const T dudr = (!DVAR && CMODE != 2) ? dudr_c : cell_dudr<DIM, DVAR, CMODE>(cb[u], G, A);
+      const T ra = cell_update<DIM, DVAR, CMODE>(U, bq[u], SY, SZ, SZ, cb[u], G, A, dudr);
       if (bf[u] & 2) mr = fmax(mr, (double)ra);
     }
   }
@@ -694,6 +716,7 @@ k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, con
   for (int q = threadIdx.x; q < (DIM == 3 ? L * L * L : L * L); q += kThreads) U[q] = (T)0;
   int qc[NC], col[NC];
   Coef<T> cf[NC];
+  T dudr[NC];
 #pragma unroll
   for (int v = 0; v < NC; ++v) {
     const int t = threadIdx.x + v * kThreads;
@@ -703,6 +726,7 @@ k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, con
       qc[v] = (i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0);
       col[v] = (i + j + k) & 1;
       cf[v] = load_coef<DIM, DVAR, CMODE>(G, A, i, j, k);
+      dudr[v] = cell_dudr<DIM, DVAR, CMODE>(cf[v], G, A);
     }
   }
   __syncthreads();
@@ -728,7 +752,7 @@ k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, con
 #pragma unroll
       for (int v = 0; v < NC; ++v)
         if (qc[v] >= 0 && col[v] == colour)
-          cell_update<DIM, DVAR, CMODE>(U, qc[v], SY, SZ, SZ, cf[v], G, A);
+          cell_update<DIM, DVAR, CMODE>(U, qc[v], SY, SZ, SZ, cf[v], G, A, dudr[v]);
       __syncthreads();
     }
 #pragma unroll

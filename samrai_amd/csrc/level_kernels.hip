@@ -75,14 +75,6 @@ __device__ inline long cf_pos(const LevelGeom& L, int face, const int* c, const 
   return idx;
 }
 
-// Is the ghost cell across face `face` of patch p, next to interior cell c,
-// a coarse-fine ghost?
-__device__ inline bool cf_flag(const LevelGeom& L, int p, int face, const int* c, const Box& b) {
-  long off = L.cf_off[p * 6 + face];
-  if (off < 0) return false;
-  return L.cf[off + cf_pos(L, face, c, b)] != 0;
-}
-
 // Index of interior cell c of face `face` in the per-face tables, or -1 when
 // the ghost across that face is not coarse-fine.
 __device__ inline long cf_index(const LevelGeom& L, int p, int face, const int* c, const Box& b) {
@@ -90,48 +82,6 @@ __device__ inline long cf_index(const LevelGeom& L, int p, int face, const int* 
   if (off < 0) return -1;
   const long idx = off + cf_pos(L, face, c, b);
   return L.cf[idx] ? idx : -1;
-}
-
-// Flux D grad u on the face of axis a with face index c (between c-e_a and
-// c), with the Ewing coarse-fine correction on flagged patch-boundary faces
-// (solv_cellpoissonfacops3d.f:10-74 and 604-1011), in FP64 from u in
-// storage FP64, FP32 or BF16 (P a global pointer to it).
-template <int DIM, bool DVAR, class P>
-__device__ inline double face_flux(const LevelGeom& L, int p, const Box& b, P u, const View& vu,
-                                   const double* const* Dp, const View* vd, double dconst, int a,
-                                   int i, int j, int k, double dai) {
-  int c[3] = {i, j, k};
-  int m[3] = {i, j, k};
-  m[a] -= 1;
-  double D = DVAR ? Dp[a][vd[a].at(i, j, k)] : dconst;
-  double F = dai * D * (ld64(u, vu.at(i, j, k)) - ld64(u, vu.at(m[0], m[1], m[2])));
-  int side = c[a] == b.lo[a] ? 0 : (c[a] == b.hi[a] + 1 ? 1 : -1);
-  if (side >= 0) {
-    // the interior cell next to the face, by value (a pointer chosen at run
-    // time between two local arrays would force them into scratch memory)
-    int in[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) in[d] = side == 0 ? c[d] : m[d];
-    if (cf_flag(L, p, 2 * a + side, in, b)) {
-      int r = L.ratio;
-      int q[3] = {in[0], in[1], in[2]};
-      int t1, t2;
-      tangential(a, DIM, t1, t2);
-      int bs1 = in[t1] - (in[t1] - cdiv(in[t1], r) * r);
-      q[t1] = bs1 + (r - 1 - (in[t1] - bs1));
-      if (DIM == 3) {
-        int bs2 = in[t2] - (in[t2] - cdiv(in[t2], r) * r);
-        q[t2] = bs2 + (r - 1 - (in[t2] - bs2));
-      }
-      double tranwt = ewing_tranwt(r);
-      double longwt = 2 * tranwt;
-      double h = L.dx[a];
-      double tv = tranwt * D *
-                  (ld64(u, vu.at(q[0], q[1], q[2])) - ld64(u, vu.at(in[0], in[1], in[2]))) / h;
-      F = side == 0 ? longwt * F + tv : longwt * F - tv;
-    }
-  }
-  return F;
 }
 
 // Coarse-solve convergence test of smoothErrorByRedBlack
@@ -406,40 +356,51 @@ k_ewing_faces(const BoxItem* items, const Chunk* chunks, LevelGeom L, const T* c
   const BoxItem it = items[ch.item];
   const int p = it.p, face = it.q, a = face >> 1, side = face & 1;
   const Box b = L.box[p];
-  const View vu = vview(b, ug, DIM);
-  View vd[3];
-  const double* Dp[3] = {nullptr, nullptr, nullptr};
-  if (DVAR) {
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-      vd[d] = vview(b, ps.dghost, DIM, d);
-      Dp[d] = ps.D[d][p];
-    }
-  }
+  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
+  const int us1 = nx + 2 * ug, us2 = us1 * (ny + 2 * ug);
+  const int sa = a == 0 ? 1 : (a == 1 ? us1 : us2);  // stride of u along the face normal
+  // D of axis a: side data with ghost dghost, one more entry along a
+  const int dg = ps.dghost;
+  const int ds1 = nx + 2 * dg + (a == 0 ? 1 : 0);
+  const int ds2 = ds1 * (ny + 2 * dg + (a == 1 ? 1 : 0));
   const auto u = gmem(uu[p]);
+  const auto Da = gmem(DVAR ? (a == 0 ? ps.D[0][p] : (a == 1 ? ps.D[1][p] : ps.D[2][p]))
+                            : nullptr);
+  const long off = L.cf_off[p * 6 + face];
   const double dai = 1.0 / (a == 0 ? L.dx[0] : (a == 1 ? L.dx[1] : L.dx[2]));
+  const double h = a == 0 ? L.dx[0] : (a == 1 ? L.dx[1] : L.dx[2]);
+  const int r = L.ratio;
+  const double tranwt = ewing_tranwt(r);
+  const double longwt = 2 * tranwt;
   for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
     int c[3];
     if (!decode_box(it, (long)ch.start + t, c)) break;
-    long x;
-    if (all) {
-      x = L.cf_off[p * 6 + face] + cf_pos(L, face, c, b);
-    } else {
-      x = cf_index(L, p, face, c, b);
-      if (x < 0) continue;
+    // the interior cell (ii,jj,kk) next to the face, patch-relative
+    const int ii = c[0] - b.lo[0], jj = c[1] - b.lo[1], kk = DIM == 3 ? c[2] - b.lo[2] : 0;
+    const int pos = a == 0 ? jj + ny * kk : (a == 1 ? ii + nx * kk : ii + nx * jj);
+    const long x = off + pos;
+    const bool cfc = L.cf[x] != 0;
+    if (!all && !cfc) continue;
+    const int o = (ii + ug) + us1 * (jj + ug) + us2 * (DIM == 3 ? kk + ug : 0);
+    // face index along a: the cell's own for the lower face, +1 for the upper
+    const int fi = ii + (a == 0 ? side : 0), fj = jj + (a == 1 ? side : 0);
+    const int fk = kk + (a == 2 ? side : 0);
+    const double D =
+        DVAR ? Da[(fi + dg) + ds1 * (fj + dg) + ds2 * (DIM == 3 ? fk + dg : 0)] : ps.dconst;
+    const double uin = ld64(u, o), ug_ = ld64(u, side ? o + sa : o - sa);
+    // dai * D * (u(face index) - u(face index - e_a))
+    double F = side ? dai * D * (ug_ - uin) : dai * D * (uin - ug_);
+    if (cfc) {
+      // Ewing fix: the interior cell reflected tangentially in its coarse cell
+      auto mirror = [&](int ct) {
+        const int bs = ct - (ct - cdiv(ct, r) * r);
+        return bs + (r - 1 - (ct - bs)) - ct;
+      };
+      int q = o + mirror(a == 0 ? c[1] : c[0]) * (a == 0 ? us1 : 1);
+      if (DIM == 3) q += mirror(a == 2 ? c[1] : c[2]) * (a == 2 ? us1 : us2);
+      const double tv = tranwt * D * (ld64(u, q) - uin) / h;
+      F = side == 0 ? longwt * F + tv : longwt * F - tv;
     }
-    // one call per axis, each with a constant axis (a run-time axis index
-    // into the local arrays would put them in scratch memory)
-    double F;
-    if (a == 0)
-      F = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 0, c[0] + side, c[1], c[2],
-                               dai);
-    else if (a == 1)
-      F = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 1, c[0], c[1] + side, c[2],
-                               dai);
-    else
-      F = face_flux<DIM, DVAR>(L, p, b, u, vu, Dp, vd, ps.dconst, 2, c[0], c[1], c[2] + side,
-                               dai);
     L.cf_flux[x] = F;
   }
 }
