@@ -123,13 +123,31 @@ __device__ inline T cell_dudr(const Coef<T>& k, const BoxLevel& G, const BoxArra
 // Gauss-Seidel update of one cell held in LDS at q (neighbours at q -+ 1,
 // q -+ SY, q - SZd and q + SZu), in the expression order of
 // rbgswithfluxmax*, with dudr = cell_dudr of the cell.  Returns |residual|.
+// bmask (optional): bit 2a+side set when the face of axis a on that side
+// is on the physical boundary; that neighbour is then the Robin ghost of the
+// cell's own value, formed here (the value a ghost fill of the current state
+// writes there, since the cell is the ghost's only interior neighbour)
+// instead of read from U.
 template <int DIM, bool DVAR, int CMODE, typename T>
 __device__ inline T cell_update(T* U, int q, int SY, int SZd, int SZu, const Coef<T>& k,
-                                const BoxLevel& G, const BoxArraysT<T>& A, T dudr) {
+                                const BoxLevel& G, const BoxArraysT<T>& A, T dudr,
+                                int bmask = 0, const RobinDev* bc = nullptr,
+                                int homogeneous = 1) {
   const T dxi = (T)(1.0 / G.dx[0]), dyi = (T)(1.0 / G.dx[1]);
   const T dzi = DIM == 3 ? (T)(1.0 / G.dx[2]) : (T)0;
   const T dc = (T)A.dconst;
   const T uc = U[q];
+  T uxl = U[q - 1], uxh = U[q + 1], uyl = U[q - SY], uyh = U[q + SY];
+  T uzl = DIM == 3 ? U[q - SZd] : (T)0, uzh = DIM == 3 ? U[q + SZu] : (T)0;
+  if (bmask != 0) {
+    // a branch, not selects: few cells are on the boundary
+    if (bmask & 1) uxl = robin_ghost<DIM>(G, *bc, homogeneous, 0, uc);
+    if (bmask & 2) uxh = robin_ghost<DIM>(G, *bc, homogeneous, 1, uc);
+    if (bmask & 4) uyl = robin_ghost<DIM>(G, *bc, homogeneous, 2, uc);
+    if (bmask & 8) uyh = robin_ghost<DIM>(G, *bc, homogeneous, 3, uc);
+    if (DIM == 3 && (bmask & 16)) uzl = robin_ghost<DIM>(G, *bc, homogeneous, 4, uc);
+    if (DIM == 3 && (bmask & 32)) uzh = robin_ghost<DIM>(G, *bc, homogeneous, 5, uc);
+  }
   T Dxl, Dxh, Dyl, Dyh, Dzl = 0, Dzh = 0;
   if (DVAR) {
     Dxl = k.xl;
@@ -143,14 +161,14 @@ __device__ inline T cell_update(T* U, int q, int SY, int SZd, int SZu, const Coe
   } else {
     Dxl = Dxh = Dyl = Dyh = Dzl = Dzh = dc;
   }
-  const T fxl = dxi * Dxl * (uc - U[q - 1]);
-  const T fxh = dxi * Dxh * (U[q + 1] - uc);
-  const T fyl = dyi * Dyl * (uc - U[q - SY]);
-  const T fyh = dyi * Dyh * (U[q + SY] - uc);
+  const T fxl = dxi * Dxl * (uc - uxl);
+  const T fxh = dxi * Dxh * (uxh - uc);
+  const T fyl = dyi * Dyl * (uc - uyl);
+  const T fyh = dyi * Dyh * (uyh - uc);
   T fzl = 0, fzh = 0;
   if (DIM == 3) {
-    fzl = dzi * Dzl * (uc - U[q - SZd]);
-    fzh = dzi * Dzh * (U[q + SZu] - uc);
+    fzl = dzi * Dzl * (uc - uzl);
+    fzh = dzi * Dzh * (uzh - uc);
   }
   T scal;
   if (CMODE == 2) scal = k.c;
@@ -694,10 +712,10 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArraysT<T> Af, BoxLevel Gc, T* Dc0, T* Dc1, T*
 }
 
 // The smallest grid (at most kMGSmallEdge cells per axis): u = 0, then
-// `sweeps` red-black sweeps with the whole grid and its ghosts in LDS and
-// each thread's cell coefficients in registers; one work-group of one
-// wavefront (one cell per lane), so its barriers cost no more than the LDS
-// wait they carry.
+// `sweeps` red-black sweeps with the whole grid in LDS, each thread's cell
+// coefficients and reciprocal diagonal in registers and the Robin ghosts
+// formed inside the updates; one work-group of one wavefront (one cell per
+// lane), one barrier per colour.
 constexpr int kMGSmallEdge = 4;
 constexpr int kMGSmallThreads = 64;
 template <int DIM, bool DVAR, int CMODE, typename T>
@@ -714,7 +732,7 @@ k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, con
   const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
   const int ncell = n0 * n1 * n2;
   for (int q = threadIdx.x; q < (DIM == 3 ? L * L * L : L * L); q += kThreads) U[q] = (T)0;
-  int qc[NC], col[NC];
+  int qc[NC], col[NC], bm[NC];
   Coef<T> cf[NC];
   T dudr[NC];
 #pragma unroll
@@ -727,32 +745,20 @@ k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, con
       col[v] = (i + j + k) & 1;
       cf[v] = load_coef<DIM, DVAR, CMODE>(G, A, i, j, k);
       dudr[v] = cell_dudr<DIM, DVAR, CMODE>(cf[v], G, A);
+      // every face of this grid is on the physical boundary: the Robin
+      // ghosts are formed inside the update, from the cell's own value
+      bm[v] = (i == 0 ? 1 : 0) | (i == n0 - 1 ? 2 : 0) | (j == 0 ? 4 : 0) |
+              (j == n1 - 1 ? 8 : 0) |
+              (DIM == 3 ? (k == 0 ? 16 : 0) | (k == n2 - 1 ? 32 : 0) : 0);
     }
   }
   __syncthreads();
-  auto ghosts = [&]() {
-#pragma unroll
-    for (int v = 0; v < NC; ++v) {
-      if (qc[v] < 0) continue;
-      const int t = threadIdx.x + v * kThreads;
-      const int i = t % n0, j = (t / n0) % n1, k = DIM == 3 ? t / (n0 * n1) : 0;
-      const int q = qc[v];
-      if (i == 0) U[q - 1] = robin_ghost<DIM>(G, bc, 1, 0, U[q]);
-      if (i == n0 - 1) U[q + 1] = robin_ghost<DIM>(G, bc, 1, 1, U[q]);
-      if (j == 0) U[q - SY] = robin_ghost<DIM>(G, bc, 1, 2, U[q]);
-      if (j == n1 - 1) U[q + SY] = robin_ghost<DIM>(G, bc, 1, 3, U[q]);
-      if (DIM == 3 && k == 0) U[q - SZ] = robin_ghost<DIM>(G, bc, 1, 4, U[q]);
-      if (DIM == 3 && k == n2 - 1) U[q + SZ] = robin_ghost<DIM>(G, bc, 1, 5, U[q]);
-    }
-  };
   for (int s = 0; s < sweeps; ++s)
     for (int colour = 0; colour < 2; ++colour) {
-      ghosts();
-      __syncthreads();
 #pragma unroll
       for (int v = 0; v < NC; ++v)
         if (qc[v] >= 0 && col[v] == colour)
-          cell_update<DIM, DVAR, CMODE>(U, qc[v], SY, SZ, SZ, cf[v], G, A, dudr[v]);
+          cell_update<DIM, DVAR, CMODE>(U, qc[v], SY, SZ, SZ, cf[v], G, A, dudr[v], bm[v], &bc);
       __syncthreads();
     }
 #pragma unroll
