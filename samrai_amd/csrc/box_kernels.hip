@@ -567,11 +567,10 @@ __device__ inline T mg_at(const BoxLevel& G, const RobinDev& bc, const T* u, int
 // Residual f - (div D grad u + C u) of one cell, in the smoother's
 // expression order (global-memory neighbours, Robin ghosts at the domain).
 template <int DIM, bool DVAR, int CMODE, typename T>
-__device__ inline T mg_residual(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc,
-                                const T* u, int i, int j, int k) {
+__device__ inline T mg_residual_q(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc,
+                                  const T* u, int i, int j, int k, const Coef<T>& q) {
   const int n0 = G.n[0], n1 = G.n[1];
   const long c = (long)i + (long)n0 * (j + (long)n1 * k);
-  const Coef<T> q = load_coef<DIM, DVAR, CMODE>(G, A, i, j, k);
   const T dxi = (T)(1.0 / G.dx[0]), dyi = (T)(1.0 / G.dx[1]);
   const T dzi = DIM == 3 ? (T)(1.0 / G.dx[2]) : (T)0;
   const T dc = (T)A.dconst;
@@ -606,9 +605,29 @@ __device__ inline T mg_residual(const BoxLevel& G, const BoxArraysT<T>& A, const
     return q.f - (dxi * (fxh - fxl) + dyi * (fyh - fyl) + dzi * (fzh - fzl)) - scal * uc;
   return q.f - (dxi * (fxh - fxl) + dyi * (fyh - fyl)) - scal * uc;
 }
+template <int DIM, bool DVAR, int CMODE, typename T>
+__device__ inline T mg_residual(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc,
+                                const T* u, int i, int j, int k) {
+  return mg_residual_q<DIM, DVAR, CMODE>(G, A, bc, u, i, j, k,
+                                         load_coef<DIM, DVAR, CMODE>(G, A, i, j, k));
+}
+
+// Mean over the 2^dim children of coarse cell (I, J, K) of the fine
+// residual (children summed in i, j, k order).
+template <int DIM, bool DVAR, int CMODE, typename T>
+__device__ inline T mg_restrict_value(const BoxLevel& Gf, const BoxArraysT<T>& A,
+                                      const RobinDev& bc, const T* u, int I, int J, int K) {
+  T sum = 0;
+  for (int c2 = 0; c2 < (DIM == 3 ? 2 : 1); ++c2)
+    for (int c1 = 0; c1 < 2; ++c1)
+      for (int c0 = 0; c0 < 2; ++c0)
+        sum += mg_residual<DIM, DVAR, CMODE>(Gf, A, bc, u, 2 * I + c0, 2 * J + c1,
+                                             DIM == 3 ? 2 * K + c2 : 0);
+  return sum * (T)(DIM == 3 ? 0.125 : 0.25);
+}
 
 // fc = mean over the 2^dim children of the fine residual (one thread per
-// coarse cell; children summed in i, j, k order).
+// coarse cell).
 template <int DIM, bool DVAR, int CMODE, typename T>
 __global__ void __launch_bounds__(kThreads)
 k_mg_restrict_residual(BoxLevel Gf, BoxArraysT<T> A, RobinDev bc, const T* u, BoxLevel Gc, T* fc,
@@ -620,14 +639,32 @@ k_mg_restrict_residual(BoxLevel Gf, BoxArraysT<T> A, RobinDev bc, const T* u, Bo
     const int I = (int)(t % Gc.n[0]);
     const int J = (int)((t / Gc.n[0]) % Gc.n[1]);
     const int K = DIM == 3 ? (int)(t / ((long)Gc.n[0] * Gc.n[1])) : 0;
-    T sum = 0;
-    for (int c2 = 0; c2 < (DIM == 3 ? 2 : 1); ++c2)
-      for (int c1 = 0; c1 < 2; ++c1)
-        for (int c0 = 0; c0 < 2; ++c0)
-          sum += mg_residual<DIM, DVAR, CMODE>(Gf, A, bc, u, 2 * I + c0, 2 * J + c1,
-                                               DIM == 3 ? 2 * K + c2 : 0);
-    fc[t] = sum * (T)(DIM == 3 ? 0.125 : 0.25);
+    fc[t] = mg_restrict_value<DIM, DVAR, CMODE>(Gf, A, bc, u, I, J, K);
   }
+}
+
+// Separable linear prolongation of the coarse error to fine cell (i, j, k):
+// weights 3/4 (the parent) and 1/4 (its neighbour towards the child) per
+// axis, homogeneous Robin ghosts beyond the domain.
+template <int DIM, typename T>
+__device__ inline T mg_prolong_value(const BoxLevel& Gc, const RobinDev& bc, const T* ec, int i,
+                                     int j, int k) {
+  const int I = i >> 1, J = j >> 1, K = k >> 1;
+  const int sx = (i & 1) ? 1 : -1, sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
+  const T w3 = (T)0.75, w1 = (T)0.25;
+  if (DIM == 3) {
+    T z0 = w3 * (w3 * mg_at<3>(Gc, bc, ec, I, J, K) + w1 * mg_at<3>(Gc, bc, ec, I + sx, J, K)) +
+           w1 * (w3 * mg_at<3>(Gc, bc, ec, I, J + sy, K) +
+                 w1 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K));
+    T z1 = w3 * (w3 * mg_at<3>(Gc, bc, ec, I, J, K + sz) +
+                 w1 * mg_at<3>(Gc, bc, ec, I + sx, J, K + sz)) +
+           w1 * (w3 * mg_at<3>(Gc, bc, ec, I, J + sy, K + sz) +
+                 w1 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K + sz));
+    return w3 * z0 + w1 * z1;
+  }
+  return w3 * (w3 * mg_at<2>(Gc, bc, ec, I, J, 0) + w1 * mg_at<2>(Gc, bc, ec, I + sx, J, 0)) +
+         w1 * (w3 * mg_at<2>(Gc, bc, ec, I, J + sy, 0) +
+               w1 * mg_at<2>(Gc, bc, ec, I + sx, J + sy, 0));
 }
 
 // ef += linear interpolation of ec (one thread per fine cell).
@@ -641,25 +678,7 @@ k_mg_prolong_add(BoxLevel Gc, RobinDev bc, const T* ec, BoxLevel Gf, T* ef, cons
     const int i = (int)(t % Gf.n[0]);
     const int j = (int)((t / Gf.n[0]) % Gf.n[1]);
     const int k = DIM == 3 ? (int)(t / ((long)Gf.n[0] * Gf.n[1])) : 0;
-    const int I = i >> 1, J = j >> 1, K = k >> 1;
-    const int sx = (i & 1) ? 1 : -1, sy = (j & 1) ? 1 : -1, sz = (k & 1) ? 1 : -1;
-    const T w3 = (T)0.75, w1 = (T)0.25;
-    T v;
-    if (DIM == 3) {
-      T z0 = w3 * (w3 * mg_at<3>(Gc, bc, ec, I, J, K) + w1 * mg_at<3>(Gc, bc, ec, I + sx, J, K)) +
-             w1 * (w3 * mg_at<3>(Gc, bc, ec, I, J + sy, K) +
-                   w1 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K));
-      T z1 = w3 * (w3 * mg_at<3>(Gc, bc, ec, I, J, K + sz) +
-                   w1 * mg_at<3>(Gc, bc, ec, I + sx, J, K + sz)) +
-             w1 * (w3 * mg_at<3>(Gc, bc, ec, I, J + sy, K + sz) +
-                   w1 * mg_at<3>(Gc, bc, ec, I + sx, J + sy, K + sz));
-      v = w3 * z0 + w1 * z1;
-    } else {
-      v = w3 * (w3 * mg_at<2>(Gc, bc, ec, I, J, 0) + w1 * mg_at<2>(Gc, bc, ec, I + sx, J, 0)) +
-          w1 * (w3 * mg_at<2>(Gc, bc, ec, I, J + sy, 0) +
-                w1 * mg_at<2>(Gc, bc, ec, I + sx, J + sy, 0));
-    }
-    ef[t] = ef[t] + v;
+    ef[t] = ef[t] + mg_prolong_value<DIM>(Gc, bc, ec, i, j, k);
   }
 }
 
@@ -718,17 +737,18 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArraysT<T> Af, BoxLevel Gc, T* Dc0, T* Dc1, T*
 // lane), one barrier per colour.
 constexpr int kMGSmallEdge = 4;
 constexpr int kMGSmallThreads = 64;
-template <int DIM, bool DVAR, int CMODE, typename T>
-__global__ void __launch_bounds__(kMGSmallThreads)
-k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, const int* stop) {
-  constexpr int kThreads = kMGSmallThreads;
+constexpr int kMGSmallLds = (kMGSmallEdge + 2) * (kMGSmallEdge + 2) * (kMGSmallEdge + 2);
+// The solve itself, for a work-group of NT threads (NT >= the grid's cells
+// keeps one cell per thread), with U an LDS array of kMGSmallLds values.
+template <int DIM, bool DVAR, int CMODE, typename T, int NT>
+__device__ inline void mg_small_block(const BoxLevel& G, const BoxArraysT<T>& A,
+                                      const RobinDev& bc, T* u, int sweeps, T* U) {
+  constexpr int kThreads = NT;
   constexpr int L = kMGSmallEdge + 2;
   constexpr int SY = L, SZ = L * L;
   constexpr int NMAX = DIM == 3 ? kMGSmallEdge * kMGSmallEdge * kMGSmallEdge
                                 : kMGSmallEdge * kMGSmallEdge;
   constexpr int NC = (NMAX + kThreads - 1) / kThreads;
-  __shared__ T U[DIM == 3 ? L * L * L : L * L];
-  if (*stop) return;
   const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
   const int ncell = n0 * n1 * n2;
   for (int q = threadIdx.x; q < (DIM == 3 ? L * L * L : L * L); q += kThreads) U[q] = (T)0;
@@ -764,6 +784,14 @@ k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, con
 #pragma unroll
   for (int v = 0; v < NC; ++v)
     if (qc[v] >= 0) u[threadIdx.x + v * kThreads] = U[qc[v]];
+}
+
+template <int DIM, bool DVAR, int CMODE, typename T>
+__global__ void __launch_bounds__(kMGSmallThreads)
+k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, const int* stop) {
+  __shared__ T U[kMGSmallLds];
+  if (*stop) return;
+  mg_small_block<DIM, DVAR, CMODE, T, kMGSmallThreads>(G, A, bc, u, sweeps, U);
 }
 
 // Partial sums of the squared residual (x == nullptr) or of x^2 over a

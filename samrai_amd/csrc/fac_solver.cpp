@@ -1189,7 +1189,7 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
                                d_stream);
     colour_pass(0, red);
     sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
-                             nullptr, 1, bc, stop, d_stream);
+                             nullptr, 1, bc, stop, d_stream, nullptr, nullptr, 0);
     if (ewing)
       sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, e.table(ln), e.ghost(), ps, stop,
                                d_stream);
@@ -1200,7 +1200,7 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
     if (tol >= 0.0 || isweep + 1 < num_sweeps)
       sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
                                nullptr, 1, bc, stop, d_stream,
-                               (tol >= 0.0 && np > 0) ? &chk : nullptr);
+                               (tol >= 0.0 && np > 0) ? &chk : nullptr, nullptr, 1);
   }
   // edges/corners too, as the reference's last NoCoarse fill does
   ghostFill(e, ln, false, true, false);

@@ -36,10 +36,14 @@ struct CfRefine {
   int method;
   double cdx[3];
 };
+// colour >= 0: the fill right after colour pass `colour` of launch_rbgs on
+// this data: same-level copies and Robin face ghosts whose source cell that
+// pass did not change are skipped (they already hold those values).
 void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int dg,
                         const double* const* coarse, int cg, const Box* coarse_box,
                         int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s,
-                        const SweepCheck* check = nullptr, const CfRefine* cf = nullptr);
+                        const SweepCheck* check = nullptr, const CfRefine* cf = nullptr,
+                        int colour = -1);
 
 // One red (colour 0) or black (colour 1) Gauss-Seidel pass with the flux
 // (and Ewing coarse-fine correction) computed on the fly.

@@ -121,10 +121,22 @@ __device__ inline void sweep_check(const SweepCheck& k) {
 
 // ------------------------------------------------------------ ghost fill
 
+// Does the colour pass `colour` of k_rbgs update interior cell c of the
+// patch with box b?  (Its cell selection: the first cell of row (j, k) has
+// the colour of the truncating parity of lo0 + j + k in 3-D, lo1 + j in 2-D,
+// patch-relative as in the reference; the row then alternates.)  A ghost
+// fill after a colour pass only needs the ghosts whose source it changed:
+// the others still hold the values the previous fill wrote from the same,
+// unchanged sources.
+__device__ inline bool updated_by(const Box& b, int dim, const int* c, int colour) {
+  const int par = dim == 3 ? fmod2(b.lo[0] + c[1] + c[2]) : fmod2(b.lo[1] + c[1]);
+  return ((c[0] - b.lo[0]) & 1) == (par != colour ? 1 : 0);
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* const* dst, int dg,
               const double* const* coarse, int cg, const Box* cbox, int homogeneous,
-              RobinDev bc, const int* stop, SweepCheck chk, CfRefine cfr) {
+              RobinDev bc, const int* stop, SweepCheck chk, CfRefine cfr, int colour) {
   if (stop && *stop) return;
   if (chk.stop && blockIdx.x == 0) sweep_check(chk);
   const int ci = xcd_chunk();
@@ -140,6 +152,7 @@ k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* co
     if (!decode_box(it, (long)ch.start + t, c)) break;
     if (kind == kCopy) {
       const Box sb = L.box[it.q];
+      if (colour >= 0 && !updated_by(sb, L.dim, c, colour)) continue;
       x[vd.at(c[0], c[1], c[2])] = dst[it.q][vview(sb, dg, L.dim).at(c[0], c[1], c[2])];
     } else if (kind == kCoarseFine) {
       // the refine operator of cf_discretization over the coarse patch
@@ -153,6 +166,7 @@ k_ghost_items(const BoxItem* items, const Chunk* chunks, LevelGeom L, double* co
       int n = loc >> 1, up = loc & 1;
       int in[3] = {c[0], c[1], c[2]};
       in[n] += up ? -1 : 1;
+      if (colour >= 0 && !updated_by(b, L.dim, in, colour)) continue;
       double h = L.dx[n], av = bc.a[loc], bv = bc.b[loc];
       double uin = x[vd.at(in[0], in[1], in[2])];
       double v;
@@ -776,7 +790,7 @@ inline void check_launch() {
 void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const* dst, int dg,
                         const double* const* coarse, int cg, const Box* coarse_box,
                         int homogeneous, const RobinDev& bc, const int* stop, hipStream_t s,
-                        const SweepCheck* check, const CfRefine* cf) {
+                        const SweepCheck* check, const CfRefine* cf, int colour) {
   SweepCheck chk{};
   if (check) chk = *check;
   CfRefine cfr{};
@@ -787,7 +801,7 @@ void launch_ghost_items(const GhostLaunch& w, const LevelGeom& L, double* const*
     return;
   }
   hipLaunchKernelGGL(k_ghost_items, dim3(xcd_grid(w.nchunks)), dim3(kThreads), 0, s, w.items, w.chunks, L,
-                     dst, dg, coarse, cg, coarse_box, homogeneous, bc, stop, chk, cfr);
+                     dst, dg, coarse, cg, coarse_box, homogeneous, bc, stop, chk, cfr, colour);
   check_launch();
 }
 
