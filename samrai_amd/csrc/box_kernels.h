@@ -35,7 +35,7 @@ struct BoxLevel {
 // sweeps, transfers and multigrid compute in FP32; the early-exit maxima and
 // norm partial sums stay FP64.
 struct BoxArrays {
-  const double* ein;    // u before the sweep: n0*n1*n2, no ghosts
+  const double* ein;    // u before the sweep: n0*n1*n2, no ghosts (nullptr: u = 0)
   double* eout;         // u after the sweep (a second buffer: work-groups read
                         // their halos from ein while others write eout)
   const double* r;      // same shape

@@ -286,7 +286,8 @@ k_box_rbgs(BoxLevel G, BoxArraysT<T> A, RobinDev bc, int homogeneous, BoxCheck K
     const int col = threadIdx.x + c * kThreads;
     const int lx = col % LX, ly = col / LX;
     const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-    const bool xy = col < LX * LY && i >= 0 && i < n0 && j >= 0 && j < n1;
+    // ein == nullptr: u = 0 before the sweep (nothing to load)
+    const bool xy = A.ein && col < LX * LY && i >= 0 && i < n0 && j >= 0 && j < n1;
     const T* src = A.ein + ((long)i + (long)n0 * j);
 #pragma unroll
     for (int lz = 0; lz < LZ; ++lz) {

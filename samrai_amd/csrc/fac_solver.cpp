@@ -900,14 +900,21 @@ sfac::BoxArrays CellPoissonFACOps::mgArrays(int m) const {
   return A;
 }
 
-void CellPoissonFACOps::mgSweeps(int m, int n) {
+// zero_in: the error starts at 0 (the first sweep reads no input and
+// writes every cell, so no fill is needed before it).
+void CellPoissonFACOps::mgSweeps(int m, int n, bool zero_in) {
   MGLevel& L = d_mg[m];
   sfac::BoxArrays A = mgArrays(m);
   sfac::BoxCheck K{};
   K.stop = d_mg_flag;
   const sfac::RobinDev bc = robinDev();
+  if (zero_in && n == 0) {
+    const int dim = d_h.dim();
+    const long nc = (long)L.G.n[0] * L.G.n[1] * (dim == 3 ? L.G.n[2] : 1);
+    sfac::launch_fill(L.e[0], nc, 0.0, d_stream);  // FP64-sized buffer either way
+  }
   for (int s = 0; s < n; ++s) {
-    A.ein = L.e[s & 1];
+    A.ein = (s == 0 && zero_in) ? nullptr : L.e[s & 1];
     A.eout = L.e[(s + 1) & 1];
     sfac::launch_box_rbgs(L.G, A, bc, 1, K, d_box.variant, d_stream);
   }
@@ -915,8 +922,6 @@ void CellPoissonFACOps::mgSweeps(int m, int n) {
 
 void CellPoissonFACOps::mgVcycle(int m) {
   MGLevel& L = d_mg[m];
-  const int dim = d_h.dim();
-  const long nc = (long)L.G.n[0] * L.G.n[1] * (dim == 3 ? L.G.n[2] : 1);
   const sfac::RobinDev bc = robinDev();
   const bool last = m + 1 == (int)d_mg.size();
   if (last && d_mg_small_direct) {
@@ -924,12 +929,13 @@ void CellPoissonFACOps::mgVcycle(int m) {
                                 d_stream);
     return;
   }
-  if (m > 0) sfac::launch_fill(L.e[0], nc, 0.0, d_stream);
+  // the error of a coarser level starts at 0: its first sweep reads no
+  // input (the grid-parallel sweep writes every cell of the level)
   if (last) {
-    mgSweeps(m, kMGLastSweeps);
+    mgSweeps(m, kMGLastSweeps, m > 0);
     return;
   }
-  mgSweeps(m, kMGPre);
+  mgSweeps(m, kMGPre, m > 0);
   MGLevel& C = d_mg[m + 1];
   sfac::launch_mg_restrict_residual(L.G, mgArrays(m), bc, L.e[0], C.G, C.f, d_mg_flag, d_stream);
   mgVcycle(m + 1);

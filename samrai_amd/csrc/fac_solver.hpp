@@ -336,7 +336,7 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   void setupMG();
   void releaseMG();
   sfac::BoxArrays mgArrays(int m) const;
-  void mgSweeps(int m, int n);
+  void mgSweeps(int m, int n, bool zero_in = false);
   void mgVcycle(int m);
   void solveBoxMG(pdat::PatchDataSet& e, const pdat::PatchDataSet& r);
   bool d_use_graphs = true;
