@@ -147,7 +147,18 @@ void FACPreconditioner::facCycle_Recursive(SAMRAIVectorReal& e, SAMRAIVectorReal
   d_ops.computeCompositeResidualOnLevel(d_tr, e, r, ln, true);
   d_ops.restrictResidual(d_tr, r, ln - 1);
   facCycle_Recursive(e, r, u, lmax, lmin, ln - 1);
-  d_ops.prolongErrorAndCorrect(e, e, ln);
+  // level ln - 1 was just smoothed or solved, which ends with the ghost fill
+  // the prolongation needs
+  {
+    struct Filled {  // cleared however the call ends
+      CellPoissonFACOps* g;
+      ~Filled() {
+        if (g) g->setProlongCoarseFilled(false);
+      }
+    } filled{d_gpu};
+    if (d_gpu) d_gpu->setProlongCoarseFilled(true);
+    d_ops.prolongErrorAndCorrect(e, e, ln);
+  }
   if (d_gpu) d_gpu->setSmoothingPhase(1);
   d_ops.smoothError(e, r, ln, d_post);
   if (d_gpu) d_gpu->setSmoothingPhase(0);

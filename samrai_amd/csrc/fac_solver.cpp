@@ -1530,7 +1530,9 @@ void CellPoissonFACOps::prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAI
   const auto& Tc = d_h.tables(dest_ln - 1);
   // coarse values outside the coarse patch come from its same-level and
   // homogeneous physical ghosts, as RefineSchedule's coarse fill sets them
-  ghostFill(*s.data, dest_ln - 1, false, true, false);
+  // (already set when the caller just ended the coarser level's smoothing
+  // or coarse solve with that same fill)
+  if (!d_prolong_coarse_filled) ghostFill(*s.data, dest_ln - 1, false, true, false);
   const int method = d_prolongation == "CONSTANT_REFINE" ? sfac::kProlongConstant
                      : d_prolongation == "LINEAR_REFINE" ? sfac::kProlongLinear
                                                          : sfac::kProlongConservative;
@@ -1543,7 +1545,7 @@ void CellPoissonFACOps::prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAI
         reinterpret_cast<void* const*>(d.data->table(dest_ln)), d.data->ghost(), method,
         d_stream);
   else
-    sfac::launch_prolong_add(Tf.prolong_from_coarser.launch(), Tc.geom, Tf.geom,
+    sfac::launch_prolong_add(Tf.prolong_rows.launch(), Tc.geom, Tf.geom,
                              s.data->table(dest_ln - 1), s.data->ghost(), d.data->table(dest_ln),
                              d.data->ghost(), method, d_stream);
 }

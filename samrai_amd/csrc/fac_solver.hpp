@@ -178,6 +178,11 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   // FACOperatorStrategy
   void restrictSolution(const SAMRAIVectorReal& s, SAMRAIVectorReal& d, int dest_ln) override;
   void restrictResidual(const SAMRAIVectorReal& s, SAMRAIVectorReal& d, int dest_ln) override;
+  // The next prolongErrorAndCorrect's source level already holds the full
+  // same-level / homogeneous Robin ghost fill it would do: smoothError and
+  // solveCoarsestLevel end with that fill, so FACPreconditioner's recursive
+  // cycle, which prolongs right after them, sets this around the call.
+  void setProlongCoarseFilled(bool on) { d_prolong_coarse_filled = on; }
   void prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAIVectorReal& d,
                               int dest_ln) override;
   void smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal& r, int ln,
@@ -330,6 +335,7 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   };
   std::vector<MGLevel> d_mg;  // [0] aliases the box arrays
   bool d_mg_small_direct = false;
+  bool d_prolong_coarse_filled = false;
   double* d_mg_partial = nullptr;
   double* d_mg_scal = nullptr;  // [0] ||b||^2, [1] ||r||^2, [2] relative residual
   int* d_mg_flag = nullptr;     // [0] stop, [1] iterations

@@ -568,6 +568,7 @@ void PatchHierarchy::buildLevel(int ln) {
     }
     T.restrict_to_coarser = uploadList(rs);
     T.prolong_from_coarser = uploadList(pr);
+    T.prolong_rows = uploadList(pr, kCellRows);
     T.flux_to_coarser = uploadList(fc);
     // the coarser level's faces these coarsened fluxes replace: per coarse
     // cell, bit 2a its lower face and bit 2a+1 its upper face along a

@@ -76,6 +76,7 @@ class PatchHierarchy {
     DevList full_nc[4], full_wc[4];  // all ghosts by phase
     DevList restrict_to_coarser;     // p coarse, q fine, coarse box
     DevList prolong_from_coarser;    // p fine, q coarse, fine box
+    DevList prolong_rows;            // the same items in whole-row chunks
     DevList flux_to_coarser;         // p coarse, q fine, coarse faces
     long ncells = 0;
   };

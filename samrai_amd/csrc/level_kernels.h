@@ -89,8 +89,8 @@ void launch_restrict(const GhostLaunch& w, const LevelGeom& coarse, const LevelG
                      double* const* dst, int dg, const double* const* src, int sg,
                      hipStream_t s);
 
-// Refine the coarse error and e_f += it (items: p fine, q coarse, fine box),
-// by the prolongation method below.
+// Refine the coarse error and e_f += it (items: p fine, q coarse, fine box,
+// in whole-row chunks), by the prolongation method below.
 enum ProlongMethod { kProlongConservative = 0, kProlongConstant = 1, kProlongLinear = 2 };
 void launch_prolong_add(const GhostLaunch& w, const LevelGeom& coarse, const LevelGeom& fine,
                         const double* const* ce, int cg, double* const* fe, int fg,

@@ -655,12 +655,55 @@ k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom 
   const View vc = vview(C.box[it.q], cg, DIM), vf = vview(Fl.box[it.p], fg, DIM);
   const double* uc_ = ce[it.q];
   double* uf = fe[it.p];
-  for (int t = threadIdx.x; t < kCellsPerChunk; t += kThreads) {
-    int f[3];
-    if (!decode_box(it, (long)ch.start + t, f)) break;
-    const double v = refine_value<DIM>(method, uc_, vc, f, Fl.ratio, C.dx, Fl.dx);
-    long o = vf.at(f[0], f[1], f[2]);
-    uf[o] = uf[o] + v;
+  const int r = Fl.ratio;
+  // whole rows (j,k) of the item box; a lane takes the fine cells of one
+  // coarse parent along x, so the parent's limited slopes (conservative
+  // linear refine) are formed once for its children of the row
+  const int nx = it.hi[0] - it.lo[0] + 1, ny = it.hi[1] - it.lo[1] + 1;
+  const int nz = DIM == 3 ? it.hi[2] - it.lo[2] + 1 : 1;
+  const int P0 = cdiv(it.lo[0], r), npx = cdiv(it.hi[0], r) - P0 + 1;
+  const int rpc = kRowCellsPerChunk / nx > 0 ? kRowCellsPerChunk / nx : 1;
+  const int row_end = min(ch.start + rpc, ny * nz);
+  const int lanes = npx < kThreads ? npx : kThreads;
+  const int rows_per_iter = kThreads / lanes;
+  if ((int)threadIdx.x >= rows_per_iter * lanes) return;
+  const int m0 = (int)threadIdx.x % lanes;
+  for (int row = ch.start + (int)threadIdx.x / lanes; row < row_end; row += rows_per_iter) {
+    const int j = it.lo[1] + row % ny, k = DIM == 3 ? it.lo[2] + row / ny : 0;
+    for (int I = P0 + m0; I < P0 + npx; I += lanes) {
+      const int i0 = max(I * r, it.lo[0]), i1 = min(I * r + r - 1, it.hi[0]);
+      if (method == kProlongConservative) {
+        // refine_value's conservative branch, the parent's part once
+        const int c1 = cdiv(j, r), c2 = DIM == 3 ? cdiv(k, r) : 0;
+        const double uc = uc_[vc.at(I, c1, c2)];
+        const double s0 =
+            clin_slope(uc - uc_[vc.at(I - 1, c1, c2)], uc_[vc.at(I + 1, c1, c2)] - uc, C.dx[0]);
+        const double s1 =
+            clin_slope(uc - uc_[vc.at(I, c1 - 1, c2)], uc_[vc.at(I, c1 + 1, c2)] - uc, C.dx[1]);
+        const int l1 = j - c1 * r;
+        const double d1 = ((double)l1 + 0.5) * Fl.dx[1] - C.dx[1] * 0.5;
+        double s2 = 0.0, d2 = 0.0;
+        if (DIM == 3) {
+          s2 = clin_slope(uc - uc_[vc.at(I, c1, c2 - 1)], uc_[vc.at(I, c1, c2 + 1)] - uc, C.dx[2]);
+          const int l2 = k - c2 * r;
+          d2 = ((double)l2 + 0.5) * Fl.dx[2] - C.dx[2] * 0.5;
+        }
+        for (int i = i0; i <= i1; ++i) {
+          const int l0 = i - I * r;
+          const double d0 = ((double)l0 + 0.5) * Fl.dx[0] - C.dx[0] * 0.5;
+          const double v = DIM == 3 ? uc + s0 * d0 + s1 * d1 + s2 * d2 : uc + s0 * d0 + s1 * d1;
+          const long o = vf.at(i, j, k);
+          uf[o] = uf[o] + v;
+        }
+      } else {
+        for (int i = i0; i <= i1; ++i) {
+          const int f[3] = {i, j, k};
+          const double v = refine_value<DIM>(method, uc_, vc, f, r, C.dx, Fl.dx);
+          const long o = vf.at(i, j, k);
+          uf[o] = uf[o] + v;
+        }
+      }
+    }
   }
 }
 
