@@ -578,18 +578,18 @@ int CellPoissonFACOps::tuneFusedVariant(int ln) {
     return ms;
   };
   // the patch-wise colour passes (what smoothErrorByRedBlack runs otherwise):
-  // per sweep, [Ewing snapshot] red, face ghost fill, [Ewing snapshot]
-  // black, face ghost fill
+  // per sweep, Ewing snapshot, red, face ghost fill, black, face ghost fill
+  // (3-D: one snapshot serves both colours)
   for (int i = 0; i < 8; ++i) {
     if (i == 2) hip_check(hipEventRecord(t0, d_stream), "hipEventRecord");
     for (int colour = 0; colour < 2; ++colour) {
-      if (ewing)
+      if (ewing && colour == 0)
         sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, zero.table(ln), 1, ps, nullptr,
                                  d_stream);
       sfac::launch_rbgs(T.smooth.smooth(), T.geom, zero.table(ln), 1, rhs.table(ln), F.rg, ps,
                         ewing ? 2 : 0, colour, nullptr, nullptr, d_stream);
       sfac::launch_ghost_items(T.face_nc.launch(), T.geom, zero.table(ln), 1, nullptr, 0,
-                               nullptr, 1, bc, nullptr, d_stream);
+                               nullptr, 1, bc, nullptr, d_stream, nullptr, nullptr, colour);
     }
   }
   int best = -1;
@@ -1196,7 +1196,11 @@ int CellPoissonFACOps::smoothErrorByRedBlack(PatchDataSet& e, const PatchDataSet
     colour_pass(0, red);
     sfac::launch_ghost_items(T.face_nc.launch(), T.geom, e.table(ln), e.ghost(), nullptr, 0,
                              nullptr, 1, bc, stop, d_stream, nullptr, nullptr, 0);
-    if (ewing)
+    // a black cell's corrected flux reads the cell, its coarse-fine ghost
+    // and its mirror cell, which in 3-D is black too: none changed in the red
+    // pass, so the snapshot taken before it still holds the black cells'
+    // fluxes; in 2-D the mirror is red and the snapshot is retaken
+    if (ewing && d_h.dim() != 3)
       sfac::launch_ewing_faces(T.cf_faces.launch(), T.geom, e.table(ln), e.ghost(), ps, stop,
                                d_stream);
     colour_pass(1, blk);
@@ -1340,7 +1344,8 @@ int CellPoissonFACOps::smoothColourLP(PatchDataSet& e, const PatchDataSet& r, in
     colour_pass(0, red);
     sfac::launch_ghost_items_lp(elem, T.face_nc.launch(), T.geom, et, e.ghost(), nullptr, 0,
                                 nullptr, 1, bc, stop, d_stream);
-    if (ewing)
+    // 3-D: the black cells' snapshot fluxes are unchanged by the red pass
+    if (ewing && d_h.dim() != 3)
       sfac::launch_ewing_faces_lp(elem, T.cf_faces.launch(), T.geom, et, e.ghost(), ps, stop,
                                   d_stream);
     colour_pass(1, blk);
