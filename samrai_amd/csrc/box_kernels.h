@@ -132,6 +132,12 @@ void launch_mg_small_solve(const BoxLevel& G, const BoxArrays& A, const RobinDev
 // mg_partials() partial sums of x^2 (x != nullptr) or of the squared residual of u.
 void launch_mg_sumsq(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, const double* u,
                      const double* x, double* partial, const int* stop, hipStream_t s);
+// `sweeps` red-black sweeps of u in place (from 0 when zero_in) on a level
+// of at most mg_lds_edge() cells per axis, in one work-group with the level
+// in LDS: the values the same number of launch_box_rbgs sweeps leave.
+int mg_lds_edge();
+void launch_mg_lds_sweeps(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, double* u,
+                          int sweeps, int zero_in, const int* stop, hipStream_t s);
 // Relative-residual test after iteration it (it < 0: b = 0 test only).
 void launch_mg_check(const double* rr, const double* bb, double tol, int it, int* stop,
                      int* iters, double* relres, hipStream_t s);

@@ -731,28 +731,38 @@ k_mg_coarsen_coef(BoxLevel Gf, BoxArraysT<T> Af, BoxLevel Gc, T* Dc0, T* Dc1, T*
   }
 }
 
-// The smallest grid (at most kMGSmallEdge cells per axis): u = 0, then
-// `sweeps` red-black sweeps with the whole grid in LDS, each thread's cell
-// coefficients and reciprocal diagonal in registers and the Robin ghosts
-// formed inside the updates; one work-group of one wavefront (one cell per
-// lane), one barrier per colour.
 constexpr int kMGSmallEdge = 4;
 constexpr int kMGSmallThreads = 64;
-constexpr int kMGSmallLds = (kMGSmallEdge + 2) * (kMGSmallEdge + 2) * (kMGSmallEdge + 2);
-// The solve itself, for a work-group of NT threads (NT >= the grid's cells
-// keeps one cell per thread), with U an LDS array of kMGSmallLds values.
-template <int DIM, bool DVAR, int CMODE, typename T, int NT>
-__device__ inline void mg_small_block(const BoxLevel& G, const BoxArraysT<T>& A,
-                                      const RobinDev& bc, T* u, int sweeps, T* U) {
+// Levels of at most kMGLdsEdge cells per axis run their red-black sweeps
+// with the whole grid in one work-group's LDS (k_mg_lds_sweeps).
+constexpr int kMGLdsEdge = 8;
+constexpr int kMGLdsThreads = 512;
+template <int DIM, int EDGE>
+constexpr int mg_lds_size() {
+  return DIM == 3 ? (EDGE + 2) * (EDGE + 2) * (EDGE + 2) : (EDGE + 2) * (EDGE + 2);
+}
+// `sweeps` red-black sweeps of a grid of at most EDGE cells per axis held in
+// LDS (U, mg_lds_size values) by a work-group of NT threads, each thread's
+// cell coefficients and reciprocal diagonals in registers and the Robin
+// ghosts formed inside the updates (the cell is its face ghost's only
+// interior neighbour); u starts from uin (nullptr: 0) and ends in uout.
+// Red <=> (level indices + shift) even, red first, as k_box_rbgs; updates of
+// one colour are independent, so sweeping in place equals its ping-pong
+// tiles bit for bit.
+template <int DIM, bool DVAR, int CMODE, typename T, int NT, int EDGE>
+__device__ inline void mg_lds_block(const BoxLevel& G, const BoxArraysT<T>& A,
+                                    const RobinDev& bc, const T* uin, T* uout, int sweeps,
+                                    T* U) {
   constexpr int kThreads = NT;
-  constexpr int L = kMGSmallEdge + 2;
+  constexpr int L = EDGE + 2;
   constexpr int SY = L, SZ = L * L;
-  constexpr int NMAX = DIM == 3 ? kMGSmallEdge * kMGSmallEdge * kMGSmallEdge
-                                : kMGSmallEdge * kMGSmallEdge;
+  constexpr int NMAX = DIM == 3 ? EDGE * EDGE * EDGE : EDGE * EDGE;
   constexpr int NC = (NMAX + kThreads - 1) / kThreads;
   const int n0 = G.n[0], n1 = G.n[1], n2 = DIM == 3 ? G.n[2] : 1;
   const int ncell = n0 * n1 * n2;
-  for (int q = threadIdx.x; q < (DIM == 3 ? L * L * L : L * L); q += kThreads) U[q] = (T)0;
+  const int par0 = G.lo[0] + G.lo[1] + (DIM == 3 ? G.lo[2] : 0) + G.shift;
+  for (int q = threadIdx.x; q < mg_lds_size<DIM, EDGE>(); q += kThreads) U[q] = (T)0;
+  __syncthreads();
   int qc[NC], col[NC], bm[NC];
   Coef<T> cf[NC];
   T dudr[NC];
@@ -763,14 +773,14 @@ __device__ inline void mg_small_block(const BoxLevel& G, const BoxArraysT<T>& A,
     if (t < ncell) {
       const int i = t % n0, j = (t / n0) % n1, k = DIM == 3 ? t / (n0 * n1) : 0;
       qc[v] = (i + 1) + SY * (j + 1) + SZ * (DIM == 3 ? k + 1 : 0);
-      col[v] = (i + j + k) & 1;
+      col[v] = pos2(par0 + i + j + k);
       cf[v] = load_coef<DIM, DVAR, CMODE>(G, A, i, j, k);
       dudr[v] = cell_dudr<DIM, DVAR, CMODE>(cf[v], G, A);
-      // every face of this grid is on the physical boundary: the Robin
-      // ghosts are formed inside the update, from the cell's own value
+      // every face of this grid is on the physical boundary
       bm[v] = (i == 0 ? 1 : 0) | (i == n0 - 1 ? 2 : 0) | (j == 0 ? 4 : 0) |
               (j == n1 - 1 ? 8 : 0) |
               (DIM == 3 ? (k == 0 ? 16 : 0) | (k == n2 - 1 ? 32 : 0) : 0);
+      if (uin) U[qc[v]] = uin[t];
     }
   }
   __syncthreads();
@@ -784,15 +794,32 @@ __device__ inline void mg_small_block(const BoxLevel& G, const BoxArraysT<T>& A,
     }
 #pragma unroll
   for (int v = 0; v < NC; ++v)
-    if (qc[v] >= 0) u[threadIdx.x + v * kThreads] = U[qc[v]];
+    if (qc[v] >= 0) uout[threadIdx.x + v * kThreads] = U[qc[v]];
 }
 
+// The smallest grid (at most kMGSmallEdge cells per axis): u = 0, then
+// `sweeps` sweeps (mg_lds_block); one wavefront, one cell per lane.
 template <int DIM, bool DVAR, int CMODE, typename T>
 __global__ void __launch_bounds__(kMGSmallThreads)
 k_mg_small_solve(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, const int* stop) {
-  __shared__ T U[kMGSmallLds];
+  __shared__ T U[mg_lds_size<DIM, kMGSmallEdge>()];
   if (*stop) return;
-  mg_small_block<DIM, DVAR, CMODE, T, kMGSmallThreads>(G, A, bc, u, sweeps, U);
+  mg_lds_block<DIM, DVAR, CMODE, T, kMGSmallThreads, kMGSmallEdge>(G, A, bc, nullptr, u, sweeps,
+                                                                    U);
+}
+
+// `sweeps` red-black sweeps of a level of at most kMGLdsEdge cells per axis
+// in one launch (the grid-parallel sweep takes one launch per sweep, and on
+// such a level its time is the latency of its steps, not its bytes); u in
+// place, from 0 when zero_in.
+template <int DIM, bool DVAR, int CMODE, typename T>
+__global__ void __launch_bounds__(kMGLdsThreads)
+k_mg_lds_sweeps(BoxLevel G, BoxArraysT<T> A, RobinDev bc, T* u, int sweeps, int zero_in,
+                const int* stop) {
+  __shared__ T U[mg_lds_size<DIM, kMGLdsEdge>()];
+  if (*stop) return;
+  mg_lds_block<DIM, DVAR, CMODE, T, kMGLdsThreads, kMGLdsEdge>(G, A, bc, zero_in ? nullptr : u,
+                                                                u, sweeps, U);
 }
 
 // Partial sums of the squared residual (x == nullptr) or of x^2 over a
@@ -1001,6 +1028,17 @@ struct SumsqRun {
   }
 };
 
+template <int DIM, bool DVAR, int CMODE>
+struct LdsSweepsRun {
+  template <typename T>
+  static void run(const BoxLevel& G, const BoxArraysT<T>& A, const RobinDev& bc, T* u,
+                  int sweeps, int zero_in, const int* stop, hipStream_t s) {
+    hipLaunchKernelGGL((k_mg_lds_sweeps<DIM, DVAR, CMODE, T>), dim3(1), dim3(kMGLdsThreads), 0,
+                       s, G, A, bc, u, sweeps, zero_in, stop);
+    check_launch();
+  }
+};
+
 template <typename T>
 const T* as(const double* p) { return reinterpret_cast<const T*>(p); }
 template <typename T>
@@ -1008,6 +1046,17 @@ T* as(double* p) { return reinterpret_cast<T*>(p); }
 }  // namespace
 
 int mg_small_edge() { return kMGSmallEdge; }
+int mg_lds_edge() { return kMGLdsEdge; }
+
+void launch_mg_lds_sweeps(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, double* u,
+                          int sweeps, int zero_in, const int* stop, hipStream_t s) {
+  for (int d = 0; d < G.dim; ++d)
+    if (G.n[d] > kMGLdsEdge || G.n[d] < 1) throw hipErrorInvalidValue;
+  if (A.fp32)
+    dispatch3<LdsSweepsRun>(G, typed<float>(A), bc, as<float>(u), sweeps, zero_in, stop, s);
+  else
+    dispatch3<LdsSweepsRun>(G, typed<double>(A), bc, u, sweeps, zero_in, stop, s);
+}
 int mg_partials() { return kMGGrid; }
 
 void launch_mg_restrict_residual(const BoxLevel& Gf, const BoxArrays& A, const RobinDev& bc,

@@ -908,6 +908,12 @@ void CellPoissonFACOps::mgSweeps(int m, int n, bool zero_in) {
   sfac::BoxCheck K{};
   K.stop = d_mg_flag;
   const sfac::RobinDev bc = robinDev();
+  bool lds = n > 0 && n % 2 == 0;  // even counts end in e[0], as the ping-pong sweeps do
+  for (int d = 0; d < d_h.dim(); ++d) lds = lds && L.G.n[d] <= sfac::mg_lds_edge();
+  if (lds) {
+    sfac::launch_mg_lds_sweeps(L.G, A, bc, L.e[0], n, zero_in ? 1 : 0, d_mg_flag, d_stream);
+    return;
+  }
   if (zero_in && n == 0) {
     const int dim = d_h.dim();
     const long nc = (long)L.G.n[0] * L.G.n[1] * (dim == 3 ? L.G.n[2] : 1);
