@@ -225,7 +225,9 @@ def learned_section(torch, seeds, steps, batch, log=print):
     variable-coefficient problems (one-level 64^3 hierarchies of 16^3 patches,
     residuals after 1, 2 and 3 FAC cycles, targets the FP64 block solutions),
     training on the GPU by the library's own forward/backward kernels, the
-    matrix-core rate of the forward convolutions, and the gain on unseen seeds."""
+    matrix-core rate of the forward convolutions, the gain on unseen seeds, and
+    whole solves with the correction on its own two-level problem and on C2."""
+    from samrai_amd import decks, fac
     from samrai_amd import learned as SL
     t0 = time.perf_counter()
     train = [SL.fac_training_pairs(torch, s, cycles=(1, 2, 3)) for s in range(seeds)]
@@ -340,10 +342,41 @@ def learned_section(torch, seeds, steps, batch, log=print):
     sv.close()
     P.close()
     log("bench: learned FAC solves %s" % json.dumps(fac_solves))
+    # the same network on C2 (patches of 8..48 cells a side, covered by 16^3
+    # windows): whole solves with red-black alone and with the correction
+    deck2, h2, u2, f2, D2, s2 = decks.build_c2_problem(fac, torch, MG_PARAMS)
+    nwin = max(sum(int(np.prod([-(-(hh - ll + 1) // 16) for ll, hh in zip(*b)])) for b in lv)
+               for lv in deck2["levels"][1:])
+    big = SL.LearnedSmoother(torch, nwin, seed=0)
+    big.set_params(net.params())
+    c2_solves = {}
+    for name, om in (("rbgs", None), ("learned_omega_1", 1.0), ("learned_omega_0.5", 0.5)):
+        SL.set_learned_smoother(s2, None if om is None else big, om or 1.0)
+        u2.fill(0.0)
+        s2.solve_system(u2, f2)  # graph capture
+        u2.fill(0.0)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        ok = s2.solve_system(u2, f2)
+        s2.synchronize()
+        dt = time.perf_counter() - t2
+        c2_solves[name] = dict(s_per_solve=round(dt, 4), cycles=s2.number_of_iterations(),
+                               converged=bool(ok), final_residual=float(s2.history()[-1]),
+                               learned_still_active=SL.learned_active(s2),
+                               first_factors=[round(b / a, 4) for a, b in
+                                              zip(s2.history()[:4], s2.history()[1:5])])
+    SL.set_learned_smoother(s2, None)
+    s2.close()
+    big.close()
+    log("bench: learned FAC solves on C2 %s" % json.dumps(c2_solves))
     out = dict(
         fac_solves=dict(workload="two levels, 64^3 + 128^3 in 16^3 patches (576 patches), "
                                  "D = exp(GRF), 8 Gaussian sources, unseen seed; RBGS 2+2, mg "
                                  "coarse, rel tol 1e-10", **fac_solves),
+        fac_solves_c2=dict(workload="c2-3d-s0 (2147 patches of 8..48 cells a side), the "
+                                    "same network over 16^3 windows (%d blocks on the largest "
+                                    "level); RBGS 2+2, mg coarse, rel tol 1e-10" % nwin,
+                           **c2_solves),
         level_smoothing=dict(workload="one-level 64^3 problem of 16^3 patches (unseen seed), "
                                       "8 iterations from u = 0", **level),
         workload="pairs from FAC iterates (1-3 cycles, 2 coarse RBGS sweeps per cycle) on "

@@ -105,25 +105,28 @@ int sfac_learned_loss(void* net, float* loss);
  * cell), prediction (FP32 per cell), per-block scale s (FP64). */
 int sfac_learned_buffers(void* net, void** features, float** pred, double** scale);
 
-/* The learned correction on level ln of a solver's hierarchy whose patches
- * on that level are all 16x16x16 (cubic cells): u += omega * s * net(r h^2 / s,
- * log D) on every patch interior.  r: cell data with ghost 0 (for example
- * from sfac_ops_residual); D: the side data given to the solver (ghost 0),
- * or NULL for log D = 0; u: cell data of any ghost width (ghosts are not
- * touched: fill them before the next smoothing).  Runs on the solver's
- * stream; the network must hold at least as many blocks as the level has
- * patches.  A smoothError implementation calls it before red-black sweeps. */
+/* The learned correction on level ln of a solver's hierarchy (cubic
+ * cells): each patch is covered by 16x16x16 windows from its lower corner
+ * (one network block each; window cells past the patch read 0 and are not
+ * corrected), and u += omega * s * net(r h^2 / s, log D) on every patch
+ * interior.  r: cell data with ghost 0 (for example from sfac_ops_residual);
+ * D: the side data given to the solver (ghost 0), or NULL for log D = 0; u:
+ * cell data of any ghost width (ghosts are not touched: fill them before the
+ * next smoothing).  Runs on the solver's stream; the network must hold at
+ * least as many blocks as the level has windows (a 16x16x16 patch is one).
+ * A smoothError implementation calls it before red-black sweeps. */
 int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac_patch_data* r,
                              sfac_patch_data* D, int ln, double omega);
 
 /* Attach the network to a solver (net NULL detaches): CellPoissonFACOps::
  * smoothError then applies omega times the learned correction of the error
  * equation's residual before its red-black sweeps, on every level above the
- * coarsest whose patches are all 16x16x16 with cubic cells.  Each
- * solveSystem starts with it enabled and falls back to red-black alone for
- * the rest of the solve after a cycle whose residual reduction factor
- * exceeds 0.5; the FP64 residual remains the convergence test.  The network
- * must hold at least as many blocks as such a level has patches. */
+ * coarsest with cubic cells (patches of any size, covered by 16x16x16
+ * windows as above).  Each solveSystem starts with it enabled and falls back
+ * to red-black alone for the rest of the solve after a cycle whose residual
+ * reduction factor exceeds 0.5; the FP64 residual remains the convergence
+ * test.  The network must hold at least as many blocks as such a level has
+ * windows (checked when attached to an initialized solver). */
 int sfac_solver_set_learned_smoother(sfac_solver* s, void* net, double omega);
 /* Which smoothing calls of the recursive V-cycle apply it: bit 0 the
  * pre-smoothing, bit 1 the post-smoothing (default 3, both). */

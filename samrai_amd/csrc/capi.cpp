@@ -49,8 +49,8 @@ namespace samrai_amd {
 // Used by entry points defined in other translation units (learned smoother).
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
-                          double* const* const* dtab, double logd_const, int nblocks, double h,
-                          double omega, hipStream_t s);
+                          double* const* const* dtab, double logd_const, const int* windows,
+                          int nblocks, double h, double omega, hipStream_t s);
 }  // namespace samrai_amd
 
 namespace {
@@ -655,10 +655,6 @@ int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac
     if (H.dim() != 3) fail("learned correction: 3-D hierarchies only");
     if (ln < 0 || ln >= H.numberOfLevels()) fail("learned correction: level out of range");
     const hier::PatchLevel& L = H.level(ln);
-    for (const Box& b : L.boxes)
-      for (int d = 0; d < 3; ++d)
-        if (b.hi[d] - b.lo[d] + 1 != 16)
-          fail("learned correction: every patch of the level must be 16x16x16");
     if (L.dx[0] != L.dx[1] || L.dx[0] != L.dx[2])
       fail("learned correction: cells must be cubes");
     if (u->d->centring() != pdat::PatchDataSet::kCell || r->d->centring() != pdat::PatchDataSet::kCell)
@@ -672,9 +668,13 @@ int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac
   double* const* dt[3] = {nullptr, nullptr, nullptr};
   if (D)
     for (int a = 0; a < 3; ++a) dt[a] = D->d->table(ln, a);
+  int nwin = 0;
+  const int* win = nullptr;
+  rc = guarded([&] { win = s->s->ops().learnedWindows(ln, &nwin); });
+  if (rc != 0) return rc;
   return samrai_amd::learned_correct_level(net, u->d->table(ln), u->d->ghost(), r->d->table(ln),
-                                           D ? dt : nullptr, 0.0, (int)L.boxes.size(), L.dx[0],
-                                           omega, s->s->ops().stream());
+                                           D ? dt : nullptr, 0.0, win, nwin, L.dx[0], omega,
+                                           s->s->ops().stream());
 }
 
 int sfac_ops_restrict_solution(sfac_solver* s, sfac_patch_data* u, int dest_ln) {

@@ -18,11 +18,12 @@
 #include <cstring>
 
 namespace samrai_amd {
-// learned_kernels.hip: the learned correction of a level of 16^3 patches
+// learned_kernels.hip: the learned correction of a level over 16^3 windows
 int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
-                          double* const* const* dtab, double logd_const, int nblocks, double h,
-                          double omega, hipStream_t s);
+                          double* const* const* dtab, double logd_const, const int* windows,
+                          int nblocks, double h, double omega, hipStream_t s);
 int learned_attach(void* net, int delta);
+int learned_max_blocks(void* net);
 
 namespace pdat {
 
@@ -210,6 +211,8 @@ CellPoissonFACOps::~CellPoissonFACOps() {
   if (d_learned) (void)learned_attach(d_learned, -1);
   deallocateOperatorState();
   for (hipEvent_t e : d_event_pool) (void)hipEventDestroy(e);
+  for (int* w : d_lwin)
+    if (w) (void)hipFree(w);
   (void)hipFree(d_stop);
   (void)hipFree(d_scalars);
   (void)hipHostFree(h_pinned);
@@ -1399,6 +1402,15 @@ void CellPoissonFACOps::setLearnedSmoother(void* net, double omega) {
 void CellPoissonFACOps::checkLearnedSpecification() {
   if (d_spec.cMode() != 0)
     fail("learned smoother: needs C = 0 (the network models div(D grad e) = r only)");
+  for (int ln = d_ln_min + 1; ln <= d_ln_max; ++ln) {
+    if (!learnedLevel(ln)) continue;
+    int nwin = 0;
+    learnedWindows(ln, &nwin);
+    if (nwin > samrai_amd::learned_max_blocks(d_learned))
+      fail("learned smoother: level " + std::to_string(ln) + " needs " + std::to_string(nwin) +
+           " blocks of 16^3, the network holds " +
+           std::to_string(samrai_amd::learned_max_blocks(d_learned)));
+  }
   if (d_spec.dIsConstant()) {
     const double d = d_spec.getDConstant();
     if (!(d > 0.0 && std::isfinite(d)))
@@ -1445,11 +1457,41 @@ bool CellPoissonFACOps::learnedLevel(int ln) const {
   if (d_h.dim() != 3 || ln <= d_ln_min) return false;
   if (d_spec.cMode() != 0) return false;
   const hier::PatchLevel& L = d_h.level(ln);
-  if (L.dx[0] != L.dx[1] || L.dx[0] != L.dx[2]) return false;
-  for (const Box& b : L.boxes)
-    for (int d = 0; d < 3; ++d)
-      if (b.hi[d] - b.lo[d] + 1 != 16) return false;
-  return true;
+  return L.dx[0] == L.dx[1] && L.dx[0] == L.dx[2];
+}
+
+// 16^3 windows tiling every patch of level ln from its lower corner (the
+// last window along an axis reaches past a patch whose extent is not a
+// multiple of 16; those cells are skipped), built once per level.
+const int* CellPoissonFACOps::learnedWindows(int ln, int* count) {
+  constexpr int kB = 16;
+  if ((int)d_lwin.size() < d_h.numberOfLevels()) {
+    d_lwin.resize(d_h.numberOfLevels(), nullptr);
+    d_lwin_n.resize(d_h.numberOfLevels(), 0);
+  }
+  if (!d_lwin[ln]) {
+    std::vector<int> w;
+    const auto& boxes = d_h.level(ln).boxes;
+    for (int p = 0; p < (int)boxes.size(); ++p) {
+      int n[3];
+      for (int d = 0; d < 3; ++d) n[d] = boxes[p].hi[d] - boxes[p].lo[d] + 1;
+      for (int z = 0; z < n[2]; z += kB)
+        for (int y = 0; y < n[1]; y += kB)
+          for (int x = 0; x < n[0]; x += kB) {
+            const int rec[8] = {p, x, y, z, n[0], n[1], n[2], 0};
+            w.insert(w.end(), rec, rec + 8);
+          }
+    }
+    int* dw = nullptr;
+    hip_check(hipMalloc(&dw, sizeof(int) * std::max<size_t>(w.size(), 8)), "hipMalloc(windows)");
+    if (!w.empty())
+      hip_check(hipMemcpy(dw, w.data(), sizeof(int) * w.size(), hipMemcpyHostToDevice),
+                "hipMemcpy(windows)");
+    d_lwin[ln] = dw;
+    d_lwin_n[ln] = (int)(w.size() / 8);
+  }
+  *count = d_lwin_n[ln];
+  return d_lwin[ln];
 }
 
 void CellPoissonFACOps::smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal& r, int ln,
@@ -1466,10 +1508,12 @@ void CellPoissonFACOps::smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal&
       for (int a = 0; a < 3; ++a) dt[a] = D->table(ln, a);
     // constant D: log D enters as a uniform channel
     const double logd_const = D ? 0.0 : std::log(d_spec.getDConstant());
+    int nwin = 0;
+    const int* win = learnedWindows(ln, &nwin);
     if (samrai_amd::learned_correct_level(d_learned, e.data->table(ln), e.data->ghost(),
                                           d_learned_r->table(ln), D ? dt : nullptr, logd_const,
-                                          (int)d_h.level(ln).boxes.size(), d_h.level(ln).dx[0],
-                                          d_learned_omega, d_stream) != 0)
+                                          win, nwin, d_h.level(ln).dx[0], d_learned_omega,
+                                          d_stream) != 0)
       fail("learned smoother: correction failed");
   }
   smoothErrorByRedBlack(*e.data, *r.data, ln, num_sweeps, -1.0);

@@ -183,6 +183,9 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   // solveCoarsestLevel end with that fill, so FACPreconditioner's recursive
   // cycle, which prolongs right after them, sets this around the call.
   void setProlongCoarseFilled(bool on) { d_prolong_coarse_filled = on; }
+  // 16^3 windows covering level ln's patches (device, 8 ints each) for the
+  // learned correction; count = the number of windows.
+  const int* learnedWindows(int ln, int* count);
   void prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAIVectorReal& d,
                               int dest_ln) override;
   void smoothError(SAMRAIVectorReal& e, const SAMRAIVectorReal& r, int ln,
@@ -336,6 +339,8 @@ class CellPoissonFACOps : public FACOperatorStrategy {
   std::vector<MGLevel> d_mg;  // [0] aliases the box arrays
   bool d_mg_small_direct = false;
   bool d_prolong_coarse_filled = false;
+  std::vector<int*> d_lwin;  // learned-correction windows per level (device)
+  std::vector<int> d_lwin_n;
   double* d_mg_partial = nullptr;
   double* d_mg_scal = nullptr;  // [0] ||b||^2, [1] ||r||^2, [2] relative residual
   int* d_mg_flag = nullptr;     // [0] stop, [1] iterations

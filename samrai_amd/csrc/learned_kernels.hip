@@ -501,37 +501,67 @@ __global__ __launch_bounds__(256) void k_block_residual(const double* __restrict
 // Level <-> blocks (every patch of the level is one 16^3 block).  Patch
 // arrays are ArrayData layout (x fastest over the ghost box); cell data with
 // ghost 0 and side data with ghost 0 have the block layouts directly.
-__global__ void k_gather_blocks(double* const* __restrict__ tab, int nblocks,
-                                double* __restrict__ out) {
+// The level's patches are covered by 16^3 windows (one network block each):
+// record w = {p, ox, oy, oz, nx, ny, nz, 0}: window cell (x, y, z) is cell
+// (ox + x, oy + y, oz + z) of patch p's interior (nx, ny, nz cells); window
+// cells beyond the patch read 0 and get no correction.  A 16^3 patch is one
+// window at the origin.
+struct WinCell {
+  int p, px, py, pz, nx, ny, nz;
+  bool inside;
+};
+__device__ inline WinCell win_cell(const int* __restrict__ win, long i) {
+  const int w = (int)(i / NCELL), c = (int)(i % NCELL);
+  const int* r = win + 8 * w;
+  WinCell q;
+  q.p = r[0];
+  q.px = r[1] + c % NB;
+  q.py = r[2] + (c / NB) % NB;
+  q.pz = r[3] + c / (NB * NB);
+  q.nx = r[4];
+  q.ny = r[5];
+  q.nz = r[6];
+  q.inside = q.px < q.nx && q.py < q.ny && q.pz < q.nz;
+  return q;
+}
+
+__global__ void k_gather_blocks(double* const* __restrict__ tab, const int* __restrict__ win,
+                                int nblocks, double* __restrict__ out) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= (long)nblocks * NCELL) return;
-  out[i] = tab[i / NCELL][i % NCELL];
+  const WinCell q = win_cell(win, i);
+  out[i] = q.inside ? tab[q.p][(q.pz * q.ny + q.py) * q.nx + q.px] : 0.0;
 }
 
 __global__ void k_logd_blocks(double* const* __restrict__ tx, double* const* __restrict__ ty,
-                              double* const* __restrict__ tz, int nblocks,
-                              double* __restrict__ logd) {
+                              double* const* __restrict__ tz, const int* __restrict__ win,
+                              int nblocks, double* __restrict__ logd) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= (long)nblocks * NCELL) return;
-  const int b = (int)(i / NCELL), c = (int)(i % NCELL);
-  const int x = c % NB, y = (c / NB) % NB, z = c / (NB * NB);
-  const double* X = tx[b];
-  const double* Y = ty[b];
-  const double* Z = tz[b];
-  const double s = log(X[(z * NB + y) * (NB + 1) + x]) + log(X[(z * NB + y) * (NB + 1) + x + 1]) +
-                   log(Y[(z * (NB + 1) + y) * NB + x]) + log(Y[(z * (NB + 1) + y + 1) * NB + x]) +
-                   log(Z[(z * NB + y) * NB + x]) + log(Z[((z + 1) * NB + y) * NB + x]);
+  const WinCell q = win_cell(win, i);
+  if (!q.inside) {
+    logd[i] = 0.0;
+    return;
+  }
+  const int x = q.px, y = q.py, z = q.pz, nx = q.nx, ny = q.ny;
+  const double* X = tx[q.p];
+  const double* Y = ty[q.p];
+  const double* Z = tz[q.p];
+  const double s = log(X[(z * ny + y) * (nx + 1) + x]) + log(X[(z * ny + y) * (nx + 1) + x + 1]) +
+                   log(Y[(z * (ny + 1) + y) * nx + x]) + log(Y[(z * (ny + 1) + y + 1) * nx + x]) +
+                   log(Z[(z * ny + y) * nx + x]) + log(Z[((z + 1) * ny + y) * nx + x]);
   logd[i] = s / 6.0;
 }
 
-__global__ void k_add_blocks(double* const* __restrict__ tab, int ghost, int nblocks,
+__global__ void k_add_blocks(double* const* __restrict__ tab, int ghost,
+                             const int* __restrict__ win, int nblocks,
                              const double* __restrict__ e) {
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= (long)nblocks * NCELL) return;
-  const int b = (int)(i / NCELL), c = (int)(i % NCELL);
-  const int x = c % NB, y = (c / NB) % NB, z = c / (NB * NB);
-  const int n = NB + 2 * ghost;
-  tab[b][((z + ghost) * n + (y + ghost)) * n + (x + ghost)] += e[i];
+  const WinCell q = win_cell(win, i);
+  if (!q.inside) return;
+  const int sx = q.nx + 2 * ghost, sy = q.ny + 2 * ghost;
+  tab[q.p][((q.pz + ghost) * sy + (q.py + ghost)) * sx + (q.px + ghost)] += e[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -752,26 +782,33 @@ int learned_attach(void* net, int delta) {
   });
 }
 
-// The learned correction of one level whose patches are all 16^3 blocks:
-// u += omega * s * net(r h^2 / s, log D), r cell data with ghost 0, D side
-// data with ghost 0 (dtab null: log D = logd_const everywhere), u cell data
-// with ghost `ughost`.
+// Blocks the network holds buffers for.
+int learned_max_blocks(void* net) {
+  return net ? static_cast<Net*>(net)->max_blocks : 0;
+}
+
+// The learned correction of one level over the 16^3 windows `windows`
+// (device, 8 ints each, see win_cell) covering its patches:
+// u += omega * s * net(r h^2 / s, log D) on every window cell inside its
+// patch, r cell data with ghost 0, D side data with ghost 0 (dtab null:
+// log D = logd_const everywhere), u cell data with ghost `ughost`.
 // Called by sfac_ops_learned_correct (capi.cpp) after its checks.
 int learned_correct_level(void* net, double* const* utab, int ughost, double* const* rtab,
-                          double* const* const* dtab, double logd_const, int nblocks, double h,
-                          double omega, hipStream_t s) {
+                          double* const* const* dtab, double logd_const, const int* windows,
+                          int nblocks, double h, double omega, hipStream_t s) {
   return guarded([&] {
     Net* n = net_of(net, nblocks);
     need(utab && rtab, "null patch tables");
+    need(windows || nblocks == 0, "null window table");
     need(h > 0.0, "h must be positive");
     if (nblocks == 0) return;
     const int ncells = nblocks * NCELL;
     const dim3 grid((ncells + 255) / 256), block(256);
-    hipLaunchKernelGGL(k_gather_blocks, grid, block, 0, s, rtab, nblocks, n->blk[0]);
+    hipLaunchKernelGGL(k_gather_blocks, grid, block, 0, s, rtab, windows, nblocks, n->blk[0]);
     const double* logd = nullptr;
     if (dtab) {
-      hipLaunchKernelGGL(k_logd_blocks, grid, block, 0, s, dtab[0], dtab[1], dtab[2], nblocks,
-                         n->blk[1]);
+      hipLaunchKernelGGL(k_logd_blocks, grid, block, 0, s, dtab[0], dtab[1], dtab[2], windows,
+                         nblocks, n->blk[1]);
       logd = n->blk[1];
     }
     hipLaunchKernelGGL(k_features, dim3(nblocks), dim3(256), 0, s, n->blk[0], logd, logd_const, h * h,
@@ -779,7 +816,8 @@ int learned_correct_level(void* net, double* const* utab, int ughost, double* co
     forward(n, nblocks, s);
     hipLaunchKernelGGL(k_correction, grid, block, 0, s, n->pred, n->scale, omega, ncells,
                        n->blk[2]);
-    hipLaunchKernelGGL(k_add_blocks, grid, block, 0, s, utab, ughost, nblocks, n->blk[2]);
+    hipLaunchKernelGGL(k_add_blocks, grid, block, 0, s, utab, ughost, windows, nblocks,
+                       n->blk[2]);
   });
 }
 }  // namespace samrai_amd

@@ -226,7 +226,8 @@ class LearnedSmoother:
 
 
 def learned_correct(solver, net, u, r, D, ln, omega=1.0):
-    """u += omega * s * net(r h^2 / s, log D) on every 16^3 patch of level ln
+    """u += omega * s * net(r h^2 / s, log D) over the 16^3 windows covering
+    the patches of level ln
     (sfac_ops_learned_correct); r: cell data with ghost 0, D: side data or None."""
     fac._check(lib().sfac_ops_learned_correct(solver.s, net.h, u.d, r.d,
                                               None if D is None else D.d, int(ln), float(omega)))
