@@ -337,6 +337,7 @@ def learned_section(torch, seeds, steps, batch, log=print):
         fac_solves[name] = dict(s_per_solve=round(dt, 4), cycles=sv.number_of_iterations(),
                                 converged=bool(ok), final_residual=float(sv.history()[-1]),
                                 learned_still_active=SL.learned_active(sv),
+                                rollbacks=SL.learned_rollbacks(sv),
                                 max_diff_vs_rbgs=float((uf - u_ref).abs().max()))
     SL.set_learned_smoother(sv, None)
     sv.close()
@@ -363,6 +364,7 @@ def learned_section(torch, seeds, steps, batch, log=print):
         c2_solves[name] = dict(s_per_solve=round(dt, 4), cycles=s2.number_of_iterations(),
                                converged=bool(ok), final_residual=float(s2.history()[-1]),
                                learned_still_active=SL.learned_active(s2),
+                               rollbacks=SL.learned_rollbacks(s2),
                                first_factors=[round(b / a, 4) for a, b in
                                               zip(s2.history()[:4], s2.history()[1:5])])
     SL.set_learned_smoother(s2, None)

@@ -134,6 +134,10 @@ int sfac_solver_set_learned_phases(sfac_solver* s, int mask);
 /* 1 if the learned correction is attached and enabled (it is switched off
  * by the fallback), 0 if not, -1 for a null solver. */
 int sfac_solver_learned_active(sfac_solver* s);
+/* Cycles of the last solveSystem that the fallback undid (0 or 1: after
+ * one the correction stays off for the rest of that solve), -1 for a null
+ * solver. */
+int sfac_solver_learned_rollbacks(sfac_solver* s);
 
 #ifdef __cplusplus
 }

@@ -138,8 +138,10 @@ void launch_mg_sumsq(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, 
 int mg_lds_edge();
 void launch_mg_lds_sweeps(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, double* u,
                           int sweeps, int zero_in, const int* stop, hipStream_t s);
-// Relative-residual test after iteration it (it < 0: b = 0 test only).
-void launch_mg_check(const double* rr, const double* bb, double tol, int it, int* stop,
-                     int* iters, double* relres, hipStream_t s);
+// *out = the sum of n partials (launch_sum's order), then the
+// relative-residual test of iteration it on rr = out (it < 0: the b = 0 test
+// only), in one launch (nothing once *stop is set).
+void launch_mg_sum_check(const double* partial, int n, double* out, const double* bb, double tol,
+                         int it, int* stop, int* iters, double* relres, hipStream_t s);
 
 }  // namespace sfac

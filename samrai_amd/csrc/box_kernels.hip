@@ -856,9 +856,9 @@ k_mg_sumsq(BoxLevel G, BoxArraysT<T> A, RobinDev bc, const T* u, const T* x, dou
 
 // Convergence test after iteration `it` (it < 0: before the first): stop
 // when ||r|| <= tol ||b|| (or b = 0), as hypre's relative-residual test.
-__global__ void k_mg_check(const double* rr, const double* bb, double tol, int it, int* stop,
-                           int* iters, double* relres) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || *stop) return;
+// Relative-residual test after iteration it (it < 0: the b = 0 test only).
+__device__ inline void mg_check(const double* rr, const double* bb, double tol, int it, int* stop,
+                                int* iters, double* relres) {
   const double b = sqrt(*bb);
   if (it < 0) {
     if (!(b > 0.0)) {
@@ -872,6 +872,29 @@ __global__ void k_mg_check(const double* rr, const double* bb, double tol, int i
   *iters = it + 1;
   *relres = rel;
   if (!(rel > tol)) *stop = 1;
+}
+
+// *out = sum of the n partials (in k_sum's order: a strided sum per thread,
+// then the wavefront and work-group reductions), then the relative-residual
+// test on it, in one launch; nothing once the solve has stopped.
+constexpr int kMGSumThreads = 1024;
+__global__ void __launch_bounds__(kMGSumThreads)
+k_mg_sum_check(const double* partial, int n, double* out, const double* bb, double tol, int it,
+               int* stop, int* iters, double* relres) {
+  if (*stop) return;
+  double acc = 0.0;
+#pragma unroll 4
+  for (int t = threadIdx.x; t < n; t += kMGSumThreads) acc += partial[t];
+  __shared__ double sm[kMGSumThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+    for (int w = 0; w < kMGSumThreads / 64; ++w) v += sm[w];
+    out[0] = v;
+    mg_check(out, bb, tol, it, stop, iters, relres);
+  }
 }
 
 }  // namespace
@@ -1125,9 +1148,10 @@ void launch_mg_sumsq(const BoxLevel& G, const BoxArrays& A, const RobinDev& bc, 
     dispatch3<SumsqRun>(G, typed<double>(A), bc, u, x, partial, stop, s);
 }
 
-void launch_mg_check(const double* rr, const double* bb, double tol, int it, int* stop,
-                     int* iters, double* relres, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(64), 0, s, rr, bb, tol, it, stop, iters, relres);
+void launch_mg_sum_check(const double* partial, int n, double* out, const double* bb, double tol,
+                         int it, int* stop, int* iters, double* relres, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_sum_check, dim3(1), dim3(kMGSumThreads), 0, s, partial, n, out, bb, tol,
+                     it, stop, iters, relres);
   check_launch();
 }
 

@@ -644,6 +644,11 @@ int sfac_solver_learned_active(sfac_solver* s) {
   return s->s->ops().learnedActive() ? 1 : 0;
 }
 
+int sfac_solver_learned_rollbacks(sfac_solver* s) {
+  if (!s) return -1;
+  return s->s->preconditioner().rollbacks();
+}
+
 int sfac_ops_learned_correct(sfac_solver* s, void* net, sfac_patch_data* u, sfac_patch_data* r,
                              sfac_patch_data* D, int ln, double omega) {
   int rc = guarded([&] {

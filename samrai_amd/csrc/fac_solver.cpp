@@ -971,18 +971,18 @@ void CellPoissonFACOps::solveBoxMG(PatchDataSet& e, const PatchDataSet& r) {
   sfac::launch_fill_int(d_mg_flag, 2, 0, d_stream);
   sfac::launch_mg_sumsq(d_mg[0].G, mgArrays(0), bc, nullptr, d_mg[0].f, d_mg_partial, d_mg_flag,
                         d_stream);
-  sfac::launch_sum(d_mg_partial, P, d_mg_scal, 0, d_stream);
   // FP32 cannot meet relative tolerances near its rounding level
   const double tol = d_box.fp32 ? std::max(d_coarse_tol, kCoarseFP32Tol) : d_coarse_tol;
-  sfac::launch_mg_check(d_mg_scal + 1, d_mg_scal, tol, -1, d_mg_flag, d_mg_flag + 1,
-                        d_mg_scal + 2, d_stream);
+  // ||b||^2 into d_mg_scal[0] and the b = 0 test; after each V-cycle
+  // ||r||^2 into d_mg_scal[1] and the relative-residual test
+  sfac::launch_mg_sum_check(d_mg_partial, P, d_mg_scal, d_mg_scal, tol, -1, d_mg_flag,
+                            d_mg_flag + 1, d_mg_scal + 2, d_stream);
   for (int it = 0; it < d_coarse_max; ++it) {
     mgVcycle(0);
     sfac::launch_mg_sumsq(d_mg[0].G, mgArrays(0), bc, d_mg[0].e[0], nullptr, d_mg_partial,
                           d_mg_flag, d_stream);
-    sfac::launch_sum(d_mg_partial, P, d_mg_scal + 1, 0, d_stream);
-    sfac::launch_mg_check(d_mg_scal + 1, d_mg_scal, tol, it, d_mg_flag, d_mg_flag + 1,
-                          d_mg_scal + 2, d_stream);
+    sfac::launch_mg_sum_check(d_mg_partial, P, d_mg_scal + 1, d_mg_scal, tol, it, d_mg_flag,
+                              d_mg_flag + 1, d_mg_scal + 2, d_stream);
   }
   sfac::launch_box_cells(T.cells.launch(), T.geom, e.table(ln), e.ghost(), d_box.e[0],
                          d_box.e[0], nullptr, 0, d_box.G, 0, d_stream, d_box.fp32,

@@ -60,6 +60,7 @@ def lib():
     L.sfac_ops_learned_correct.argtypes = [_vp, _vp, _vp, _vp, _vp, i, d]
     L.sfac_solver_set_learned_smoother.argtypes = [_vp, _vp, d]
     L.sfac_solver_learned_active.argtypes = [_vp]
+    L.sfac_solver_learned_rollbacks.argtypes = [_vp]
     L.sfac_solver_set_learned_phases.argtypes = [_vp, i]
     _bound = True
     return L
@@ -252,6 +253,11 @@ def learned_active(solver):
     return lib().sfac_solver_learned_active(solver.s) == 1
 
 
+def learned_rollbacks(solver):
+    """Cycles of the last solve that the fallback undid."""
+    return int(lib().sfac_solver_learned_rollbacks(solver.s))
+
+
 class BlockLevelProblem:
     """A seeded hierarchy tiled by 16^3 patches on the unit cube (one n^3 level,
     or with two_level an (n/2)^3 level under a full n^3 level):
@@ -286,10 +292,19 @@ class BlockLevelProblem:
         self.hier.finalize()
         self.u = fac.PatchData(self.hier, 1)
         self.f = fac.PatchData(self.hier, 0)
-        self.r = fac.PatchData(self.hier, 0)
+        # r over device tensors: the fine level's patches are the blocks of one
+        # [patches, 16, 16, 16] tensor, so its residual needs no gather
+        kw = dict(dtype=torch.float64, device="cuda")
+        self.rblk = torch.zeros((len(self.boxes), NB, NB, NB), **kw)
+        arrays = []
+        for ln, boxes in enumerate(deck["levels"]):
+            if ln == self.fine:
+                arrays += [self.rblk[p] for p in range(len(boxes))]
+            else:
+                arrays += [torch.zeros((NB, NB, NB), **kw) for _ in boxes]
+        self.r = fac.PatchData.wrap(self.hier, 0, arrays)
         self.D = fac.PatchData(self.hier, 0, "side")
         modes = decks.grf_modes(seed)
-        kw = dict(dtype=torch.float64, device="cuda")
         faces = [[], [], []]
         for ln, boxes in enumerate(deck["levels"]):
             for p, box in enumerate(boxes):
@@ -316,10 +331,9 @@ class BlockLevelProblem:
         return s
 
     def residual_blocks(self):
-        """The residual patch data as a [patches, 16, 16, 16] device tensor."""
-        torch = self.torch
-        return torch.stack([torch.as_tensor(self.r.get(self.fine, p), device="cuda").view(NB, NB, NB)
-                            for p in range(len(self.boxes))]).contiguous()
+        """The fine level's residual as a [patches, 16, 16, 16] device tensor
+        (a copy of the tensor the residual patch data is bound to)."""
+        return self.rblk.clone()
 
     def close(self):
         for x in (self.u, self.f, self.r, self.D):
@@ -352,7 +366,9 @@ def fac_training_pairs(torch, seed, cycles=(1, 2, 3), n=64, coarse_sweeps=2, sor
     hh.finalize()
     u = fac.PatchData(hh, 1)
     f = fac.PatchData(hh, 0)
-    rv = fac.PatchData(hh, 0)
+    # the residual is written straight into the block tensor (patch p = block p)
+    rblk = torch.empty((len(boxes), NB, NB, NB), dtype=torch.float64, device="cuda")
+    rv = fac.PatchData.wrap(hh, 0, [rblk[p] for p in range(len(boxes))])
     D = fac.PatchData(hh, 0, "side")
     modes = decks.grf_modes(seed)
     kw = dict(dtype=torch.float64, device="cuda")
@@ -382,8 +398,7 @@ def fac_training_pairs(torch, seed, cycles=(1, 2, 3), n=64, coarse_sweeps=2, sor
         s.solve_system(u, f)
         s.residual(rv, u, f, 0, False)
         s.synchronize()
-        rs.append(torch.stack([torch.as_tensor(rv.get(0, p), device="cuda").view(NB, NB, NB)
-                               for p in range(len(boxes))]))
+        rs.append(rblk.clone())
         s.close()
     r = torch.cat(rs).contiguous()
     k = len(cycles)

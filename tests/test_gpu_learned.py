@@ -388,3 +388,64 @@ def test_learned_correction_on_patches_of_any_size(torch_gpu):
             assert np.array_equal(v, want)
     s.close()
     net.close()
+
+
+def test_fac_solve_with_learned_smoother_on_uneven_patches(torch_gpu):
+    """Whole solves on a reduced variable-coefficient deck of the batched
+    family (3 levels, patch sides 8..16, so the finer levels are covered by
+    partial 16^3 windows) with the learned correction in every smoothing
+    call and in post-smoothing only: each converges to 1e-10 and agrees with
+    the red-black solve to that level, with at most one cycle undone by the
+    fallback; a solve that repeats the red-black history undid its first
+    learned cycle and ends on the red-black solution bit for bit."""
+    torch = torch_gpu
+    from samrai_amd import decks, fac
+    from samrai_amd import learned as SL
+    deck = decks.deck_3d_c2(seed=2, base=32, nlevels=3, smin=8, smax=16)
+    sides = {hi[d] - lo[d] + 1 for lv in deck["levels"][1:] for lo, hi in lv for d in range(3)}
+    assert sides != {16}
+    nwin = max(sum(int(np.prod([-(-(hh - ll + 1) // 16) for ll, hh in zip(*b)])) for b in lv)
+               for lv in deck["levels"][1:])
+    _say("learned FAC solve on a reduced uneven deck: %s patches, %d windows at most per level"
+         % ([len(l) for l in deck["levels"]], nwin))
+    train = [SL.fac_training_pairs(torch, s, cycles=(1, 2)) for s in range(4)]
+    cat = {k: torch.cat([p[k] for p in train]).contiguous() for k in ("r", "logd", "e")}
+    net = SL.LearnedSmoother(torch, max(128, nwin), seed=3)
+    net.train(cat["r"], cat["logd"], cat["e"], train[0]["h"], steps=600, batch=32, lr=2e-3)
+    params = dict(max_cycles=60, residual_tol=1e-300, relative_residual_tol=1e-10,
+                  num_pre_sweeps=2, num_post_sweeps=2, coarse_solver_choice="mg",
+                  coarse_solver_tolerance=1e-8, coarse_solver_max_iterations=10,
+                  prolongation_method="CONSERVATIVE_LINEAR_REFINE")
+    _, h, u, f, D, s = decks.build_c2_problem(fac, torch, params, deck=deck)
+
+    def solution():
+        return [np.asarray(u.get(ln, p)) for ln, lv in enumerate(deck["levels"])
+                for p in range(len(lv))]
+
+    out = {}
+    for name, on, phases in (("rbgs", False, 3), ("both", True, 3), ("post", True, 2)):
+        SL.set_learned_smoother(s, net if on else None)
+        SL.set_learned_phases(s, phases)
+        u.fill(0.0)
+        ok = s.solve_system(u, f)
+        s.synchronize()
+        out[name] = (ok, list(s.history()), solution(), SL.learned_rollbacks(s))
+        _say("%s: converged %s in %d cycles, rollbacks %d" % (name, ok, len(out[name][1]),
+                                                            out[name][3]))
+    ok0, hist0, u0, rb0 = out["rbgs"]
+    assert ok0 and rb0 == 0
+    scale = max(np.abs(a).max() for a in u0)
+    for name in ("both", "post"):
+        ok, hist, u1, rb = out[name]
+        assert ok and rb in (0, 1)
+        assert max(np.abs(a - b).max() for a, b in zip(u1, u0)) <= 1e-7 * scale
+        if hist == hist0:
+            # the fallback undid the first learned cycle: red-black from there
+            assert rb == 1
+            assert all(np.array_equal(a, b) for a, b in zip(u1, u0))
+    SL.set_learned_smoother(s, None)
+    s.close()
+    for x in (u, f, D):
+        x.close()
+    h.close()
+    net.close()
