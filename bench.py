@@ -220,19 +220,26 @@ LEARNED_MACS_PER_CELL = 27 * (2 * 16 + 16 * 16 + 16 * 1)
 LEARNED_PADDED_MACS_PER_CELL = 27 * 16 * 16 * 3
 
 
-def learned_section(torch, seeds, steps, batch, log=print):
-    """C4 (learned smoother): pairs generated by the FAC solver on seeded
-    variable-coefficient problems (one-level 64^3 hierarchies of 16^3 patches,
-    residuals after 1, 2 and 3 FAC cycles, targets the FP64 block solutions),
+def learned_section(torch, seeds, steps, batch, log=print, n=128, cycles=(1, 2, 3, 4, 5, 6, 7, 8)):
+    """C4 (learned smoother): pairs generated on the device by the FAC solver
+    on seeded variable-coefficient (D, f) draws (one-level n^3 hierarchies of
+    16^3 patches, residuals after each cycle count in `cycles`, targets the
+    FP64 block solutions; 50 draws at n = 128 and cycles 1..8 give 204,800),
     training on the GPU by the library's own forward/backward kernels, the
     matrix-core rate of the forward convolutions, the gain on unseen seeds, and
     whole solves with the correction on its own two-level problem and on C2."""
     from samrai_amd import decks, fac
     from samrai_amd import learned as SL
     t0 = time.perf_counter()
-    train = [SL.fac_training_pairs(torch, s, cycles=(1, 2, 3)) for s in range(seeds)]
-    test = [SL.fac_training_pairs(torch, 1000 + s, cycles=(1, 2, 3)) for s in range(2)]
+    train = []
+    for sd in range(seeds):
+        train.append(SL.fac_training_pairs(torch, sd, cycles=cycles, n=n, faces=False))
+        if (sd + 1) % 10 == 0:
+            log("bench: learned: pairs from %d of %d draws (%.1f s)" %
+                (sd + 1, seeds, time.perf_counter() - t0))
+    test = [SL.fac_training_pairs(torch, 1000 + s, cycles=cycles, n=n) for s in range(2)]
     cat = {k: torch.cat([p[k] for p in train]).contiguous() for k in ("r", "logd", "e")}
+    del train[1:]
     tst = {k: torch.cat([p[k] for p in test]).contiguous()
            for k in ("r", "logd", "e", "dxf", "dyf", "dzf")}
     h = train[0]["h"]
@@ -292,7 +299,8 @@ def learned_section(torch, seeds, steps, batch, log=print):
     P = SL.BlockLevelProblem(torch, 2000)
     sv = P.solver()
     level = {}
-    for learned in (False, True):
+
+    def level_run(learned, sweeps):
         P.u.fill(0.0)
         hist = []
         t_it = 0.0
@@ -302,13 +310,22 @@ def learned_section(torch, seeds, steps, batch, log=print):
             if learned:
                 sv.residual(P.r, P.u, P.f, 0, False)
                 SL.learned_correct(sv, net, P.u, P.r, P.D, 0)
-            sv.smooth(P.u, P.f, 0, 2)
+            sv.smooth(P.u, P.f, 0, sweeps)
             sv.synchronize()
             t_it += time.perf_counter() - t2
             sv.residual(P.r, P.u, P.f, 0, False)
             hist.append(sv.rms_norm(P.r))
-        level["learned_then_rbgs2" if learned else "rbgs2"] = dict(
-            rms_residual=[float("%.4e" % v) for v in hist], ms_per_iteration=round(t_it / 8 * 1e3, 3))
+        return dict(rms_residual=[float("%.4e" % v) for v in hist],
+                    ms_per_iteration=round(t_it / 8 * 1e3, 3))
+
+    for learned in (False, True):
+        level_run(learned, 2)  # warm-up
+        level["learned_then_rbgs2" if learned else "rbgs2"] = level_run(learned, 2)
+    # red-black alone at the learned iteration's cost (sweeps scaled by the
+    # measured time per iteration)
+    k_eq = max(2, int(round(2 * level["learned_then_rbgs2"]["ms_per_iteration"] /
+                            level["rbgs2"]["ms_per_iteration"])))
+    level["rbgs%d_equal_cost" % k_eq] = level_run(False, k_eq)
     sv.close()
     P.close()
     # whole FAC solves to 1e-10 on a two-level problem (64^3 + 128^3 in 16^3
@@ -381,10 +398,11 @@ def learned_section(torch, seeds, steps, batch, log=print):
                            **c2_solves),
         level_smoothing=dict(workload="one-level 64^3 problem of 16^3 patches (unseen seed), "
                                       "8 iterations from u = 0", **level),
-        workload="pairs from FAC iterates (1-3 cycles, 2 coarse RBGS sweeps per cycle) on "
-                 "one-level 64^3 hierarchies of 16^3 patches, D = exp(GRF corr 0.1 sigma 1), "
-                 "8 Gaussian sources; target = FP64 solution of the block error equation "
-                 "(zero Dirichlet on the block faces, 400 SOR sweeps)",
+        workload="pairs from FAC iterates (cycles %s, 2 coarse RBGS sweeps per cycle) on "
+                 "one-level %d^3 hierarchies of 16^3 patches, D = exp(GRF corr 0.1 sigma 1), "
+                 "8 Gaussian sources, one (D, f) draw per seed; target = FP64 solution of the "
+                 "block error equation (zero Dirichlet on the block faces, 400 SOR sweeps)"
+                 % (",".join(str(c) for c in cycles), n),
         network="3x3x3 conv 2->16->16->1, ReLU, BF16 weights/activations, FP32 accumulation "
                 "(v_mfma_f32_16x16x16_bf16), Adam on FP32 master weights",
         train_blocks=int(cat["r"].shape[0]), train_seeds=seeds, test_blocks=int(tst["r"].shape[0]),
@@ -424,6 +442,10 @@ def main():
     ap.add_argument("--learned-seeds", type=int, default=50)
     ap.add_argument("--learned-steps", type=int, default=1500)
     ap.add_argument("--learned-batch", type=int, default=64)
+    ap.add_argument("--learned-n", type=int, default=128,
+                    help="cells per axis of each training draw's one-level hierarchy")
+    ap.add_argument("--learned-cycles", default="1,2,3,4,5,6,7,8",
+                    help="FAC cycle counts whose residuals become training pairs")
     ap.add_argument("--mode", choices=["solve", "smoother", "profile", "learned"],
                     default="solve",
                     help="smoother: a 256 MiB copy of known size and a few smoother calls on "
@@ -452,9 +474,11 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
 
+    learned_cycles = tuple(int(c) for c in args.learned_cycles.split(","))
     if args.mode == "learned":
         log("bench: learned %s" % json.dumps(learned_section(
-            torch, args.learned_seeds, args.learned_steps, args.learned_batch, log)))
+            torch, args.learned_seeds, args.learned_steps, args.learned_batch, log,
+            args.learned_n, learned_cycles)))
         return
 
     def problem(deck_name, params):
@@ -611,7 +635,7 @@ def main():
     learned = None
     if not args.no_learned:
         learned = learned_section(torch, args.learned_seeds, args.learned_steps,
-                                  args.learned_batch, log)
+                                  args.learned_batch, log, args.learned_n, learned_cycles)
         log("bench: learned %s" % json.dumps(learned))
 
     cpu = None

@@ -89,12 +89,16 @@ def split(params):
 
 
 def features(r, logd, h):
-    """X[b, z, y, x, :] = (r h^2 / s, log D, 0, ...), s = RMS over the block of r h^2."""
+    """X[b, z, y, x, :] = (r h^2 / s, log D, 0, ...), s = RMS over the block of r h^2.
+    A block whose residual is zero has s = 0: its first channel is 0, and
+    the correction s * pred and its loss term are 0."""
     rh = r * h * h
     s = np.sqrt(np.mean(rh * rh, axis=(1, 2, 3)))
-    s = np.where(s > 0, s, 1.0)
+    live = s > 0
     x = np.zeros(r.shape + (CH,), np.float32)
-    x[..., 0] = (rh / s[:, None, None, None]).astype(np.float32)
+    x[..., 0] = np.where(live[:, None, None, None],
+                         rh / np.where(live, s, 1.0)[:, None, None, None], 0.0).astype(np.float32)
+    s = np.where(live, s, 0.0)
     if logd is not None:
         x[..., 1] = logd.astype(np.float32)
     return x, s
@@ -124,11 +128,12 @@ def loss_and_grads(layers, x, target, scale, emulate=True):
     acts = forward(layers, x, emulate)
     pred = acts[3]
     n = pred.size
-    tgt = target / scale[:, None, None, None]
+    live = (scale > 0)[:, None, None, None]
+    tgt = target / np.where(scale > 0, scale, 1.0)[:, None, None, None]
     if emulate:
-        d = pred - tgt.astype(np.float32)
+        d = np.where(live, pred - tgt.astype(np.float32), np.float32(0.0))
     else:
-        d = pred - tgt
+        d = np.where(live, pred - tgt, 0.0)
     loss = float(np.sum(np.asarray(d, np.float64) ** 2) / n)
     g = np.zeros(pred.shape + (CH,), np.float32 if emulate else np.float64)
     g[..., 0] = 2.0 * d / n

@@ -277,15 +277,17 @@ __global__ __launch_bounds__(256) void k_features(const double* __restrict__ r,
     if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
     __syncthreads();
   }
-  double s = sqrt(red[0] / NCELL);
-  if (!(s > 0.0)) s = 1.0;
-  if (threadIdx.x == 0) scale[b] = s;
+  // a block whose residual is zero (or not finite) gets scale 0: zero
+  // features, a zero correction and no loss term
+  const double s = sqrt(red[0] / NCELL);
+  const bool live = s > 0.0;
+  if (threadIdx.x == 0) scale[b] = live ? s : 0.0;
   uint16_t* xb = x + (size_t)b * NCELL * CH;
   for (int c = threadIdx.x; c < NCELL; c += 256) {
     uint16_t v[CH];
 #pragma unroll
     for (int k = 0; k < CH; ++k) v[k] = 0;
-    v[0] = f2bf((float)(rb[c] * h2 / s));
+    v[0] = live ? f2bf((float)(rb[c] * h2 / s)) : (uint16_t)0;
     v[1] = f2bf((float)(logd ? logd[(size_t)b * NCELL + c] : logd_const));
     uint4* dst = reinterpret_cast<uint4*>(xb + (size_t)c * CH);
     dst[0] = make_uint4(v[0] | (v[1] << 16), 0u, 0u, 0u);
@@ -304,7 +306,8 @@ __global__ __launch_bounds__(256) void k_loss(const float* __restrict__ pred,
   const int c = blockIdx.x * 256 + threadIdx.x;
   float d2 = 0.f;
   if (c < ncells) {
-    const float d = pred[c] - (float)(target[c] / scale[c / NCELL]);
+    const double sc = scale[c / NCELL];
+    const float d = sc > 0.0 ? pred[c] - (float)(target[c] / sc) : 0.f;
     d2 = d * d;
     uint4* dst = reinterpret_cast<uint4*>(gout + (size_t)c * CH);
     dst[0] = make_uint4((uint32_t)f2bf(2.f * d * inv_n), 0u, 0u, 0u);

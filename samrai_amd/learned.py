@@ -342,17 +342,23 @@ class BlockLevelProblem:
 
 
 def fac_training_pairs(torch, seed, cycles=(1, 2, 3), n=64, coarse_sweeps=2, sor_sweeps=400,
-                       sor_omega=1.8):
+                       sor_omega=1.8, faces=True):
     """(residual, log D, block error, D faces) for every 16^3 block of a
     one-level n^3 hierarchy (patches of 16^3) after each FAC cycle count in
-    `cycles`, on a seeded problem: D = exp(g), g a Gaussian random field of
-    correlation length 0.1 and unit variance, f = 8 Gaussian sources of width
-    0.02, homogeneous Dirichlet.  The FAC solver (red-black coarse solve with
-    `coarse_sweeps` sweeps per cycle) produces the iterates; the residual is
-    its composite residual; the target is the FP64 solution of the block
-    problem div(D grad e) = r with zero Dirichlet data on the block faces."""
+    `cycles` (increasing), on a seeded problem: D = exp(g), g a Gaussian
+    random field of correlation length 0.1 and unit variance, f = 8 Gaussian
+    sources of width 0.02, homogeneous Dirichlet.  The FAC solver (red-black
+    coarse solve with `coarse_sweeps` sweeps per cycle) produces the iterates,
+    one cycle at a time from u = 0; the residual is its composite residual;
+    the target is the FP64 solution of the block problem div(D grad e) = r
+    with zero Dirichlet data on the block faces.  Everything stays on the
+    device; faces=False leaves the D face blocks out of the result (they are
+    needed only to judge a correction by its residual).  n = 128 with cycles
+    1..8 gives 4096 pairs per seed."""
     from . import decks
     assert n % NB == 0
+    cycles = [int(c) for c in cycles]
+    assert all(c > 0 for c in cycles) and cycles == sorted(set(cycles)), "cycles must increase"
     m = n // NB
     rng = np.random.default_rng(5000 + seed)
     deck = dict(name="learned-pairs-s%d" % seed, dim=3, ratio=2, dlo=(0, 0, 0),
@@ -386,20 +392,24 @@ def fac_training_pairs(torch, seed, cycles=(1, 2, 3), n=64, coarse_sweeps=2, sor
             torch.log(dyf[:, :, 1:]) + torch.log(dzf[:, :-1]) + torch.log(dzf[:, 1:])) / 6.0
     h = 1.0 / n
     rs = []
+    # one-cycle solves continue from the current iterate, so the residuals
+    # after 1, 2, ... cycles come from one pass
+    s = fac.CellPoissonFACSolver(hh, max_cycles=1, relative_residual_tol=1e-30,
+                                 coarse_solver_choice="redblack", coarse_solver_tolerance=1e-30,
+                                 coarse_solver_max_iterations=int(coarse_sweeps))
+    s.set_d_patch_data(D)
+    s.set_dirichlet(0.0)
+    u.fill(0.0)
+    s.initialize_solver_state(u, f)
+    done = 0
     for c in cycles:
-        s = fac.CellPoissonFACSolver(hh, max_cycles=int(c), relative_residual_tol=1e-30,
-                                     coarse_solver_choice="redblack",
-                                     coarse_solver_tolerance=1e-30,
-                                     coarse_solver_max_iterations=int(coarse_sweeps))
-        s.set_d_patch_data(D)
-        s.set_dirichlet(0.0)
-        u.fill(0.0)
-        s.initialize_solver_state(u, f)
-        s.solve_system(u, f)
+        while done < c:
+            s.solve_system(u, f)
+            done += 1
         s.residual(rv, u, f, 0, False)
-        s.synchronize()
         rs.append(rblk.clone())
-        s.close()
+    s.synchronize()
+    s.close()
     r = torch.cat(rs).contiguous()
     k = len(cycles)
     dxf, dyf, dzf, logd = (t.repeat(k, 1, 1, 1).contiguous() for t in (dxf, dyf, dzf, logd))
@@ -407,4 +417,6 @@ def fac_training_pairs(torch, seed, cycles=(1, 2, 3), n=64, coarse_sweeps=2, sor
     for x in (u, f, rv, D):
         x.close()
     hh.close()
+    if not faces:
+        return dict(r=r, logd=logd, e=e, h=h)
     return dict(r=r, logd=logd, e=e, dxf=dxf, dyf=dyf, dzf=dzf, h=h)

@@ -98,21 +98,24 @@ def _problem(torch, rng, nb):
 def test_network_loss_and_gradients_match_oracle(torch_gpu):
     torch = torch_gpu
     from samrai_amd import learned as SL
-    _say("network loss and gradients, 2 blocks")
+    _say("network loss and gradients, 3 blocks (the last with a zero residual)")
     rng = np.random.default_rng(30)
     net = SL.LearnedSmoother(torch, 4, seed=1)
     layers = _rand_layers(rng)
     net.set_params(layers)
     np.testing.assert_array_equal(np.concatenate(net.params()), np.concatenate(layers))
-    r, logd, target = _problem(torch, rng, 2)
+    r, logd, target = _problem(torch, rng, 3)
+    r[2] = 0.0  # scale 0: no features, no correction, no loss term
     h = 1.0 / 64
     net.forward(r, logd, h)
     net.backward(target)
     torch.cuda.synchronize()
     feat_ptr, _, scale_ptr = net.buffers()
     # features as the device made them (bits), and its block scales
-    feat = _wrap(torch, feat_ptr, 2 * 4096 * 16, torch.int16).view(2, 16, 16, 16, 16)
-    scale = _wrap(torch, scale_ptr, 2, torch.float64).cpu().numpy()
+    feat = _wrap(torch, feat_ptr, 3 * 4096 * 16, torch.int16).view(3, 16, 16, 16, 16)
+    scale = _wrap(torch, scale_ptr, 3, torch.float64).cpu().numpy()
+    assert scale[2] == 0.0 and scale[0] > 0.0
+    assert not bool(feat[2, ..., 0].any())
     x = _f32(feat)
     xo, so = L.features(r.cpu().numpy(), logd.cpu().numpy(), h)
     np.testing.assert_allclose(scale, so, rtol=1e-13)
@@ -126,6 +129,8 @@ def test_network_loss_and_gradients_match_oracle(torch_gpu):
         err = np.linalg.norm(grads[l] - ref) / np.linalg.norm(ref)
         _say("layer %d: relative gradient error %.2e" % (l, err))
         assert err < 1e-4, (l, err)
+    e = net.correction(r, logd, h)
+    assert not bool(e[2].any()) and bool(e[0].any())
     net.close()
 
 

@@ -146,3 +146,24 @@ def test_block_sor_one_sweep_is_red_black_gauss_seidel():
                             den += c
                     ref[z, y, x] = num / den
     np.testing.assert_allclose(e[0], ref, rtol=1e-12, atol=1e-14)
+
+
+def test_zero_residual_block_has_no_features_and_no_loss():
+    """A block whose residual is zero gets scale 0, a zero first channel and
+    no loss term: the loss over [live, zero] blocks is half the loss of the
+    live block alone (the mean runs over every cell), with the same
+    gradients halved."""
+    rng = np.random.default_rng(7)
+    layers = _rand_layers(rng)
+    r = rng.standard_normal((2, 16, 16, 16))
+    r[1] = 0.0
+    logd = rng.uniform(-1, 1, (2, 16, 16, 16))
+    x, s = L.features(r, logd, 1.0 / 64)
+    assert s[1] == 0.0 and s[0] > 0.0
+    assert not x[1, ..., 0].any() and x[1, ..., 1].any()
+    target = rng.standard_normal((2, 16, 16, 16)) * s[0]
+    loss2, g2 = L.loss_and_grads(layers, x, target, s, emulate=False)
+    loss1, g1 = L.loss_and_grads(layers, x[:1], target[:1], s[:1], emulate=False)
+    assert abs(loss2 - 0.5 * loss1) <= 1e-12 * loss1
+    for a, b in zip(g2, g1):
+        np.testing.assert_allclose(a, 0.5 * b, rtol=1e-10, atol=1e-14)
