@@ -454,3 +454,49 @@ def test_fac_solve_with_learned_smoother_on_uneven_patches(torch_gpu):
         x.close()
     h.close()
     net.close()
+
+
+def test_learned_phases_select_the_smoothing_calls(torch_gpu):
+    """sfac_solver_set_learned_phases: mask 0 applies the correction nowhere
+    (the red-black history, bit for bit); masks 1 (pre-smoothing), 2
+    (post-smoothing) and 3 (both) each change the history, and differ from
+    one another, unless the fallback undid the first learned cycle (then
+    the history is red-black's)."""
+    torch = torch_gpu
+    from samrai_amd import learned as SL
+    _say("learned phases 0..3 on a 2-level problem of 16^3 patches")
+    train = [SL.fac_training_pairs(torch, s, cycles=(1, 2)) for s in range(4)]
+    cat = {k: torch.cat([p[k] for p in train]).contiguous() for k in ("r", "logd", "e")}
+    net = SL.LearnedSmoother(torch, 128, seed=3)
+    net.train(cat["r"], cat["logd"], cat["e"], train[0]["h"], steps=600, batch=32, lr=2e-3)
+    P = SL.BlockLevelProblem(torch, 13, n=64, two_level=True)
+    s = P.solver(max_cycles=3, residual_tol=1e-300, relative_residual_tol=1e-300,
+                 num_pre_sweeps=2, num_post_sweeps=2, coarse_solver_choice="mg",
+                 coarse_solver_tolerance=1e-8, coarse_solver_max_iterations=10,
+                 prolongation_method="CONSERVATIVE_LINEAR_REFINE")
+    hist = {}
+    for mask in (None, 0, 1, 2, 3):
+        SL.set_learned_smoother(s, None if mask is None else net)
+        SL.set_learned_phases(s, 3 if mask is None else mask)
+        P.u.fill(0.0)
+        s.solve_system(P.u, P.f)
+        s.synchronize()
+        hist[mask] = (list(s.history()), SL.learned_rollbacks(s))
+        _say("mask %s: history %s, rollbacks %d" % (mask, ["%.4e" % v for v in hist[mask][0]],
+                                                    hist[mask][1]))
+    rb = hist[None][0]
+    assert hist[0][0] == rb
+    applied = []
+    for mask in (1, 2, 3):
+        h, undone = hist[mask]
+        if h == rb:
+            assert undone == 1
+        else:
+            applied.append(h)
+    for i in range(len(applied)):
+        for j in range(i + 1, len(applied)):
+            assert applied[i] != applied[j]
+    SL.set_learned_smoother(s, None)
+    s.close()
+    P.close()
+    net.close()
