@@ -17,8 +17,11 @@ import numpy as np
 _dp = C.POINTER(C.c_double)
 
 
-def _p(a):
+def _hp(a):
     return a.ctypes.data_as(_dp) if a is not None else C.cast(None, _dp)
+
+
+_p = _hp
 
 
 def _ri(v):
@@ -230,8 +233,23 @@ def _boxargs_ref(lo, hi):
 
 # --------------------------------------------------------------------- ref
 
-def run_ref(lib, case):
+def run_ref(lib, case, torch=None):
+    """The by-reference (SAMRAI_F77_FUNC) calling convention: the
+    flang-built reference with host arrays, or, with torch, the library's
+    link-compatible twins (include/samrai_f77.h) with the arrays on the
+    device and scalars and maxres on the host."""
     A = _copy_arrays(case)
+    dev = {}
+    if torch is not None:
+        for k, v in A.items():
+            dev[k] = torch.from_numpy(v).cuda()
+
+    def _p(a):
+        if torch is None or a is None:
+            return _hp(a)
+        key = next(k for k, v in A.items() if v is a)
+        return C.cast(C.c_void_p(dev[key].data_ptr()), _dp)
+
     dim = case["dim"]
     s = "3d" if dim == 3 else "2d"
     k = case["kernel"]
@@ -297,6 +315,11 @@ def run_ref(lib, case):
                            _ri(case["loc"]))
     else:
         raise ValueError(k)
+    if torch is not None:
+        torch.cuda.synchronize()
+        for o in case["outs"]:
+            if o != "maxres":
+                A[o] = dev[o].cpu().numpy()
     return {o: A[o] for o in case["outs"]}
 
 

@@ -17,6 +17,9 @@ def declared_symbols():
         for m in re.finditer(r"^\s*(?:const\s+)?[a-z_]+\s*\*?\s*(sfac_[a-z0-9_]+)\s*\(", text,
                              flags=re.M):
             names.add(m.group(1))
+        # link-compatible Fortran twins (samrai_f77.h): lower case, trailing underscore
+        for m in re.finditer(r"^void\s+([a-z0-9]+_)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
     return names
 
 
@@ -26,6 +29,11 @@ def test_header_declares_the_api():
                  "sfac_cartclinrefcelldoub", "sfac_ops_smooth", "sfac_settype3cells3d"):
         assert must in names
     assert len(names) >= 55
+    f77 = {n for n in names if n.endswith("_")}
+    for must in ("rbgswithfluxmaxvardcvarsf3d_", "compresconsca2d_", "ewingfixfluxcondc3d_",
+                 "settype1cells3d_", "compfluxvardc2d_"):
+        assert must in f77
+    assert len(f77) == 25
 
 
 def test_library_exports_every_declared_symbol():
