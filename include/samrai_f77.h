@@ -1,9 +1,13 @@
 /* samrai_f77.h -- link-compatible twins of the Fortran kernels that SAMRAI's
  * FAC operator and Robin boundary helper call through SAMRAI_F77_FUNC
  * (lower case, trailing underscore, every argument by address):
- * solv/CellPoissonFACOps.C:62-543 and solv/CartesianRobinBcHelper.C:32-73
- * (the C++ declarations there take some scalars as `const double&` /
- * `const int&`, which the C ABI passes as the pointers declared here).
+ * solv/CellPoissonFACOps.C:62-543, solv/CartesianRobinBcHelper.C:32-73 and
+ * the cell transfer operators' calls (geom/CartesianCellDoubleWeightedAverage.C,
+ * geom/CartesianCellDoubleConservativeLinearRefine.C,
+ * geom/CartesianCellDoubleLinearRefine.C, pdat/CellDoubleConstantRefine.C;
+ * box corners one by-address int per axis).  The C++ declarations there take
+ * some scalars as `const double&` / `const int&`, which the C ABI passes as
+ * the pointers declared here.
  *
  * Arrays are device pointers into ArrayData storage (the reference
  * allocates it through tbox::ResourceAllocator / Umpire,
@@ -152,6 +156,51 @@ void settype2cells3d_(double* data, const int* dil, const int* dih, const int* d
 void settype3cells3d_(double* data, const int* dil, const int* dih, const int* djl,
     const int* djh, const int* dkl, const int* dkh, const int* lower, const int* upper,
     const int* location);
+void cartwgtavgcelldoub2d_(const int* ifc0, const int* ifc1, const int* ilc0, const int* ilc1,
+    const int* filo0, const int* filo1, const int* fihi0, const int* fihi1, const int* cilo0,
+    const int* cilo1, const int* cihi0, const int* cihi1, const int* ratio, const double* dxf,
+    const double* dxc, const double* arrayf, double* arrayc);
+void cartwgtavgcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const int* ilc0,
+    const int* ilc1, const int* ilc2, const int* filo0, const int* filo1, const int* filo2,
+    const int* fihi0, const int* fihi1, const int* fihi2, const int* cilo0, const int* cilo1,
+    const int* cilo2, const int* cihi0, const int* cihi1, const int* cihi2, const int* ratio,
+    const double* dxf, const double* dxc, const double* arrayf, double* arrayc);
+void cartclinrefcelldoub2d_(const int* ifc0, const int* ifc1, const int* ilc0,
+    const int* ilc1, const int* iff0, const int* iff1, const int* ilf0, const int* ilf1,
+    const int* cilo0, const int* cilo1, const int* cihi0, const int* cihi1, const int* filo0,
+    const int* filo1, const int* fihi0, const int* fihi1, const int* ratio, const double* dxc,
+    const double* dxf, const double* arrayc, double* arrayf, double* diff0, double* slope0,
+    double* diff1, double* slope1);
+void cartclinrefcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2,
+    const int* ilc0, const int* ilc1, const int* ilc2, const int* iff0, const int* iff1,
+    const int* iff2, const int* ilf0, const int* ilf1, const int* ilf2, const int* cilo0,
+    const int* cilo1, const int* cilo2, const int* cihi0, const int* cihi1, const int* cihi2,
+    const int* filo0, const int* filo1, const int* filo2, const int* fihi0, const int* fihi1,
+    const int* fihi2, const int* ratio, const double* dxc, const double* dxf,
+    const double* arrayc, double* arrayf, double* diff0, double* slope0, double* diff1,
+    double* slope1, double* diff2, double* slope2);
+void cartlinrefcelldoub2d_(const int* ifc0, const int* ifc1, const int* ilc0, const int* ilc1,
+    const int* iff0, const int* iff1, const int* ilf0, const int* ilf1, const int* cilo0,
+    const int* cilo1, const int* cihi0, const int* cihi1, const int* filo0, const int* filo1,
+    const int* fihi0, const int* fihi1, const int* ratio, const double* dxc,
+    const double* dxf, const double* arrayc, double* arrayf);
+void cartlinrefcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const int* ilc0,
+    const int* ilc1, const int* ilc2, const int* iff0, const int* iff1, const int* iff2,
+    const int* ilf0, const int* ilf1, const int* ilf2, const int* cilo0, const int* cilo1,
+    const int* cilo2, const int* cihi0, const int* cihi1, const int* cihi2, const int* filo0,
+    const int* filo1, const int* filo2, const int* fihi0, const int* fihi1, const int* fihi2,
+    const int* ratio, const double* dxc, const double* dxf, const double* arrayc,
+    double* arrayf);
+void conrefcelldoub2d_(const int* ifc0, const int* ifc1, const int* ilc0, const int* ilc1,
+    const int* iff0, const int* iff1, const int* ilf0, const int* ilf1, const int* cilo0,
+    const int* cilo1, const int* cihi0, const int* cihi1, const int* filo0, const int* filo1,
+    const int* fihi0, const int* fihi1, const int* ratio, const double* arrayc, double* arrayf);
+void conrefcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const int* ilc0,
+    const int* ilc1, const int* ilc2, const int* iff0, const int* iff1, const int* iff2,
+    const int* ilf0, const int* ilf1, const int* ilf2, const int* cilo0, const int* cilo1,
+    const int* cilo2, const int* cihi0, const int* cihi1, const int* cihi2, const int* filo0,
+    const int* filo1, const int* filo2, const int* fihi0, const int* fihi1, const int* fihi2,
+    const int* ratio, const double* arrayc, double* arrayf);
 
 #ifdef __cplusplus
 }

@@ -1,7 +1,10 @@
 // Link-compatible twins of the Fortran kernels the reference calls through
 // SAMRAI_F77_FUNC (lower case, trailing underscore; every argument by
-// address): solv/CellPoissonFACOps.C:62-543 and
-// solv/CartesianRobinBcHelper.C:32-73.  A build of the reference that links
+// address): solv/CellPoissonFACOps.C:62-543, solv/CartesianRobinBcHelper.C:32-73
+// and the cell transfer operators (geom/CartesianCellDoubleWeightedAverage.C:37-70,
+// geom/CartesianCellDoubleConservativeLinearRefine.C:41-80,
+// geom/CartesianCellDoubleLinearRefine.C, pdat/CellDoubleConstantRefine.C:40-60).
+// A build of the reference that links
 // this library instead of its Fortran objects runs the same calls on the GPU
 // unchanged, provided its ArrayData storage is device memory (the reference
 // allocates it through tbox::ResourceAllocator / Umpire, pdat/ArrayData.C:81-112).
@@ -408,6 +411,157 @@ void settype3cells3d_(double* data, const int* dil, const int* dih, const int* d
   done(sfac_settype3cells3d(data, *dil, *dih, *djl, *djh, *dkl, *dkh, lower, upper, *location,
                             nullptr),
        "settype3cells3d");
+}
+
+// ------------------------------------------------- transfer operators
+// (per-axis box corners as separate by-address ints, in the call order of
+// geom/CartesianCellDoubleWeightedAverage.C:183-246,
+// geom/CartesianCellDoubleConservativeLinearRefine.C:263-378,
+// geom/CartesianCellDoubleLinearRefine.C:212-293 and
+// pdat/CellDoubleConstantRefine.C:154-176; the conservative refine's
+// difference and slope work arrays are not needed and are left untouched)
+
+void cartwgtavgcelldoub2d_(const int* ifc0, const int* ifc1, const int* ilc0, const int* ilc1,
+                           const int* filo0, const int* filo1, const int* fihi0,
+                           const int* fihi1, const int* cilo0, const int* cilo1,
+                           const int* cihi0, const int* cihi1, const int* ratio,
+                           const double* dxf, const double* dxc, const double* arrayf,
+                           double* arrayc) {
+  const int ifc[2] = {*ifc0, *ifc1}, ilc[2] = {*ilc0, *ilc1}, filo[2] = {*filo0, *filo1},
+            fihi[2] = {*fihi0, *fihi1}, cilo[2] = {*cilo0, *cilo1}, cihi[2] = {*cihi0, *cihi1};
+  done(sfac_cartwgtavgcelldoub(2, ifc, ilc, filo, fihi, cilo, cihi, ratio, dxf, dxc, arrayf,
+                               arrayc, nullptr),
+       "cartwgtavgcelldoub2d");
+}
+
+void cartwgtavgcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const int* ilc0,
+                           const int* ilc1, const int* ilc2, const int* filo0,
+                           const int* filo1, const int* filo2, const int* fihi0,
+                           const int* fihi1, const int* fihi2, const int* cilo0,
+                           const int* cilo1, const int* cilo2, const int* cihi0,
+                           const int* cihi1, const int* cihi2, const int* ratio,
+                           const double* dxf, const double* dxc, const double* arrayf,
+                           double* arrayc) {
+  const int ifc[3] = {*ifc0, *ifc1, *ifc2}, ilc[3] = {*ilc0, *ilc1, *ilc2},
+            filo[3] = {*filo0, *filo1, *filo2}, fihi[3] = {*fihi0, *fihi1, *fihi2},
+            cilo[3] = {*cilo0, *cilo1, *cilo2}, cihi[3] = {*cihi0, *cihi1, *cihi2};
+  done(sfac_cartwgtavgcelldoub(3, ifc, ilc, filo, fihi, cilo, cihi, ratio, dxf, dxc, arrayf,
+                               arrayc, nullptr),
+       "cartwgtavgcelldoub3d");
+}
+
+void cartclinrefcelldoub2d_(const int* ifc0, const int* ifc1, const int* ilc0, const int* ilc1,
+                            const int* iff0, const int* iff1, const int* ilf0, const int* ilf1,
+                            const int* cilo0, const int* cilo1, const int* cihi0,
+                            const int* cihi1, const int* filo0, const int* filo1,
+                            const int* fihi0, const int* fihi1, const int* ratio,
+                            const double* dxc, const double* dxf, const double* arrayc,
+                            double* arrayf, double* diff0, double* slope0, double* diff1,
+                            double* slope1) {
+  (void)diff0;
+  (void)slope0;
+  (void)diff1;
+  (void)slope1;
+  const int ifc[2] = {*ifc0, *ifc1}, ilc[2] = {*ilc0, *ilc1}, iff[2] = {*iff0, *iff1},
+            ilf[2] = {*ilf0, *ilf1}, cilo[2] = {*cilo0, *cilo1}, cihi[2] = {*cihi0, *cihi1},
+            filo[2] = {*filo0, *filo1}, fihi[2] = {*fihi0, *fihi1};
+  done(sfac_cartclinrefcelldoub(2, ifc, ilc, iff, ilf, cilo, cihi, filo, fihi, ratio, dxc, dxf,
+                                arrayc, arrayf, nullptr),
+       "cartclinrefcelldoub2d");
+}
+
+void cartclinrefcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const int* ilc0,
+                            const int* ilc1, const int* ilc2, const int* iff0, const int* iff1,
+                            const int* iff2, const int* ilf0, const int* ilf1, const int* ilf2,
+                            const int* cilo0, const int* cilo1, const int* cilo2,
+                            const int* cihi0, const int* cihi1, const int* cihi2,
+                            const int* filo0, const int* filo1, const int* filo2,
+                            const int* fihi0, const int* fihi1, const int* fihi2,
+                            const int* ratio, const double* dxc, const double* dxf,
+                            const double* arrayc, double* arrayf, double* diff0,
+                            double* slope0, double* diff1, double* slope1, double* diff2,
+                            double* slope2) {
+  (void)diff0;
+  (void)slope0;
+  (void)diff1;
+  (void)slope1;
+  (void)diff2;
+  (void)slope2;
+  const int ifc[3] = {*ifc0, *ifc1, *ifc2}, ilc[3] = {*ilc0, *ilc1, *ilc2},
+            iff[3] = {*iff0, *iff1, *iff2}, ilf[3] = {*ilf0, *ilf1, *ilf2},
+            cilo[3] = {*cilo0, *cilo1, *cilo2}, cihi[3] = {*cihi0, *cihi1, *cihi2},
+            filo[3] = {*filo0, *filo1, *filo2}, fihi[3] = {*fihi0, *fihi1, *fihi2};
+  done(sfac_cartclinrefcelldoub(3, ifc, ilc, iff, ilf, cilo, cihi, filo, fihi, ratio, dxc, dxf,
+                                arrayc, arrayf, nullptr),
+       "cartclinrefcelldoub3d");
+}
+
+void cartlinrefcelldoub2d_(const int* ifc0, const int* ifc1, const int* ilc0, const int* ilc1,
+                           const int* iff0, const int* iff1, const int* ilf0, const int* ilf1,
+                           const int* cilo0, const int* cilo1, const int* cihi0,
+                           const int* cihi1, const int* filo0, const int* filo1,
+                           const int* fihi0, const int* fihi1, const int* ratio,
+                           const double* dxc, const double* dxf, const double* arrayc,
+                           double* arrayf) {
+  const int ifc[2] = {*ifc0, *ifc1}, ilc[2] = {*ilc0, *ilc1}, iff[2] = {*iff0, *iff1},
+            ilf[2] = {*ilf0, *ilf1}, cilo[2] = {*cilo0, *cilo1}, cihi[2] = {*cihi0, *cihi1},
+            filo[2] = {*filo0, *filo1}, fihi[2] = {*fihi0, *fihi1};
+  done(sfac_cartlinrefcelldoub(2, ifc, ilc, iff, ilf, cilo, cihi, filo, fihi, ratio, dxc, dxf,
+                               arrayc, arrayf, nullptr),
+       "cartlinrefcelldoub2d");
+}
+
+void cartlinrefcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const int* ilc0,
+                           const int* ilc1, const int* ilc2, const int* iff0, const int* iff1,
+                           const int* iff2, const int* ilf0, const int* ilf1, const int* ilf2,
+                           const int* cilo0, const int* cilo1, const int* cilo2,
+                           const int* cihi0, const int* cihi1, const int* cihi2,
+                           const int* filo0, const int* filo1, const int* filo2,
+                           const int* fihi0, const int* fihi1, const int* fihi2,
+                           const int* ratio, const double* dxc, const double* dxf,
+                           const double* arrayc, double* arrayf) {
+  const int ifc[3] = {*ifc0, *ifc1, *ifc2}, ilc[3] = {*ilc0, *ilc1, *ilc2},
+            iff[3] = {*iff0, *iff1, *iff2}, ilf[3] = {*ilf0, *ilf1, *ilf2},
+            cilo[3] = {*cilo0, *cilo1, *cilo2}, cihi[3] = {*cihi0, *cihi1, *cihi2},
+            filo[3] = {*filo0, *filo1, *filo2}, fihi[3] = {*fihi0, *fihi1, *fihi2};
+  done(sfac_cartlinrefcelldoub(3, ifc, ilc, iff, ilf, cilo, cihi, filo, fihi, ratio, dxc, dxf,
+                               arrayc, arrayf, nullptr),
+       "cartlinrefcelldoub3d");
+}
+
+void conrefcelldoub2d_(const int* ifc0, const int* ifc1, const int* ilc0, const int* ilc1,
+                       const int* iff0, const int* iff1, const int* ilf0, const int* ilf1,
+                       const int* cilo0, const int* cilo1, const int* cihi0, const int* cihi1,
+                       const int* filo0, const int* filo1, const int* fihi0, const int* fihi1,
+                       const int* ratio, const double* arrayc, double* arrayf) {
+  (void)ifc0;
+  (void)ifc1;
+  (void)ilc0;
+  (void)ilc1;
+  const int iff[2] = {*iff0, *iff1}, ilf[2] = {*ilf0, *ilf1}, cilo[2] = {*cilo0, *cilo1},
+            cihi[2] = {*cihi0, *cihi1}, filo[2] = {*filo0, *filo1}, fihi[2] = {*fihi0, *fihi1};
+  done(sfac_conrefcelldoub(2, iff, ilf, cilo, cihi, filo, fihi, ratio, arrayc, arrayf, nullptr),
+       "conrefcelldoub2d");
+}
+
+void conrefcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const int* ilc0,
+                       const int* ilc1, const int* ilc2, const int* iff0, const int* iff1,
+                       const int* iff2, const int* ilf0, const int* ilf1, const int* ilf2,
+                       const int* cilo0, const int* cilo1, const int* cilo2, const int* cihi0,
+                       const int* cihi1, const int* cihi2, const int* filo0, const int* filo1,
+                       const int* filo2, const int* fihi0, const int* fihi1, const int* fihi2,
+                       const int* ratio, const double* arrayc, double* arrayf) {
+  (void)ifc0;
+  (void)ifc1;
+  (void)ifc2;
+  (void)ilc0;
+  (void)ilc1;
+  (void)ilc2;
+  const int iff[3] = {*iff0, *iff1, *iff2}, ilf[3] = {*ilf0, *ilf1, *ilf2},
+            cilo[3] = {*cilo0, *cilo1, *cilo2}, cihi[3] = {*cihi0, *cihi1, *cihi2},
+            filo[3] = {*filo0, *filo1, *filo2}, fihi[3] = {*fihi0, *fihi1, *fihi2};
+  done(sfac_conrefcelldoub(3, iff, ilf, cilo, cihi, filo, fihi, ratio, arrayc, arrayf, nullptr),
+       "conrefcelldoub3d");
 }
 
 }  // extern "C"

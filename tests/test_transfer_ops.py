@@ -353,3 +353,65 @@ def test_device_transfers_match_oracle(gpu_lib, torch_gpu, dim, ratio, clo, chi)
         b = outside(gpu, dim, axis, (clo, chi), (flo, fhi), (clo, chi), ratio, dxf, dxc, ff,
                     cshape)
         np.testing.assert_array_equal(a, b)
+
+
+def _by_axis(dim, *vecs):
+    """Box corners as the Fortran calls pass them: one by-address int per axis."""
+    out = []
+    for v in vecs:
+        out += [C.byref(C.c_int(int(v[d]))) for d in range(dim)]
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim,ratio,clo,chi", _random_cases()[:4])
+def test_fortran_link_twins_of_transfer_ops(gpu_lib, torch_gpu, dim, ratio, clo, chi):
+    """The link-compatible twins cartclinrefcelldoub{2,3}d_,
+    cartlinrefcelldoub{2,3}d_, cartwgtavgcelldoub{2,3}d_ and
+    conrefcelldoub{2,3}d_ (include/samrai_f77.h: the reference's call order,
+    box corners one by-address int per axis, device arrays) give the sfac_
+    entries' results bit for bit."""
+    torch = torch_gpu
+    gpu = Runner(torch, gpu_lib)
+    rng = np.random.default_rng(abs(hash((dim, ratio, clo, 77))) % 2**32)
+    dxc = tuple(float(x) for x in rng.uniform(0.05, 0.3, dim))
+    dxf = tuple(dxc[d] / ratio[d] for d in range(dim))
+    cg = (tuple(l - 1 for l in clo), tuple(h + 1 for h in chi))
+    flo = tuple(clo[d] * ratio[d] for d in range(dim))
+    fhi = tuple((chi[d] + 1) * ratio[d] - 1 for d in range(dim))
+    fg = (tuple(l - 1 for l in flo), tuple(h + 1 for h in fhi))
+    uc = rng.standard_normal(shape(*cg))
+    uf = rng.standard_normal(shape(*fg))
+    s = "%dd_" % dim
+
+    def dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()
+
+    def ptr(t):
+        return C.cast(C.c_void_p(t.data_ptr()), _dp)
+
+    def twin(name, ints, tail, inp, out_shape):
+        x, y = dev(inp), dev(np.full(out_shape, np.nan))
+        getattr(gpu_lib, name + s)(*ints, *tail, ptr(x), ptr(y),
+                                   *([C.c_void_p(None)] * (2 * dim if name.startswith("cartclin")
+                                                           else 0)))
+        torch.cuda.synchronize()
+        return y.cpu().numpy()
+
+    refine_box = _by_axis(dim, clo, chi, flo, fhi, cg[0], cg[1], fg[0], fg[1])
+    cases = [
+        (twin("cartclinrefcelldoub", refine_box, [iv(ratio), dv(dxc), dv(dxf)], uc, shape(*fg)),
+         clinref(gpu, dim, (clo, chi), (flo, fhi), cg, fg, ratio, dxc, dxf, uc)),
+        (twin("cartlinrefcelldoub", refine_box, [iv(ratio), dv(dxc), dv(dxf)], uc, shape(*fg)),
+         linref(gpu, dim, (clo, chi), (flo, fhi), cg, fg, ratio, dxc, dxf, uc)),
+        (twin("cartwgtavgcelldoub", _by_axis(dim, clo, chi, fg[0], fg[1], cg[0], cg[1]),
+              [iv(ratio), dv(dxf), dv(dxc)], uf, shape(*cg)),
+         wgtavg(gpu, dim, (clo, chi), fg, cg, ratio, dxf, dxc, uf)),
+        (twin("conrefcelldoub", _by_axis(dim, cg[0], cg[1], fg[0], fg[1], cg[0], cg[1], fg[0],
+                                         fg[1]), [iv(ratio)], uc, shape(*fg)),
+         conref(gpu, dim, fg, cg, fg, ratio, uc)),
+    ]
+    for a, b in cases:
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
+        np.testing.assert_array_equal(np.nan_to_num(a), np.nan_to_num(b))
+        assert not np.isnan(b).all()
