@@ -2,7 +2,8 @@
  * FAC operator and Robin boundary helper call through SAMRAI_F77_FUNC
  * (lower case, trailing underscore, every argument by address):
  * solv/CellPoissonFACOps.C:62-543, solv/CartesianRobinBcHelper.C:32-73 and
- * the cell transfer operators' calls (geom/CartesianCellDoubleWeightedAverage.C,
+ * the transfer operators' calls (geom/CartesianCellDoubleWeightedAverage.C,
+ * geom/CartesianOutersideDoubleWeightedAverage.C,
  * geom/CartesianCellDoubleConservativeLinearRefine.C,
  * geom/CartesianCellDoubleLinearRefine.C, pdat/CellDoubleConstantRefine.C;
  * box corners one by-address int per axis).  The C++ declarations there take
@@ -201,6 +202,32 @@ void conrefcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const 
     const int* cilo2, const int* cihi0, const int* cihi1, const int* cihi2, const int* filo0,
     const int* filo1, const int* filo2, const int* fihi0, const int* fihi1, const int* fihi2,
     const int* ratio, const double* arrayc, double* arrayf);
+void cartwgtavgoutsidedoub2d0_(const int* ifc0, const int* ifc1, const int* ilc0,
+    const int* ilc1, const int* filo0, const int* filo1, const int* fihi0, const int* fihi1,
+    const int* cilo0, const int* cilo1, const int* cihi0, const int* cihi1, const int* ratio,
+    const double* dxf, const double* dxc, const double* arrayf, double* arrayc);
+void cartwgtavgoutsidedoub2d1_(const int* ifc0, const int* ifc1, const int* ilc0,
+    const int* ilc1, const int* filo0, const int* filo1, const int* fihi0, const int* fihi1,
+    const int* cilo0, const int* cilo1, const int* cihi0, const int* cihi1, const int* ratio,
+    const double* dxf, const double* dxc, const double* arrayf, double* arrayc);
+void cartwgtavgoutsidedoub3d0_(const int* ifc0, const int* ifc1, const int* ifc2,
+    const int* ilc0, const int* ilc1, const int* ilc2, const int* filo0, const int* filo1,
+    const int* filo2, const int* fihi0, const int* fihi1, const int* fihi2, const int* cilo0,
+    const int* cilo1, const int* cilo2, const int* cihi0, const int* cihi1, const int* cihi2,
+    const int* ratio, const double* dxf, const double* dxc, const double* arrayf,
+    double* arrayc);
+void cartwgtavgoutsidedoub3d1_(const int* ifc0, const int* ifc1, const int* ifc2,
+    const int* ilc0, const int* ilc1, const int* ilc2, const int* filo0, const int* filo1,
+    const int* filo2, const int* fihi0, const int* fihi1, const int* fihi2, const int* cilo0,
+    const int* cilo1, const int* cilo2, const int* cihi0, const int* cihi1, const int* cihi2,
+    const int* ratio, const double* dxf, const double* dxc, const double* arrayf,
+    double* arrayc);
+void cartwgtavgoutsidedoub3d2_(const int* ifc0, const int* ifc1, const int* ifc2,
+    const int* ilc0, const int* ilc1, const int* ilc2, const int* filo0, const int* filo1,
+    const int* filo2, const int* fihi0, const int* fihi1, const int* fihi2, const int* cilo0,
+    const int* cilo1, const int* cilo2, const int* cihi0, const int* cihi1, const int* cihi2,
+    const int* ratio, const double* dxf, const double* dxc, const double* arrayf,
+    double* arrayc);
 
 #ifdef __cplusplus
 }

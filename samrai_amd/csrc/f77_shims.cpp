@@ -1,7 +1,8 @@
 // Link-compatible twins of the Fortran kernels the reference calls through
 // SAMRAI_F77_FUNC (lower case, trailing underscore; every argument by
 // address): solv/CellPoissonFACOps.C:62-543, solv/CartesianRobinBcHelper.C:32-73
-// and the cell transfer operators (geom/CartesianCellDoubleWeightedAverage.C:37-70,
+// and the transfer operators (geom/CartesianCellDoubleWeightedAverage.C:37-70,
+// geom/CartesianOutersideDoubleWeightedAverage.C:42-110,
 // geom/CartesianCellDoubleConservativeLinearRefine.C:41-80,
 // geom/CartesianCellDoubleLinearRefine.C, pdat/CellDoubleConstantRefine.C:40-60).
 // A build of the reference that links
@@ -564,4 +565,99 @@ void conrefcelldoub3d_(const int* ifc0, const int* ifc1, const int* ifc2, const 
        "conrefcelldoub3d");
 }
 
+// outerside flux average onto the coarse faces of axis 0, 1 (2-D) or 0, 1,
+// 2 (3-D), in the call order of
+// geom/CartesianOutersideDoubleWeightedAverage.C:242-400
+void cartwgtavgoutsidedoub2d0_(const int* ifc0, const int* ifc1, const int* ilc0,
+                               const int* ilc1, const int* filo0, const int* filo1,
+                               const int* fihi0, const int* fihi1, const int* cilo0,
+                               const int* cilo1, const int* cihi0, const int* cihi1,
+                               const int* ratio, const double* dxf, const double* dxc,
+                               const double* arrayf, double* arrayc);
+void cartwgtavgoutsidedoub2d1_(const int* ifc0, const int* ifc1, const int* ilc0,
+                               const int* ilc1, const int* filo0, const int* filo1,
+                               const int* fihi0, const int* fihi1, const int* cilo0,
+                               const int* cilo1, const int* cihi0, const int* cihi1,
+                               const int* ratio, const double* dxf, const double* dxc,
+                               const double* arrayf, double* arrayc);
+void cartwgtavgoutsidedoub3d0_(const int* ifc0, const int* ifc1, const int* ifc2,
+                               const int* ilc0, const int* ilc1, const int* ilc2,
+                               const int* filo0, const int* filo1, const int* filo2,
+                               const int* fihi0, const int* fihi1, const int* fihi2,
+                               const int* cilo0, const int* cilo1, const int* cilo2,
+                               const int* cihi0, const int* cihi1, const int* cihi2,
+                               const int* ratio, const double* dxf, const double* dxc,
+                               const double* arrayf, double* arrayc);
+void cartwgtavgoutsidedoub3d1_(const int* ifc0, const int* ifc1, const int* ifc2,
+                               const int* ilc0, const int* ilc1, const int* ilc2,
+                               const int* filo0, const int* filo1, const int* filo2,
+                               const int* fihi0, const int* fihi1, const int* fihi2,
+                               const int* cilo0, const int* cilo1, const int* cilo2,
+                               const int* cihi0, const int* cihi1, const int* cihi2,
+                               const int* ratio, const double* dxf, const double* dxc,
+                               const double* arrayf, double* arrayc);
+void cartwgtavgoutsidedoub3d2_(const int* ifc0, const int* ifc1, const int* ifc2,
+                               const int* ilc0, const int* ilc1, const int* ilc2,
+                               const int* filo0, const int* filo1, const int* filo2,
+                               const int* fihi0, const int* fihi1, const int* fihi2,
+                               const int* cilo0, const int* cilo1, const int* cilo2,
+                               const int* cihi0, const int* cihi1, const int* cihi2,
+                               const int* ratio, const double* dxf, const double* dxc,
+                               const double* arrayf, double* arrayc);
+
 }  // extern "C"
+
+namespace {
+
+void outside2(int axis, const int* ifc0, const int* ifc1, const int* ilc0, const int* ilc1,
+              const int* filo0, const int* filo1, const int* fihi0, const int* fihi1,
+              const int* cilo0, const int* cilo1, const int* cihi0, const int* cihi1,
+              const int* ratio, const double* dxf, const double* dxc, const double* arrayf,
+              double* arrayc, const char* name) {
+  const int ifc[2] = {*ifc0, *ifc1}, ilc[2] = {*ilc0, *ilc1}, filo[2] = {*filo0, *filo1},
+            fihi[2] = {*fihi0, *fihi1}, cilo[2] = {*cilo0, *cilo1}, cihi[2] = {*cihi0, *cihi1};
+  done(sfac_cartwgtavgoutsidedoub(2, axis, ifc, ilc, filo, fihi, cilo, cihi, ratio, dxf, dxc,
+                                  arrayf, arrayc, nullptr),
+       name);
+}
+
+void outside3(int axis, const int* ifc0, const int* ifc1, const int* ifc2, const int* ilc0,
+              const int* ilc1, const int* ilc2, const int* filo0, const int* filo1,
+              const int* filo2, const int* fihi0, const int* fihi1, const int* fihi2,
+              const int* cilo0, const int* cilo1, const int* cilo2, const int* cihi0,
+              const int* cihi1, const int* cihi2, const int* ratio, const double* dxf,
+              const double* dxc, const double* arrayf, double* arrayc, const char* name) {
+  const int ifc[3] = {*ifc0, *ifc1, *ifc2}, ilc[3] = {*ilc0, *ilc1, *ilc2},
+            filo[3] = {*filo0, *filo1, *filo2}, fihi[3] = {*fihi0, *fihi1, *fihi2},
+            cilo[3] = {*cilo0, *cilo1, *cilo2}, cihi[3] = {*cihi0, *cihi1, *cihi2};
+  done(sfac_cartwgtavgoutsidedoub(3, axis, ifc, ilc, filo, fihi, cilo, cihi, ratio, dxf, dxc,
+                                  arrayf, arrayc, nullptr),
+       name);
+}
+
+}  // namespace
+
+#define SFAC_OUTSIDE2(AX)                                                                     \
+  extern "C" void cartwgtavgoutsidedoub2d##AX##_(                                             \
+      const int* a0, const int* a1, const int* b0, const int* b1, const int* c0,              \
+      const int* c1, const int* d0, const int* d1, const int* e0, const int* e1,              \
+      const int* f0, const int* f1, const int* ratio, const double* dxf, const double* dxc,   \
+      const double* arrayf, double* arrayc) {                                                 \
+    outside2(AX, a0, a1, b0, b1, c0, c1, d0, d1, e0, e1, f0, f1, ratio, dxf, dxc, arrayf,     \
+             arrayc, "cartwgtavgoutsidedoub2d" #AX);                                          \
+  }
+#define SFAC_OUTSIDE3(AX)                                                                     \
+  extern "C" void cartwgtavgoutsidedoub3d##AX##_(                                             \
+      const int* a0, const int* a1, const int* a2, const int* b0, const int* b1,              \
+      const int* b2, const int* c0, const int* c1, const int* c2, const int* d0,              \
+      const int* d1, const int* d2, const int* e0, const int* e1, const int* e2,              \
+      const int* f0, const int* f1, const int* f2, const int* ratio, const double* dxf,       \
+      const double* dxc, const double* arrayf, double* arrayc) {                              \
+    outside3(AX, a0, a1, a2, b0, b1, b2, c0, c1, c2, d0, d1, d2, e0, e1, e2, f0, f1, f2,      \
+             ratio, dxf, dxc, arrayf, arrayc, "cartwgtavgoutsidedoub3d" #AX);                 \
+  }
+SFAC_OUTSIDE2(0)
+SFAC_OUTSIDE2(1)
+SFAC_OUTSIDE3(0)
+SFAC_OUTSIDE3(1)
+SFAC_OUTSIDE3(2)

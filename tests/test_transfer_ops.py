@@ -367,8 +367,9 @@ def _by_axis(dim, *vecs):
 @pytest.mark.parametrize("dim,ratio,clo,chi", _random_cases()[:4])
 def test_fortran_link_twins_of_transfer_ops(gpu_lib, torch_gpu, dim, ratio, clo, chi):
     """The link-compatible twins cartclinrefcelldoub{2,3}d_,
-    cartlinrefcelldoub{2,3}d_, cartwgtavgcelldoub{2,3}d_ and
-    conrefcelldoub{2,3}d_ (include/samrai_f77.h: the reference's call order,
+    cartlinrefcelldoub{2,3}d_, cartwgtavgcelldoub{2,3}d_, conrefcelldoub{2,3}d_
+    and cartwgtavgoutsidedoub{2d0,2d1,3d0,3d1,3d2}_ (include/samrai_f77.h: the
+    reference's call order,
     box corners one by-address int per axis, device arrays) give the sfac_
     entries' results bit for bit."""
     torch = torch_gpu
@@ -411,6 +412,24 @@ def test_fortran_link_twins_of_transfer_ops(gpu_lib, torch_gpu, dim, ratio, clo,
                                          fg[1]), [iv(ratio)], uc, shape(*fg)),
          conref(gpu, dim, fg, cg, fg, ratio, uc)),
     ]
+    # outerside flux average, one routine per face axis (cartwgtavgoutsidedoub<dim>d<axis>_)
+    for axis in range(dim):
+        tang = [d for d in range(dim) if d != axis]
+        if dim == 3:
+            i, o = tang
+            ff = rng.standard_normal((fhi[o] - flo[o] + 1, fhi[i] - flo[i] + 1))
+            cshape = (chi[o] - clo[o] + 1, chi[i] - clo[i] + 1)
+        else:
+            ff = rng.standard_normal(fhi[tang[0]] - flo[tang[0]] + 1)
+            cshape = (chi[tang[0]] - clo[tang[0]] + 1,)
+        x, y = dev(ff), dev(np.full(cshape, np.nan))
+        getattr(gpu_lib, "cartwgtavgoutsidedoub%dd%d_" % (dim, axis))(
+            *_by_axis(dim, clo, chi, flo, fhi, clo, chi), iv(ratio), dv(dxf), dv(dxc), ptr(x),
+            ptr(y))
+        torch.cuda.synchronize()
+        cases.append((y.cpu().numpy(),
+                      outside(gpu, dim, axis, (clo, chi), (flo, fhi), (clo, chi), ratio, dxf, dxc,
+                              ff, cshape)))
     for a, b in cases:
         np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
         np.testing.assert_array_equal(np.nan_to_num(a), np.nan_to_num(b))
