@@ -668,7 +668,18 @@ k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom 
   const int rows_per_iter = kThreads / lanes;
   if ((int)threadIdx.x >= rows_per_iter * lanes) return;
   const int m0 = (int)threadIdx.x % lanes;
-  for (int row = ch.start + (int)threadIdx.x / lanes; row < row_end; row += rows_per_iter) {
+  // conservative refine at ratio 2 on a box aligned in y: a unit is the
+  // pair of rows j (even) and j + 1 under one parent row, which share the
+  // parent's slopes; a chunk that starts on the second row of a pair takes
+  // that row alone first (lead), and the last unit may be a single row
+  const bool pairs = method == kProlongConservative && r == 2 && (it.lo[1] & 1) == 0 &&
+                     (ny & 1) == 0;
+  const int lead = pairs ? (ch.start & 1) : 0;
+  const int nrows = row_end - ch.start;
+  const int nunits = pairs ? lead + (nrows - lead + 1) / 2 : nrows;
+  for (int u = (int)threadIdx.x / lanes; u < nunits; u += rows_per_iter) {
+    const int row = pairs && u >= lead ? ch.start + lead + 2 * (u - lead) : ch.start + u;
+    const int nrow = pairs && u >= lead ? min(2, row_end - row) : 1;
     const int j = it.lo[1] + row % ny, k = DIM == 3 ? it.lo[2] + row / ny : 0;
     for (int I = P0 + m0; I < P0 + npx; I += lanes) {
       const int i0 = max(I * r, it.lo[0]), i1 = min(I * r + r - 1, it.hi[0]);
@@ -694,6 +705,18 @@ k_prolong_add(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom 
           const double v = DIM == 3 ? uc + s0 * d0 + s1 * d1 + s2 * d2 : uc + s0 * d0 + s1 * d1;
           const long o = vf.at(i, j, k);
           uf[o] = uf[o] + v;
+        }
+        if (nrow == 2) {
+          // row j + 1: the same parent, l1 + 1
+          const double d1b = ((double)(l1 + 1) + 0.5) * Fl.dx[1] - C.dx[1] * 0.5;
+          for (int i = i0; i <= i1; ++i) {
+            const int l0 = i - I * r;
+            const double d0 = ((double)l0 + 0.5) * Fl.dx[0] - C.dx[0] * 0.5;
+            const double v =
+                DIM == 3 ? uc + s0 * d0 + s1 * d1b + s2 * d2 : uc + s0 * d0 + s1 * d1b;
+            const long o = vf.at(i, j + 1, k);
+            uf[o] = uf[o] + v;
+          }
         }
       } else {
         for (int i = i0; i <= i1; ++i) {
