@@ -1595,7 +1595,8 @@ void CellPoissonFACOps::prolongErrorAndCorrect(const SAMRAIVectorReal& s, SAMRAI
     fail("CellPoissonFACOps: prolongation between vectors of different precision");
   if (d.data->elemBytes() != 8)
     sfac::launch_prolong_add_lp(
-        d.data->elemBytes(), Tf.prolong_from_coarser.launch(), Tc.geom, Tf.geom,
+        d.data->elemBytes(), Tf.prolong_from_coarser.launch(), Tf.prolong_rows.launch(), Tc.geom,
+        Tf.geom,
         reinterpret_cast<const void* const*>(s.data->table(dest_ln - 1)), s.data->ghost(),
         reinterpret_cast<void* const*>(d.data->table(dest_ln)), d.data->ghost(), method,
         d_stream);

@@ -41,9 +41,10 @@ void launch_rbgs_lp(int elem, const SmoothLaunch& w, const LevelGeom& L, void* c
 void launch_ewing_faces_lp(int elem, const GhostLaunch& w, const LevelGeom& L,
                            const void* const* u, int ug, const PoissonLPRaw& ps, const int* stop,
                            hipStream_t s);
-void launch_prolong_add_lp(int elem, const GhostLaunch& w, const LevelGeom& coarse,
-                           const LevelGeom& fine, const void* const* ce, int cg, void* const* fe,
-                           int fg, int method, hipStream_t s);
+// rows: the same items in whole-row chunks (used by the conservative method)
+void launch_prolong_add_lp(int elem, const GhostLaunch& w, const GhostLaunch& rows,
+                           const LevelGeom& coarse, const LevelGeom& fine, const void* const* ce,
+                           int cg, void* const* fe, int fg, int method, hipStream_t s);
 // y (FP64) += x (elem storage) on interior cells.
 void launch_axpy_lp(int elem, const GhostLaunch& w, const LevelGeom& L, double* const* y, int yg,
                     const void* const* x, int xg, hipStream_t s);

@@ -414,6 +414,68 @@ k_prolong_add_lp(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGe
   }
 }
 
+// e_f += conservative linear refine of e_c over whole rows of the item
+// box (the rows chunk table): a lane takes the fine cells of one coarse
+// parent along x and forms the parent's slopes once for them, for two fine
+// rows at a time when they share the parent row (ratio 2, box aligned in y);
+// each cell's value is refine_lp's conservative expression, in FP32.
+template <int DIM, class T>
+__global__ void __launch_bounds__(kThreads)
+k_prolong_add_lp_rows(const BoxItem* items, const Chunk* chunks, LevelGeom C, LevelGeom Fl,
+                      const T* const* ce, int cg, T* const* fe, int fg) {
+  const int ci = xcd_chunk();
+  const Chunk ch = chunks[ci];
+  if (ch.item < 0) return;
+  const BoxItem it = items[ch.item];
+  const View vc = vv(C.box[it.q], cg, DIM), vf = vv(Fl.box[it.p], fg, DIM);
+  const T* uc_ = ce[it.q];
+  T* uf = fe[it.p];
+  auto U = [&](int a, int b, int c) { return ld(uc_[vc.at(a, b, c)]); };
+  const int r = Fl.ratio;
+  const int nx = it.hi[0] - it.lo[0] + 1, ny = it.hi[1] - it.lo[1] + 1;
+  const int nz = DIM == 3 ? it.hi[2] - it.lo[2] + 1 : 1;
+  const int P0 = cdiv(it.lo[0], r), npx = cdiv(it.hi[0], r) - P0 + 1;
+  const int rpc = kRowCellsPerChunk / nx > 0 ? kRowCellsPerChunk / nx : 1;
+  const int row_end = min(ch.start + rpc, ny * nz);
+  const int lanes = npx < kThreads ? npx : kThreads;
+  const int rows_per_iter = kThreads / lanes;
+  if ((int)threadIdx.x >= rows_per_iter * lanes) return;
+  const int m0 = (int)threadIdx.x % lanes;
+  const bool pairs = r == 2 && (it.lo[1] & 1) == 0 && (ny & 1) == 0;
+  const int lead = pairs ? (ch.start & 1) : 0;
+  const int nrows = row_end - ch.start;
+  const int nunits = pairs ? lead + (nrows - lead + 1) / 2 : nrows;
+  const float cx = (float)C.dx[0], cy = (float)C.dx[1];
+  const float cz = DIM == 3 ? (float)C.dx[2] : 0.f;
+  for (int u = (int)threadIdx.x / lanes; u < nunits; u += rows_per_iter) {
+    const int row = pairs && u >= lead ? ch.start + lead + 2 * (u - lead) : ch.start + u;
+    const int nrow = pairs && u >= lead ? min(2, row_end - row) : 1;
+    const int j = it.lo[1] + row % ny, k = DIM == 3 ? it.lo[2] + row / ny : 0;
+    const int c1 = cdiv(j, r), c2 = DIM == 3 ? cdiv(k, r) : 0;
+    for (int I = P0 + m0; I < P0 + npx; I += lanes) {
+      const int i0 = max(I * r, it.lo[0]), i1 = min(I * r + r - 1, it.hi[0]);
+      const float uc = U(I, c1, c2);
+      const float s0 = clin_slope_f(uc - U(I - 1, c1, c2), U(I + 1, c1, c2) - uc, cx);
+      const float s1 = clin_slope_f(uc - U(I, c1 - 1, c2), U(I, c1 + 1, c2) - uc, cy);
+      float s2 = 0.f, d2 = 0.f;
+      if (DIM == 3) {
+        s2 = clin_slope_f(uc - U(I, c1, c2 - 1), U(I, c1, c2 + 1) - uc, cz);
+        d2 = ((float)(k - c2 * r) + 0.5f) * (float)Fl.dx[2] - cz * 0.5f;
+      }
+      for (int q = 0; q < nrow; ++q) {
+        const int jq = j + q;
+        const float d1 = ((float)(jq - c1 * r) + 0.5f) * (float)Fl.dx[1] - cy * 0.5f;
+        for (int i = i0; i <= i1; ++i) {
+          const float d0 = ((float)(i - I * r) + 0.5f) * (float)Fl.dx[0] - cx * 0.5f;
+          const float v = DIM == 3 ? uc + s0 * d0 + s1 * d1 + s2 * d2 : uc + s0 * d0 + s1 * d1;
+          const long o = vf.at(i, jq, k);
+          st(uf[o], ld(uf[o]) + v);
+        }
+      }
+    }
+  }
+}
+
 // y += x on interior cells, y FP64, x in T storage.
 template <int DIM, class T>
 __global__ void __launch_bounds__(kThreads)
@@ -557,10 +619,34 @@ void launch_ewing_faces_lp(int elem, const GhostLaunch& w, const LevelGeom& L,
   check_launch();
 }
 
-void launch_prolong_add_lp(int elem, const GhostLaunch& w, const LevelGeom& coarse,
-                           const LevelGeom& fine, const void* const* ce, int cg, void* const* fe,
-                           int fg, int method, hipStream_t s) {
+void launch_prolong_add_lp(int elem, const GhostLaunch& w, const GhostLaunch& rows,
+                           const LevelGeom& coarse, const LevelGeom& fine, const void* const* ce,
+                           int cg, void* const* fe, int fg, int method, hipStream_t s) {
   if (w.nchunks == 0) return;
+  if (method == kProlongConservative) {
+    const dim3 g(xcd_grid(rows.nchunks)), bl(kThreads);
+    if (elem == 4) {
+      auto c = reinterpret_cast<const float* const*>(ce);
+      auto f = reinterpret_cast<float* const*>(fe);
+      if (coarse.dim == 3)
+        hipLaunchKernelGGL((k_prolong_add_lp_rows<3, float>), g, bl, 0, s, rows.items,
+                           rows.chunks, coarse, fine, c, cg, f, fg);
+      else
+        hipLaunchKernelGGL((k_prolong_add_lp_rows<2, float>), g, bl, 0, s, rows.items,
+                           rows.chunks, coarse, fine, c, cg, f, fg);
+    } else {
+      auto c = reinterpret_cast<const bf16* const*>(ce);
+      auto f = reinterpret_cast<bf16* const*>(fe);
+      if (coarse.dim == 3)
+        hipLaunchKernelGGL((k_prolong_add_lp_rows<3, bf16>), g, bl, 0, s, rows.items,
+                           rows.chunks, coarse, fine, c, cg, f, fg);
+      else
+        hipLaunchKernelGGL((k_prolong_add_lp_rows<2, bf16>), g, bl, 0, s, rows.items,
+                           rows.chunks, coarse, fine, c, cg, f, fg);
+    }
+    check_launch();
+    return;
+  }
   const dim3 g(xcd_grid(w.nchunks)), bl(kThreads);
   if (elem == 4) {
     auto c = reinterpret_cast<const float* const*>(ce);
