@@ -5,7 +5,7 @@
 // It has no counterpart in the reference (CellPoissonFACOps only offers the
 // red-black Gauss-Seidel smoother, solv/CellPoissonFACOps.C:1338-1514); it
 // maps the residual of a block (plus log D) to an error correction, and the
-// red-black smoother stays the fallback and the judge of the FP64 residual.
+// red-black smoother stays the fallback, with the FP64 residual as the convergence test.
 //
 // Network: 2 input channels (r h^2 / s, log D) -> 16 -> 16 -> 1, ReLU after
 // the hidden layers, "same" zero padding at the block faces.  Weights are

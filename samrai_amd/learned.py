@@ -12,7 +12,7 @@ block faces (computed by FP64 red-black SOR on the device).
 
 There is no counterpart in the reference: SAMRAI's CellPoissonFACOps smooths
 with red-black Gauss-Seidel only (solv/CellPoissonFACOps.C:1338-1514); that
-smoother stays the fallback and the FP64 residual stays the judge.
+smoother stays the fallback and the FP64 residual stays the convergence test.
 Device memory comes from torch tensors; every computation runs in the HIP
 library (no CPU fallback).
 """
