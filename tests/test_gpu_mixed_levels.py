@@ -173,3 +173,82 @@ def test_mixed_precision_rejects_other_cycles_and_widths(gpu_lib):
     pr.s.set_algorithm_choice("pernice")
     with pytest.raises(fac.SolverError):
         pr.s.set_mixed_precision(16)
+
+
+def _slope32(dm, dp, dxc):
+    """The conservative refine's limited slope in FP32 (geom_m4cartopstuff.i:39-63)."""
+    f = np.float32
+    coef2 = f(0.5) * (dp + dm)
+    bound = f(2.0) * min(abs(dp), abs(dm))
+    if dm * dp > f(0.0):
+        return f(np.copysign(min(abs(coef2), bound), coef2)) / dxc
+    return f(0.0)
+
+
+@pytest.mark.parametrize("bits", [32, 16])
+def test_low_precision_prolongation_matches_fp32_emulation(gpu_lib, bits):
+    """prolongErrorAndCorrect on the FP32 / BF16 error vector: every fine
+    cell gains the conservative linear refine of its parent computed in FP32
+    (geom_m4cartrefineops3d.i:118-167, in that expression order) and is
+    stored back rounded, bit for bit against a float32 emulation from the
+    coarse values (ghosts as the operation filled them) and the fine values
+    before the call.  Ratio 2 and aligned boxes take the kernel's row-pair
+    path; ghost cells of the fine level are untouched."""
+    from samrai_amd import fac
+    case = hc.case_3d()
+    data = smooth_data(case, 4)
+    pr = hc.Pair(case, dict(PARAMS, max_cycles=1), data)
+    s = pr.s
+    s.set_mixed_precision(bits)
+    s.initialize_solver_state(pr.u, pr.f)
+    e = s.internal_vector("e")
+    rng = np.random.default_rng(40 + bits)
+    levels = case["levels"]
+    for ln in (0, 1):
+        for p, box in enumerate(levels[ln]):
+            e.put(ln, p, rng.standard_normal(e.size(ln, p)))
+    before = [e.get(1, p) for p in range(len(levels[1]))]
+    print("lp prolongation", bits, flush=True)
+    s.prolong_and_correct(e, 1)
+    s.synchronize()
+    coarse = [e.get(0, q) for q in range(len(levels[0]))]
+    rnd = (lambda x: np.float32(x)) if bits == 32 else \
+        (lambda x: np.float32(_bf16(np.array([x], np.float64))[0]))
+    f32 = np.float32
+    cdx = [f32(1.0 / 8)] * 3
+    fdx = [f32(1.0 / 16)] * 3
+
+    def U(c):
+        for q, (lo, hi) in enumerate(levels[0]):
+            if all(lo[d] <= c[d] <= hi[d] for d in range(3)):
+                break
+        else:
+            raise AssertionError(c)
+        lo, hi = levels[0][q]
+        shp = hc.cell_shape((lo, hi), 1, 3)
+        arr = coarse[q].reshape(shp, order="F")
+        return lambda a, b, cc: f32(arr[a - lo[0] + 1, b - lo[1] + 1, cc - lo[2] + 1])
+
+    checked = 0
+    for p, (lo, hi) in enumerate(levels[1]):
+        shp = hc.cell_shape((lo, hi), 1, 3)
+        got = e.get(1, p).reshape(shp, order="F")
+        old = before[p].reshape(shp, order="F")
+        exp = old.copy()
+        for k in range(lo[2], hi[2] + 1):
+            for j in range(lo[1], hi[1] + 1):
+                for i in range(lo[0], hi[0] + 1):
+                    c = (i // 2, j // 2, k // 2)
+                    V = U(c)
+                    uc = V(*c)
+                    s0 = _slope32(uc - V(c[0] - 1, c[1], c[2]), V(c[0] + 1, c[1], c[2]) - uc, cdx[0])
+                    s1 = _slope32(uc - V(c[0], c[1] - 1, c[2]), V(c[0], c[1] + 1, c[2]) - uc, cdx[1])
+                    s2 = _slope32(uc - V(c[0], c[1], c[2] - 1), V(c[0], c[1], c[2] + 1) - uc, cdx[2])
+                    d = [(f32(x - 2 * cx) + f32(0.5)) * fdx[a] - cdx[a] * f32(0.5)
+                         for a, (x, cx) in enumerate(zip((i, j, k), c))]
+                    v = uc + s0 * d[0] + s1 * d[1] + s2 * d[2]
+                    ii, jj, kk = i - lo[0] + 1, j - lo[1] + 1, k - lo[2] + 1
+                    exp[ii, jj, kk] = rnd(f32(old[ii, jj, kk]) + v)
+                    checked += 1
+        np.testing.assert_array_equal(got, exp)
+    assert checked == sum(np.prod([h - l + 1 for l, h in zip(*b)]) for b in levels[1])
